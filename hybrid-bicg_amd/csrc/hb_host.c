@@ -1,0 +1,577 @@
+/*
+ * hb_host.c -- C host side of the MI355X shifted BiCG drop-in: argument
+ * checks of the reference entry points, device plans (uploads, SpMV row
+ * binning, workspace), the iteration driver (direct launches or hipGraph
+ * chunks, one host check of a device "done" flag per chunk) and the report.
+ *
+ * The reference's shifted_lopbicg_* (src/solver_dynamic.c:3-463,
+ * src/solver_static.c:3-468, src/solver_normal.c:3-432) malloc their
+ * workspace per call and free it on return; so does the drop-in here.
+ */
+#define _POSIX_C_SOURCE 199309L
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "hb_launch.h"
+#include "hb_shared.h"
+#include "../../include/hbicg.h"
+
+/* ---- parameters and errors ----------------------------------------------- */
+static hbicg_params g_params;
+static int g_params_set = 0;
+static int g_errno = 0;
+
+void hbicg_default_params(hbicg_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->tol = EPS;
+    p->max_iter = MAX_ITER;
+    p->window = 0;
+    p->exact_order = 0;
+    p->graph = -1;
+    p->chunk = 16;
+    p->device = 0;
+    p->exit_on_error = 1;
+    p->upload_x = 1;
+}
+
+static const hbicg_params *params(void)
+{
+    if (!g_params_set) {
+        hbicg_default_params(&g_params);
+        g_params_set = 1;
+    }
+    return &g_params;
+}
+
+int hbicg_set_params(const hbicg_params *p)
+{
+    if (!p || !(p->tol > 0.0) || p->max_iter < 1 || p->window < 0 || p->window > HB_WMAX ||
+        p->chunk < 0)
+        return HBICG_E_ARG;
+    g_params = *p;
+    g_params_set = 1;
+    return 0;
+}
+
+int hbicg_get_params(hbicg_params *p)
+{
+    if (!p) return HBICG_E_ARG;
+    *p = *params();
+    return 0;
+}
+
+int hbicg_errno(void) { return g_errno; }
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+#define HIPCK(call)                                                                     \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "hbicg: %s failed: %s (%s:%d)\n", #call, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                \
+            rc = HBICG_E_HIP;                                                           \
+            goto fail;                                                                  \
+        }                                                                               \
+    } while (0)
+
+/* ---- plan ------------------------------------------------------------------ */
+struct hbicg_plan {
+    int n, M, W, cap, exact, graph, chunk, upload_x, device;
+    double tol;
+    unsigned nz;
+    hipStream_t stream;
+    int own_stream;
+    /* matrix */
+    double *val;
+    unsigned *col, *ptr, *blk;
+    int own_matrix, nblk;
+    /* workspace */
+    double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
+    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *part;
+    int *stop, *stop_iter, *act_len, *flist, *switch_k, *switch_to;
+    hb_state *st;
+    int *flag_host;      /* pinned: {done, k} */
+    hipEvent_t ev[2], t0, t1;
+    /* graph of one chunk, captured for a given x_set pointer */
+    hipGraphExec_t gexec;
+    hipGraph_t graph_obj;
+    double *graph_x;
+    hb_dev dev;
+    /* last report */
+    hbicg_report rep;
+    int *h_stop_iter, *h_switch_k, *h_switch_to;
+};
+
+static int build_blocks(const unsigned *ptr, int n, unsigned **out, int *nblk)
+{
+    unsigned *blk = (unsigned *)malloc(((size_t)n + 1) * sizeof(unsigned));
+    if (!blk) return HBICG_E_NOMEM;
+    int b = 0, r = 0;
+    while (r < n) {
+        blk[b++] = (unsigned)r;
+        const unsigned len = ptr[r + 1] - ptr[r];
+        if (len > HB_NNZ_CAP) { ++r; continue; }
+        unsigned nnz = 0;
+        const int start = r;
+        while (r < n && r - start < HB_ROWS_CAP) {
+            const unsigned l = ptr[r + 1] - ptr[r];
+            if (nnz + l > HB_NNZ_CAP) break;
+            nnz += l;
+            ++r;
+        }
+    }
+    blk[b] = (unsigned)n;
+    *out = blk;
+    *nblk = b;
+    return 0;
+}
+
+static void *dalloc(size_t bytes, int *rc)
+{
+    void *p = NULL;
+    if (bytes == 0) bytes = 8;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        *rc = HBICG_E_NOMEM;
+        return NULL;
+    }
+    return p;
+}
+
+void hbicg_plan_destroy(hbicg_plan *pl)
+{
+    if (!pl) return;
+    hipSetDevice(pl->device);
+    if (pl->stream) hipStreamSynchronize(pl->stream);
+    if (pl->gexec) hipGraphExecDestroy(pl->gexec);
+    if (pl->graph_obj) hipGraphDestroy(pl->graph_obj);
+    if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
+    void *bufs[] = {pl->blk, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
+                    pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
+                    pl->coef, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist,
+                    pl->switch_k, pl->switch_to, pl->st};
+    for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
+        if (bufs[i]) hipFree(bufs[i]);
+    if (pl->flag_host) hipHostFree(pl->flag_host);
+    for (int i = 0; i < 2; ++i) if (pl->ev[i]) hipEventDestroy(pl->ev[i]);
+    if (pl->t0) hipEventDestroy(pl->t0);
+    if (pl->t1) hipEventDestroy(pl->t1);
+    if (pl->own_stream && pl->stream) hipStreamDestroy(pl->stream);
+    free(pl->h_stop_iter); free(pl->h_switch_k); free(pl->h_switch_to);
+    free(pl);
+}
+
+static int window_for(const hbicg_params *p, int variant)
+{
+    if (p->window > 0) return p->window;
+    return variant == 0 ? 1 : 8;   /* variant 0: normal */
+}
+
+static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
+                                 const hbicg_params *p, void *stream, int variant)
+{
+    int rc = 0;
+    unsigned *hptr = NULL, *hblk = NULL;
+    hbicg_plan *pl = NULL;
+    g_errno = 0;
+    if (!A || !A->ptr || !p || M <= 0 || A->n == 0) { g_errno = HBICG_E_ARG; return NULL; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_errno = HBICG_E_NODEVICE; return NULL; }
+    pl = (hbicg_plan *)calloc(1, sizeof(*pl));
+    if (!pl) { g_errno = HBICG_E_NOMEM; return NULL; }
+    pl->device = p->device;
+    HIPCK(hipSetDevice(p->device));
+    const int n = (int)A->n;
+    pl->n = n;
+    pl->M = M;
+    pl->nz = A->nz;
+    pl->W = window_for(p, variant);
+    pl->cap = p->max_iter + 1;
+    pl->tol = p->tol;
+    pl->exact = p->exact_order ? 1 : 0;
+    pl->graph = p->graph < 0 ? (variant == 2) : (p->graph ? 1 : 0);
+    pl->chunk = p->chunk > 0 ? p->chunk : 16;
+    pl->upload_x = p->upload_x;
+    if (stream) {
+        pl->stream = (hipStream_t)stream;
+    } else {
+        HIPCK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+        pl->own_stream = 1;
+    }
+
+    /* matrix and row binning (needs the row pointers on the host) */
+    if (a_on_device) {
+        pl->val = (double *)A->val; pl->col = (unsigned *)A->col; pl->ptr = (unsigned *)A->ptr;
+        hptr = (unsigned *)malloc(((size_t)n + 1) * sizeof(unsigned));
+        if (!hptr) { rc = HBICG_E_NOMEM; goto fail; }
+        HIPCK(hipMemcpy(hptr, A->ptr, ((size_t)n + 1) * sizeof(unsigned), hipMemcpyDeviceToHost));
+    } else {
+        pl->own_matrix = 1;
+        hptr = (unsigned *)A->ptr;
+        if (hptr[n] != A->nz) { rc = HBICG_E_ARG; goto fail; }
+        pl->val = (double *)dalloc((size_t)A->nz * 8, &rc);
+        pl->col = (unsigned *)dalloc((size_t)A->nz * 4, &rc);
+        pl->ptr = (unsigned *)dalloc(((size_t)n + 1) * 4, &rc);
+        if (rc) goto fail;
+        HIPCK(hipMemcpy(pl->val, A->val, (size_t)A->nz * 8, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(pl->col, A->col, (size_t)A->nz * 4, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(pl->ptr, A->ptr, ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+    }
+    /* validate the row pointers once on the host: every SpMV workgroup
+     * indexes val/col inside [0, nz) only through them */
+    for (int i = 0; i < n; ++i)
+        if (hptr[i + 1] < hptr[i]) { rc = HBICG_E_ARG; goto fail; }
+    if (hptr[0] != 0 || hptr[n] != A->nz) { rc = HBICG_E_ARG; goto fail; }
+    if ((rc = build_blocks(hptr, n, &hblk, &pl->nblk)) != 0) goto fail;
+    pl->blk = (unsigned *)dalloc(((size_t)pl->nblk + 1) * 4, &rc);
+    if (rc) goto fail;
+    HIPCK(hipMemcpy(pl->blk, hblk, ((size_t)pl->nblk + 1) * 4, hipMemcpyHostToDevice));
+
+    /* workspace */
+    const size_t nb = (size_t)n * 8;
+    const int W = pl->W;
+    int vec_grid = (n + HB_BLOCK - 1) / HB_BLOCK;
+    if (vec_grid > 2048) vec_grid = 2048;
+    const int maxparts = pl->nblk > vec_grid ? pl->nblk : vec_grid;
+    const int gx = (n + HB_BLOCK - 1) / HB_BLOCK;
+    int per = 4096 / gx;
+    if (per < 1) per = 1;
+    int G = (M + per - 1) / per;
+    if (G < 1) G = 1;
+    const int gy = (M + G - 1) / G;
+    const size_t cap = (size_t)pl->cap;
+    pl->rhat = (double *)dalloc(nb, &rc);
+    pl->rring = (double *)dalloc(nb * (W + 1), &rc);
+    pl->qring = (double *)dalloc(nb * W, &rc);
+    pl->s = (double *)dalloc(nb, &rc);
+    pl->y = (double *)dalloc(nb, &rc);
+    pl->b_buf = (double *)dalloc(nb, &rc);
+    pl->p_set = (double *)dalloc(nb * (size_t)M, &rc);
+    pl->sigma = (double *)dalloc((size_t)M * 8, &rc);
+    pl->aa = (double *)dalloc(cap * 8, &rc);
+    pl->ba = (double *)dalloc(cap * 8, &rc);
+    pl->wa = (double *)dalloc(cap * 8, &rc);
+    pl->pi = (double *)dalloc(cap * (size_t)M * 8, &rc);
+    pl->eta = (double *)dalloc((size_t)M * 8, &rc);
+    pl->zeta = (double *)dalloc((size_t)M * 8, &rc);
+    pl->coef = (double *)dalloc((size_t)W * M * 6 * 8, &rc);
+    pl->part = (double *)dalloc((size_t)maxparts * 2 * 8, &rc);
+    pl->stop = (int *)dalloc((size_t)M * 4, &rc);
+    pl->stop_iter = (int *)dalloc((size_t)M * 4, &rc);
+    pl->act_len = (int *)dalloc((size_t)M * 4, &rc);
+    pl->flist = (int *)dalloc((size_t)M * 4, &rc);
+    pl->switch_k = (int *)dalloc((size_t)M * 4, &rc);
+    pl->switch_to = (int *)dalloc((size_t)M * 4, &rc);
+    pl->st = (hb_state *)dalloc(sizeof(hb_state), &rc);
+    if (rc) goto fail;
+    HIPCK(hipHostMalloc((void **)&pl->flag_host, 2 * sizeof(int), hipHostMallocDefault));
+    HIPCK(hipEventCreateWithFlags(&pl->ev[0], hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&pl->ev[1], hipEventDisableTiming));
+    HIPCK(hipEventCreate(&pl->t0));
+    HIPCK(hipEventCreate(&pl->t1));
+    pl->h_stop_iter = (int *)calloc((size_t)M, sizeof(int));
+    pl->h_switch_k = (int *)calloc((size_t)M, sizeof(int));
+    pl->h_switch_to = (int *)calloc((size_t)M, sizeof(int));
+    if (!pl->h_stop_iter || !pl->h_switch_k || !pl->h_switch_to) { rc = HBICG_E_NOMEM; goto fail; }
+
+    hb_dev *d = &pl->dev;
+    memset(d, 0, sizeof(*d));
+    d->n = n; d->M = M; d->W = W; d->cap = pl->cap; d->nblk = pl->nblk;
+    d->exact = pl->exact;
+    d->vec_grid = vec_grid;
+    d->shift_gx = gx; d->shift_gy = gy; d->shift_group = G;
+    d->tol = pl->tol;
+    d->val = pl->val; d->col = pl->col; d->ptr = pl->ptr; d->blk = pl->blk;
+    d->sigma = pl->sigma; d->rhat = pl->rhat; d->rring = pl->rring; d->qring = pl->qring;
+    d->s = pl->s; d->y = pl->y; d->p_set = pl->p_set; d->x_set = NULL;
+    d->aa = pl->aa; d->ba = pl->ba; d->wa = pl->wa; d->pi = pl->pi; d->eta = pl->eta;
+    d->zeta = pl->zeta; d->coef = pl->coef; d->stop = pl->stop; d->stop_iter = pl->stop_iter;
+    d->act_len = pl->act_len; d->flist = pl->flist; d->switch_k = pl->switch_k;
+    d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st;
+
+    if (a_on_device) free(hptr);
+    free(hblk);
+    return pl;
+fail:
+    if (a_on_device) free(hptr);
+    free(hblk);
+    hbicg_plan_destroy(pl);
+    g_errno = rc ? rc : HBICG_E_HIP;
+    return NULL;
+}
+
+hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,
+                              const hbicg_params *p, void *stream)
+{
+    return plan_create_v(A, a_on_device, M, p ? p : params(), stream, 1);
+}
+
+/* Enqueues one chunk of iterations followed by the done-flag publication. */
+static int enqueue_chunk(hbicg_plan *pl)
+{
+    for (int c = 0; c < pl->chunk; ++c) {
+        int e = hb_k_iteration(&pl->dev, pl->stream);
+        if (e) return e;
+    }
+    return hb_k_publish(&pl->dev, pl->flag_host, pl->stream);
+}
+
+static int capture_graph(hbicg_plan *pl)
+{
+    int rc = 0;
+    if (pl->gexec && pl->graph_x == pl->dev.x_set) return 0;
+    if (pl->gexec) { hipGraphExecDestroy(pl->gexec); pl->gexec = NULL; }
+    if (pl->graph_obj) { hipGraphDestroy(pl->graph_obj); pl->graph_obj = NULL; }
+    HIPCK(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+    if (enqueue_chunk(pl) != 0) {
+        hipGraph_t g;
+        hipStreamEndCapture(pl->stream, &g);
+        if (g) hipGraphDestroy(g);
+        return HBICG_E_HIP;
+    }
+    HIPCK(hipStreamEndCapture(pl->stream, &pl->graph_obj));
+    HIPCK(hipGraphInstantiate(&pl->gexec, pl->graph_obj, NULL, NULL, 0));
+    pl->graph_x = pl->dev.x_set;
+    return 0;
+fail:
+    return rc;
+}
+
+static int run_loop(hbicg_plan *pl)
+{
+    int rc = 0;
+    if (pl->graph && (rc = capture_graph(pl)) != 0) return rc;
+    pl->flag_host[0] = 0;
+    pl->flag_host[1] = 0;
+    const int max_chunks = pl->cap / pl->chunk + 3;
+    int launched = 0, prev = -1;
+    for (;;) {
+        if (pl->graph) {
+            HIPCK(hipGraphLaunch(pl->gexec, pl->stream));
+        } else if (enqueue_chunk(pl) != 0) {
+            return HBICG_E_HIP;
+        }
+        const int cur = launched & 1;
+        HIPCK(hipEventRecord(pl->ev[cur], pl->stream));
+        ++launched;
+        if (prev >= 0) {
+            HIPCK(hipEventSynchronize(pl->ev[prev]));
+            if (__atomic_load_n(&pl->flag_host[0], __ATOMIC_ACQUIRE)) break;
+        }
+        prev = cur;
+        if (launched > max_chunks) break;   /* every kernel returns at once past the cap */
+    }
+    HIPCK(hipStreamSynchronize(pl->stream));
+    return 0;
+fail:
+    return rc;
+}
+
+int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int seed,
+                     double *x_set, double *r_out, int on_device, hbicg_report *rep)
+{
+    int rc = 0;
+    if (!pl || !b || !sigma || !x_set || seed < 0 || seed >= pl->M) return HBICG_E_ARG;
+    const double t_start = now_ms();
+    HIPCK(hipSetDevice(pl->device));
+    const size_t nb = (size_t)pl->n * 8, xb = nb * (size_t)pl->M;
+    double *dx = x_set;
+    const double *db = b;
+    if (!on_device) {
+        if (!pl->x_host_buf) {
+            pl->x_host_buf = (double *)dalloc(xb, &rc);
+            if (rc) return rc;
+        }
+        dx = pl->x_host_buf;
+        if (pl->upload_x) HIPCK(hipMemcpyAsync(dx, x_set, xb, hipMemcpyHostToDevice, pl->stream));
+        else HIPCK(hipMemsetAsync(dx, 0, xb, pl->stream));
+        HIPCK(hipMemcpyAsync(pl->b_buf, b, nb, hipMemcpyHostToDevice, pl->stream));
+        db = pl->b_buf;
+    }
+    pl->dev.x_set = dx;
+    HIPCK(hipMemcpyAsync(pl->sigma, sigma, (size_t)pl->M * 8, hipMemcpyHostToDevice, pl->stream));
+    HIPCK(hipMemsetAsync(pl->st, 0, sizeof(hb_state), pl->stream));
+    HIPCK(hipEventRecord(pl->t0, pl->stream));
+    if (hb_k_init(&pl->dev, db, seed, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    if ((rc = run_loop(pl)) != 0) goto fail;
+    if (hb_k_finalize(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    HIPCK(hipEventRecord(pl->t1, pl->stream));
+
+    hb_state st;
+    HIPCK(hipMemcpyAsync(&st, pl->st, sizeof(st), hipMemcpyDeviceToHost, pl->stream));
+    HIPCK(hipStreamSynchronize(pl->stream));
+    const double *r_dev = pl->rring + (size_t)(st.kprev % (pl->W + 1)) * pl->n;
+    if (r_out) {
+        if (on_device) HIPCK(hipMemcpyAsync(r_out, r_dev, nb, hipMemcpyDeviceToDevice, pl->stream));
+        else HIPCK(hipMemcpyAsync(r_out, r_dev, nb, hipMemcpyDeviceToHost, pl->stream));
+    }
+    if (!on_device) HIPCK(hipMemcpyAsync(x_set, dx, xb, hipMemcpyDeviceToHost, pl->stream));
+    HIPCK(hipMemcpyAsync(pl->h_stop_iter, pl->stop_iter, (size_t)pl->M * 4, hipMemcpyDeviceToHost, pl->stream));
+    HIPCK(hipMemcpyAsync(pl->h_switch_k, pl->switch_k, (size_t)pl->M * 4, hipMemcpyDeviceToHost, pl->stream));
+    HIPCK(hipMemcpyAsync(pl->h_switch_to, pl->switch_to, (size_t)pl->M * 4, hipMemcpyDeviceToHost, pl->stream));
+    HIPCK(hipStreamSynchronize(pl->stream));
+    float ms = 0.0f;
+    HIPCK(hipEventElapsedTime(&ms, pl->t0, pl->t1));
+
+    hbicg_report *R = &pl->rep;
+    memset(R, 0, sizeof(*R));
+    R->k = st.k;
+    R->stop_count = st.stop_count;
+    R->n_switch = st.n_switch;
+    R->seed_final = st.seed;
+    R->status = st.status;
+    R->M = pl->M;
+    R->n = pl->n;
+    R->dot_r = st.dot_r;
+    R->dot_zero = st.dot_zero;
+    R->rTr = st.rTr;
+    R->t_solve_ms = ms;
+    R->t_total_ms = now_ms() - t_start;
+    if (rep) *rep = *R;
+    return st.k;
+fail:
+    hipStreamSynchronize(pl->stream);
+    return rc;
+}
+
+int hbicg_plan_report(hbicg_plan *pl, hbicg_report *rep, int *stop_iter,
+                      int *switch_k, int *switch_to, int max_switch)
+{
+    if (!pl) return HBICG_E_ARG;
+    if (rep) *rep = pl->rep;
+    if (stop_iter) memcpy(stop_iter, pl->h_stop_iter, (size_t)pl->M * sizeof(int));
+    const int ns = pl->rep.n_switch < max_switch ? pl->rep.n_switch : max_switch;
+    if (switch_k) memcpy(switch_k, pl->h_switch_k, (size_t)(ns > 0 ? ns : 0) * sizeof(int));
+    if (switch_to) memcpy(switch_to, pl->h_switch_to, (size_t)(ns > 0 ? ns : 0) * sizeof(int));
+    return pl->rep.n_switch;
+}
+
+int hbicg_plan_true_residuals(hbicg_plan *pl, const double *b, const double *sigma,
+                              const double *x_set, int on_device, double *out)
+{
+    int rc = 0;
+    double *part = NULL, *dout = NULL;
+    if (!pl || !b || !sigma || !x_set || !out) return HBICG_E_ARG;
+    HIPCK(hipSetDevice(pl->device));
+    const size_t nb = (size_t)pl->n * 8, xb = nb * (size_t)pl->M;
+    const double *dx = x_set, *db = b;
+    if (!on_device) {
+        if (!pl->x_host_buf) {
+            pl->x_host_buf = (double *)dalloc(xb, &rc);
+            if (rc) return rc;
+        }
+        HIPCK(hipMemcpyAsync(pl->x_host_buf, x_set, xb, hipMemcpyHostToDevice, pl->stream));
+        HIPCK(hipMemcpyAsync(pl->b_buf, b, nb, hipMemcpyHostToDevice, pl->stream));
+        dx = pl->x_host_buf;
+        db = pl->b_buf;
+    }
+    HIPCK(hipMemcpyAsync(pl->sigma, sigma, (size_t)pl->M * 8, hipMemcpyHostToDevice, pl->stream));
+    const int nblk = (pl->n + HB_BLOCK - 1) / HB_BLOCK;
+    part = (double *)dalloc((size_t)nblk * pl->M * 2 * 8, &rc);
+    dout = (double *)dalloc((size_t)pl->M * 8, &rc);
+    if (rc) goto fail;
+    hb_dev d = pl->dev;
+    d.x_set = (double *)dx;
+    if (hb_k_true_residual(&d, db, part, dout, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    HIPCK(hipMemcpyAsync(out, dout, (size_t)pl->M * 8, hipMemcpyDeviceToHost, pl->stream));
+    HIPCK(hipStreamSynchronize(pl->stream));
+fail:
+    if (part) hipFree(part);
+    if (dout) hipFree(dout);
+    return rc;
+}
+
+int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift)
+{
+    if (!pl || !x || !y) return HBICG_E_ARG;
+    if (hipSetDevice(pl->device) != hipSuccess) return HBICG_E_HIP;
+    if (hb_k_spmv(&pl->dev, x, y, shift, pl->stream) != 0) return HBICG_E_HIP;
+    if (hipStreamSynchronize(pl->stream) != hipSuccess) return HBICG_E_HIP;
+    return 0;
+}
+
+/* ---- reference entry points ------------------------------------------------ */
+static hbicg_report g_last;
+static int *g_stop_iter = NULL, *g_switch_k = NULL, *g_switch_to = NULL, g_last_M = 0;
+
+int hbicg_last_report(hbicg_report *rep, int *stop_iter, int M,
+                      int *switch_k, int *switch_to, int max_switch)
+{
+    if (rep) *rep = g_last;
+    if (stop_iter && g_stop_iter) memcpy(stop_iter, g_stop_iter, (size_t)(M < g_last_M ? M : g_last_M) * sizeof(int));
+    const int ns = g_last.n_switch < max_switch ? g_last.n_switch : max_switch;
+    if (ns > 0 && switch_k && g_switch_k) memcpy(switch_k, g_switch_k, (size_t)ns * sizeof(int));
+    if (ns > 0 && switch_to && g_switch_to) memcpy(switch_to, g_switch_to, (size_t)ns * sizeof(int));
+    return g_last.n_switch;
+}
+
+/* variant: 0 normal, 1 static, 2 dynamic */
+static int dropin(CSR_Matrix *A_diag, CSR_Matrix *A_offd, INFO_Matrix *A_info, double *x_set,
+                  double *r, double *sigma, int M, int seed, int variant)
+{
+    const hbicg_params *p = params();
+    if (!A_diag || !A_info || !x_set || !r || !sigma) return HBICG_E_ARG;
+    /* src/solver_dynamic.c:11-14 */
+    if (A_info->cols != A_info->rows) {
+        if (p->exit_on_error) {
+            printf("Error: matrix is not square.\n");
+            exit(1);
+        }
+        return HBICG_E_NOTSQUARE;
+    }
+    if (A_offd && A_offd->nz != 0) return HBICG_E_OFFD;
+    if (A_diag->rows != A_info->rows || A_diag->cols != A_info->cols) return HBICG_E_OFFD;
+    if (M <= 0 || seed < 0 || seed >= M) return HBICG_E_ARG;
+    if (!A_diag->ptr || (A_diag->nz > 0 && (!A_diag->val || !A_diag->col))) return HBICG_E_ARG;
+    const double t0 = now_ms();
+    hbicg_csr A = {A_diag->val, A_diag->col, A_diag->ptr, A_diag->rows, A_diag->nz};
+    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant);
+    if (!pl) return g_errno ? g_errno : HBICG_E_HIP;
+    const double t1 = now_ms();
+    const int k = hbicg_plan_solve(pl, r, sigma, seed, x_set, r, 0, NULL);
+    if (k > 0) {
+        g_last = pl->rep;
+        g_last.t_setup_ms = t1 - t0;
+        g_last.t_total_ms = now_ms() - t0;
+        if (g_last_M < M) {
+            free(g_stop_iter); free(g_switch_k); free(g_switch_to);
+            g_stop_iter = (int *)calloc((size_t)M, sizeof(int));
+            g_switch_k = (int *)calloc((size_t)M, sizeof(int));
+            g_switch_to = (int *)calloc((size_t)M, sizeof(int));
+        }
+        g_last_M = M;
+        if (g_stop_iter) memcpy(g_stop_iter, pl->h_stop_iter, (size_t)M * sizeof(int));
+        if (g_switch_k) memcpy(g_switch_k, pl->h_switch_k, (size_t)M * sizeof(int));
+        if (g_switch_to) memcpy(g_switch_to, pl->h_switch_to, (size_t)M * sizeof(int));
+    }
+    hbicg_plan_destroy(pl);
+    return k;
+}
+
+int shifted_lopbicg_dynamic(CSR_Matrix *A_loc_diag, CSR_Matrix *A_loc_offd, INFO_Matrix *A_info,
+                            double *x_loc_set, double *r_loc, double *sigma, int sigma_len, int seed)
+{
+    return dropin(A_loc_diag, A_loc_offd, A_info, x_loc_set, r_loc, sigma, sigma_len, seed, 2);
+}
+
+int shifted_lopbicg_static(CSR_Matrix *A_loc_diag, CSR_Matrix *A_loc_offd, INFO_Matrix *A_info,
+                           double *x_loc_set, double *r_loc, double *sigma, int sigma_len, int seed)
+{
+    return dropin(A_loc_diag, A_loc_offd, A_info, x_loc_set, r_loc, sigma, sigma_len, seed, 1);
+}
+
+int shifted_lopbicg_normal(CSR_Matrix *A_loc_diag, CSR_Matrix *A_loc_offd, INFO_Matrix *A_info,
+                           double *x_loc_set, double *r_loc, double *sigma, int sigma_len, int seed)
+{
+    return dropin(A_loc_diag, A_loc_offd, A_info, x_loc_set, r_loc, sigma, sigma_len, seed, 0);
+}

@@ -1,0 +1,712 @@
+/*
+ * hb_kernels.hip -- gfx950 kernels of the shifted BiCG solver with seed
+ * switching, and the C-callable launch layer declared in hb_launch.h.
+ *
+ * One iteration of the reference loop (src/solver_dynamic.c:147-379) is four
+ * kernels that keep every scalar on the device:
+ *   k_spmv<SEED1>  s = (A + sigma_s I) p_s, (r^,s) -> alpha~[k]      (:156-167)
+ *   k_spmv<SEED2>  q = r - alpha~ s fused into y = (A + sigma_s I) q,
+ *                  (q,q), (q,y) -> omega~[k]                           (:171-189)
+ *   k_seed_update  x_s, r update, (r,r), (r^,r) -> beta~[k]; then the
+ *                  scalar stage: per-shift recurrences, convergence
+ *                  check, seed switch with archive replay              (:194-204,
+ *                                                                       :239-359)
+ *   k_shift        seed p update and the windowed x_j / p_j update of
+ *                  every active shift                                  (:208-210,
+ *                                                                       :248-255)
+ * Per-shift vector updates are deferred over a window of up to W iterations
+ * and applied in one pass that reads and writes x_j and p_j once, replaying
+ * the reference's six per-iteration vector operations in their original
+ * order, so x_j and p_j are bit-identical to the per-iteration updates.
+ *
+ * Every floating-point operation is rounded separately (the library is
+ * built with -ffp-contract=off) so that each elementwise step reproduces the
+ * reference's own sequence of roundings.
+ * Reductions: workgroup partials combined by the last workgroup to arrive,
+ * always in the same order (deterministic).  With d.exact set, every dot is
+ * summed serially from 0.0 in ascending index order and every SpMV row
+ * sequentially, the reference's order (src/openmp_vector.c:68-74,
+ * src/openmp_matrix.c:47-58).
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "hb_shared.h"
+#include "hb_launch.h"
+
+#define DEV __device__ __forceinline__
+
+/* ---- reductions ----------------------------------------------------------- */
+DEV double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+/* Sums a and b over the workgroup; the result is valid in thread 0. */
+DEV void block_sum2(double &a, double &b)
+{
+    __shared__ double sh[2][HB_BLOCK / 64];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[0][w] = a; sh[1][w] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = sh[0][0]; b = sh[1][0];
+#pragma unroll
+        for (int t = 1; t < HB_BLOCK / 64; ++t) { a += sh[0][t]; b += sh[1][t]; }
+    }
+    __syncthreads();
+}
+
+/* Last-workgroup detection: returns true in every thread of the workgroup
+ * that arrives last; that workgroup then sees all other workgroups' writes. */
+DEV bool arrive_last(unsigned *cnt, unsigned total)
+{
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(cnt, 1u);
+        last = (prev == total - 1u);
+        if (last) atomicExch(cnt, 0u);
+    }
+    __syncthreads();
+    const bool l = last != 0;
+    if (l) __threadfence();
+    return l;
+}
+
+/* Combines per-workgroup partial pairs in a fixed order (thread 0 result). */
+DEV void reduce_partials(const double *part, int nb, double &a, double &b)
+{
+    double sa = 0.0, sb = 0.0;
+    for (int i = threadIdx.x; i < nb; i += HB_BLOCK) {
+        sa += part[2 * i];
+        sb += part[2 * i + 1];
+    }
+    block_sum2(sa, sb);
+    a = sa; b = sb;
+}
+
+/* sum_i x[i]*y[i] from 0.0 in ascending i (the reference's serial master
+ * sum); products staged in LDS, summed by thread 0; result in thread 0. */
+DEV double serial_dot(const double *x, const double *y, int n, double *lds)
+{
+    double acc = 0.0;
+    for (int base = 0; base < n; base += HB_NNZ_CAP) {
+        const int m = min(HB_NNZ_CAP, n - base);
+        for (int t = threadIdx.x; t < m; t += HB_BLOCK) lds[t] = x[base + t] * y[base + t];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int t = 0; t < m; ++t) acc += lds[t];
+        __syncthreads();
+    }
+    return acc;
+}
+
+DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
+#define PI(d, j, i) ((d).pi[(size_t)(j) * (size_t)(d).cap + (size_t)(i)])
+
+/* ---- SpMV ------------------------------------------------------------------
+ * Workgroup b owns rows [blk[b], blk[b+1]).  When their nonzeros fit in LDS
+ * (<= HB_NNZ_CAP, <= HB_ROWS_CAP rows), the workgroup loads val/col
+ * coalesced, gathers x, stores the products in LDS and reduces each row with
+ * T lanes, T the largest power of two with T*rows <= 256 (at most 64).
+ * Stencil rows give 256 rows per workgroup and T = 1: each row is summed
+ * sequentially in CSR order.  A row longer than HB_NNZ_CAP gets the whole
+ * workgroup.  exact forces T = 1 and sequential long rows. */
+enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
+
+struct PlainArgs {
+    const double *x;
+    double *y;
+    double shift;
+};
+
+template <int MODE>
+struct XFetch {
+    const double *a;      /* SEED1: p_s   SEED2: r_{k-1}   PLAIN: x */
+    const double *s;      /* SEED2: s                               */
+    double nalpha;        /* SEED2: -alpha~[k]                      */
+    DEV double operator()(unsigned c) const
+    {
+        if (MODE == SP_SEED2) return a[c] + nalpha * s[c];
+        return a[c];
+    }
+};
+
+template <int MODE, bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
+{
+    __shared__ double prod[HB_NNZ_CAP];
+    __shared__ unsigned rp[HB_ROWS_CAP + 1];
+    hb_state *st = d.st;
+    int k = 0, seed = 0, switched = 0;
+    double sig, zscale = 1.0;
+    XFetch<MODE> xf;
+    double *rprev = nullptr, *qout = nullptr;
+    if (MODE != SP_PLAIN) {
+        if (st->done) return;
+        k = st->k;
+        seed = st->seed;
+        sig = d.sigma[seed];
+        rprev = d.rring + off((k - 1) % (d.W + 1), d.n);
+        if (MODE == SP_SEED1) {
+            xf.a = d.p_set + off(seed, d.n);
+            switched = st->switched;
+            zscale = st->zscale;
+        } else {
+            xf.a = rprev;
+            xf.s = d.s;
+            xf.nalpha = -d.aa[k];
+            qout = d.qring + off(k % d.W, d.n);
+        }
+    } else {
+        sig = pa.shift;
+        xf.a = pa.x;
+    }
+
+    const unsigned b = blockIdx.x;
+    const unsigned r0 = d.blk[b], r1 = d.blk[b + 1];
+    const int R = (int)(r1 - r0);
+    const unsigned e0 = d.ptr[r0];
+    const int N = (int)(d.ptr[r1] - e0);
+    const int tid = threadIdx.x;
+    double part0 = 0.0, part1 = 0.0;
+
+    auto epilogue = [&](unsigned i, double t) {
+        if (MODE == SP_SEED1) {
+            const double si = t + sig * xf.a[i];
+            d.s[i] = si;
+            part0 += d.rhat[i] * si;
+            if (switched) {
+                const double rr = rprev[i] * zscale;
+                rprev[i] = rr;
+                part1 += d.rhat[i] * rr;
+            }
+        } else if (MODE == SP_SEED2) {
+            const double qi = rprev[i] + xf.nalpha * d.s[i];
+            qout[i] = qi;
+            const double yi = t + sig * qi;
+            d.y[i] = yi;
+            part0 += qi * qi;
+            part1 += qi * yi;
+        } else {
+            pa.y[i] = t + sig * pa.x[i];
+        }
+    };
+
+    if (N <= HB_NNZ_CAP) {
+        for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = d.ptr[r0 + t] - e0;
+        for (int e = tid; e < N; e += HB_BLOCK) prod[e] = d.val[e0 + e] * xf(d.col[e0 + e]);
+        __syncthreads();
+        int T = 1;
+        if (!EXACT) {
+            int q = HB_BLOCK / R;
+            T = 1;
+            while ((T << 1) <= q && (T << 1) <= 64) T <<= 1;
+        }
+        const int per_pass = HB_BLOCK / T;
+        const int lane = tid & (T - 1);
+        for (int row = tid / T; row < R; row += per_pass) {
+            double acc = 0.0;
+            const int hi = (int)rp[row + 1];
+            for (int e = (int)rp[row] + lane; e < hi; e += T) acc += prod[e];
+            for (int o = T >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+            if (lane == 0) epilogue(r0 + row, acc);
+        }
+    } else {
+        /* one row longer than the LDS stage */
+        double acc = 0.0;
+        if (EXACT) {
+            if (tid == 0)
+                for (int e = 0; e < N; ++e) acc += d.val[e0 + e] * xf(d.col[e0 + e]);
+        } else {
+            for (int e = tid; e < N; e += HB_BLOCK) acc += d.val[e0 + e] * xf(d.col[e0 + e]);
+            double dummy = 0.0;
+            block_sum2(acc, dummy);
+        }
+        if (tid == 0) epilogue(r0, acc);
+    }
+
+    if (MODE == SP_PLAIN) return;
+
+    block_sum2(part0, part1);
+    if (tid == 0) { d.part[2 * b] = part0; d.part[2 * b + 1] = part1; }
+    if (!arrive_last(&st->cnt[MODE], gridDim.x)) return;
+
+    double a0, a1;
+    if (EXACT) {
+        if (MODE == SP_SEED1) {
+            a0 = serial_dot(d.rhat, d.s, d.n, prod);
+            a1 = switched ? serial_dot(d.rhat, rprev, d.n, prod) : 0.0;
+        } else {
+            a0 = serial_dot(qout, qout, d.n, prod);
+            a1 = serial_dot(qout, d.y, d.n, prod);
+        }
+    } else {
+        reduce_partials(d.part, gridDim.x, a0, a1);
+    }
+    if (tid == 0) {
+        if (MODE == SP_SEED1) {
+            if (switched) { st->rTr = a1; st->switched = 0; }
+            st->rTs = a0;
+            const double alpha = st->rTr / a0;
+            d.aa[k] = alpha;
+            if (a0 == 0.0 || !isfinite(alpha)) st->status |= HB_ST_BREAKDOWN_RTS;
+        } else {
+            st->qTq = a0;
+            st->qTy = a1;
+            const double omega = a0 / a1;
+            d.wa[k] = omega;
+            st->rTr_old = st->rTr;
+            if (a1 == 0.0 || !isfinite(omega)) st->status |= HB_ST_BREAKDOWN_QTY;
+        }
+    }
+}
+
+/* ---- scalar stage (last workgroup of k_seed_update) ------------------------
+ * src/solver_dynamic.c:204 (beta~), :243-253 (per-shift recurrences),
+ * :278-303 (convergence check), :312-341 (seed switch). */
+DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
+{
+    __shared__ double sh_beta;
+    __shared__ double sh_zp[HB_BLOCK];
+    __shared__ int sh_j[HB_BLOCK];
+    __shared__ int sh_nc[HB_BLOCK];
+    __shared__ int sh_switch;
+    hb_state *st = d.st;
+    const int tid = threadIdx.x;
+    const int k = st->k, s = st->seed, M = d.M;
+
+    if (tid == 0) {
+        st->dot_r = dot_r;
+        const double rTr_old = st->rTr_old;
+        st->rTr = rTr;
+        const double beta = (d.aa[k] / d.wa[k]) * (rTr / rTr_old);
+        d.ba[k] = beta;
+        st->pb = beta;
+        st->pbw = -beta * d.wa[k];
+        sh_beta = beta;
+    }
+    __syncthreads();
+    const double beta = sh_beta;
+    const double ak = d.aa[k], wk = d.wa[k];
+    const double ratio = d.ba[k - 1] / d.aa[k - 1];
+    const double thr = (d.tol * d.tol) * st->dot_zero;
+    const int slot = k - st->win_k0;
+    const double sig_s = d.sigma[s];
+    double *coef = d.coef + (size_t)slot * (size_t)M * 6;
+
+    double best_zp = 1.0;
+    int best_j = -1, nconv = 0;
+    for (int j = tid; j < M; j += HB_BLOCK) {
+        if (d.stop[j]) {
+            if (slot == 0) d.act_len[j] = 0;
+            continue;
+        }
+        double zp;
+        if (j != s) {
+            const double dj = sig_s - d.sigma[j];
+            const double pim1 = PI(d, j, k - 1);
+            const double eta = ratio * ak * d.eta[j] - dj * ak * pim1;
+            const double pik = eta + pim1;
+            const double aj = (pim1 / pik) * ak;
+            const double den = 1.0 - wk * dj;
+            const double wj = wk / den;
+            const double z = d.zeta[j];
+            const double az = aj * z;
+            const double c1 = wj / (pik * z);
+            const double c2 = wj / (az * pik);
+            const double c3 = -wj / (az * pim1);
+            const double znew = den * z;
+            const double t = pim1 / pik;
+            const double bj = t * t * beta;
+            const double c4 = 1.0 / (pik * znew);
+            d.eta[j] = eta;
+            PI(d, j, k) = pik;
+            d.zeta[j] = znew;
+            double *c = coef + (size_t)j * 6;
+            c[0] = c1; c[1] = aj; c[2] = c2; c[3] = c3; c[4] = bj; c[5] = c4;
+            d.act_len[j] = slot + 1;
+            zp = fabs(1.0 / (znew * pik));
+        } else {
+            if (slot == 0) d.act_len[j] = 0;
+            zp = 1.0;
+        }
+        if (zp * zp * dot_r <= thr) {
+            d.stop[j] = 1;
+            d.stop_iter[j] = k;
+            ++nconv;
+        } else if (zp > best_zp) {
+            best_zp = zp;
+            best_j = j;
+        }
+    }
+    sh_zp[tid] = best_zp;
+    sh_j[tid] = best_j;
+    sh_nc[tid] = nconv;
+    __syncthreads();
+    for (int w = HB_BLOCK / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            const double z2 = sh_zp[tid + w];
+            const int j2 = sh_j[tid + w];
+            if (j2 >= 0 && (sh_j[tid] < 0 || z2 > sh_zp[tid] || (z2 == sh_zp[tid] && j2 < sh_j[tid]))) {
+                sh_zp[tid] = z2;
+                sh_j[tid] = j2;
+            }
+            sh_nc[tid] += sh_nc[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        st->stop_count += sh_nc[0];
+        if (sh_j[0] >= 0) st->max_sigma = sh_j[0];
+        int sw = 0;
+        if (d.stop[s] && st->stop_count < M) {
+            if (st->max_sigma >= 0) sw = 1;
+            else st->status |= HB_ST_NO_SWITCH_TARGET;
+        }
+        sh_switch = sw;
+    }
+    __syncthreads();
+
+    const int do_switch = sh_switch;
+    if (do_switch) {
+        const int m = st->max_sigma;
+        const double ds = sig_s - d.sigma[m];
+        if (tid == 0) st->zscale = 1.0 / (d.zeta[m] * PI(d, m, k));
+        for (int i = 1 + tid; i <= k; i += HB_BLOCK) {
+            const double t0 = PI(d, m, i - 1), t1 = PI(d, m, i);
+            d.aa[i] = (t0 / t1) * d.aa[i];
+            d.ba[i] = (t0 / t1) * (t0 / t1) * d.ba[i];
+            d.wa[i] = d.wa[i] / (1.0 - d.wa[i] * ds);
+        }
+        __syncthreads();
+        const double sig_m = d.sigma[m];
+        for (int j = tid; j < M; j += HB_BLOCK) {
+            double e = 0.0, z = 1.0;
+            if (!d.stop[j] && j != m) {
+                const double dm = sig_m - d.sigma[j];
+                for (int i = 1; i <= k; ++i) {
+                    const double pm1 = PI(d, j, i - 1);
+                    e = (d.ba[i - 1] / d.aa[i - 1]) * d.aa[i] * e - dm * d.aa[i] * pm1;
+                    PI(d, j, i) = e + pm1;
+                    z = (1.0 - d.wa[i] * dm) * z;
+                }
+            }
+            d.eta[j] = e;
+            d.zeta[j] = z;
+        }
+        __syncthreads();
+    }
+
+    if (tid == 0) {
+        const int done_next = (st->stop_count >= M) || (k + 1 >= d.cap) ||
+                              (st->status & HB_ST_NO_SWITCH_TARGET);
+        const int len = slot + 1;
+        st->kprev = k;
+        st->prev_seed = s;
+        if (do_switch) {
+            d.switch_k[st->n_switch] = k;
+            d.switch_to[st->n_switch] = st->max_sigma;
+            st->n_switch += 1;
+            st->seed = st->max_sigma;
+            st->switched = 1;
+        }
+        if (do_switch || done_next || len >= d.W) {
+            int cnt = 0;
+            for (int j = 0; j < M; ++j)
+                if (d.act_len[j] > 0) d.flist[cnt++] = j;
+            st->flush = 1;
+            st->flush_k0 = st->win_k0;
+            st->flush_len = len;
+            st->flush_cnt = cnt;
+            st->win_k0 = k + 1;
+        } else {
+            st->flush = 0;
+        }
+        st->k4_todo = 1;
+        st->k = k + 1;
+        st->done = done_next;
+    }
+}
+
+/* ---- seed update: x_s, r, (r,r), (r^,r) (src/solver_dynamic.c:194-204) --- */
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d)
+{
+    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
+    hb_state *st = d.st;
+    if (st->done) return;
+    const int k = st->k, s = st->seed, n = d.n;
+    const double a = d.aa[k], w = d.wa[k], nw = -d.wa[k];
+    const double *q = d.qring + off(k % d.W, n);
+    double *rn = d.rring + off(k % (d.W + 1), n);
+    const double *ps = d.p_set + off(s, n);
+    double *xs = d.x_set + off(s, n);
+    double part0 = 0.0, part1 = 0.0;
+    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < n; i += gridDim.x * HB_BLOCK) {
+        const double qi = q[i];
+        double xi = xs[i];
+        xi = xi + a * ps[i];
+        xi = xi + w * qi;
+        xs[i] = xi;
+        const double ri = qi + nw * d.y[i];
+        rn[i] = ri;
+        part0 += ri * ri;
+        part1 += d.rhat[i] * ri;
+    }
+    block_sum2(part0, part1);
+    if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = part0; d.part[2 * blockIdx.x + 1] = part1; }
+    if (!arrive_last(&st->cnt[2], gridDim.x)) return;
+    double dr, rr;
+    if (EXACT) {
+        dr = serial_dot(rn, rn, n, lds);
+        rr = serial_dot(d.rhat, rn, n, lds);
+    } else {
+        reduce_partials(d.part, gridDim.x, dr, rr);
+    }
+    __shared__ double sh[2];
+    if (threadIdx.x == 0) { sh[0] = dr; sh[1] = rr; }
+    __syncthreads();
+    scalar_stage(d, sh[0], sh[1]);
+}
+
+/* ---- shift window kernel ---------------------------------------------------
+ * Row 0 of the grid applies the seed p update of the finished iteration
+ * (src/solver_dynamic.c:208-210).  When a flush is due, workgroup row y
+ * applies the pending window to shifts flist[y*G .. y*G+G): for each window
+ * iteration b the six vector operations of src/solver_dynamic.c:248-255 in
+ * their order, on values held in registers. */
+template <int WM>
+__global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d)
+{
+    hb_state *st = d.st;
+    if (!st->k4_todo) return;
+    const int n = d.n, W = d.W, M = d.M;
+    const int i = blockIdx.x * HB_BLOCK + threadIdx.x;
+    const int kp = st->kprev;
+    const double *rk = d.rring + off(kp % (W + 1), n);
+    if (blockIdx.y == 0 && i < n) {
+        double *ps = d.p_set + off(st->prev_seed, n);
+        const double pb = st->pb, pbw = st->pbw;
+        double p = ps[i];
+        p = pb * p;
+        p = p + rk[i];
+        p = p + pbw * d.s[i];
+        ps[i] = p;
+    }
+    if (st->flush && i < n) {
+        const int L = st->flush_len, k0 = st->flush_k0, cnt = st->flush_cnt;
+        double R[WM + 1], Q[WM];
+        R[0] = d.rring[off((k0 - 1) % (W + 1), n) + i];
+#pragma unroll
+        for (int b = 0; b < WM; ++b) {
+            if (b < L) {
+                Q[b] = d.qring[off((k0 + b) % W, n) + i];
+                R[b + 1] = d.rring[off((k0 + b) % (W + 1), n) + i];
+            } else {
+                Q[b] = 0.0;
+                R[b + 1] = 0.0;
+            }
+        }
+        const int G = d.shift_group;
+        const int j0 = blockIdx.y * G;
+        const int j1 = min(cnt, j0 + G);
+        for (int jj = j0; jj < j1; ++jj) {
+            const int j = d.flist[jj];
+            const int len = d.act_len[j];
+            double *xp = d.x_set + off(j, n) + i;
+            double *pp = d.p_set + off(j, n) + i;
+            double x = *xp, p = *pp;
+#pragma unroll
+            for (int b = 0; b < WM; ++b) {
+                if (b < len) {
+                    const double *c = d.coef + ((size_t)b * M + j) * 6;
+                    x = x + c[0] * Q[b];
+                    x = x + c[1] * p;
+                    p = p + c[2] * Q[b];
+                    p = p + c[3] * R[b];
+                    p = c[4] * p;
+                    p = p + c[5] * R[b + 1];
+                }
+            }
+            *xp = x;
+            *pp = p;
+        }
+    }
+    if (arrive_last(&st->cnt[3], gridDim.x * gridDim.y) && threadIdx.x == 0) st->k4_todo = 0;
+}
+
+/* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b, int seed)
+{
+    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
+    const int n = d.n, M = d.M;
+    double *rhat = (double *)d.rhat;
+    double part0 = 0.0, part1 = 0.0;
+    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < n; i += gridDim.x * HB_BLOCK) {
+        const double bi = b[i];
+        rhat[i] = bi;
+        d.rring[i] = bi;
+        for (int j = 0; j < M; ++j) d.p_set[off(j, n) + i] = bi;
+        part0 += bi * bi;
+    }
+    block_sum2(part0, part1);
+    if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = part0; d.part[2 * blockIdx.x + 1] = 0.0; }
+    if (!arrive_last(&d.st->cnt[4], gridDim.x)) return;
+    double bb, unused;
+    if (EXACT) bb = serial_dot(b, b, n, lds);
+    else reduce_partials(d.part, gridDim.x, bb, unused);
+    for (int j = threadIdx.x; j < M; j += HB_BLOCK) {
+        d.eta[j] = 0.0;
+        d.zeta[j] = 1.0;
+        d.stop[j] = 0;
+        d.stop_iter[j] = 0;
+        d.act_len[j] = 0;
+        PI(d, j, 0) = 1.0;
+        PI(d, j, 1) = 1.0;
+    }
+    if (threadIdx.x == 0) {
+        hb_state *st = d.st;
+        d.aa[0] = 1.0;
+        d.ba[0] = 0.0;
+        st->rTr = bb;
+        st->rTr_old = bb;
+        st->dot_r = bb;
+        st->dot_zero = bb;
+        st->k = 1;
+        st->kprev = 0;
+        st->seed = seed;
+        st->prev_seed = seed;
+        st->stop_count = 0;
+        st->max_sigma = -1;
+        st->done = (d.cap <= 1);
+        st->switched = 0;
+        st->n_switch = 0;
+        st->win_k0 = 1;
+        st->flush = 0;
+        st->k4_todo = 0;
+        st->status = 0;
+        st->zscale = 1.0;
+    }
+}
+
+__global__ void k_publish(const hb_state *st, int *flag)
+{
+    if (threadIdx.x == 0) {
+        flag[0] = st->done;
+        flag[1] = st->k;
+        __threadfence_system();
+    }
+}
+
+/* A switch in the last iteration leaves r unscaled; apply it as the
+ * reference does before returning r (src/solver_dynamic.c:322). */
+__global__ __launch_bounds__(HB_BLOCK) void k_finalize(hb_dev d)
+{
+    const hb_state *st = d.st;
+    if (!st->switched) return;
+    double *r = d.rring + off(st->kprev % (d.W + 1), d.n);
+    const double c = st->zscale;
+    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < d.n; i += gridDim.x * HB_BLOCK) r[i] = r[i] * c;
+}
+
+/* ---- true residual of every shift ---------------------------------------- */
+__global__ __launch_bounds__(HB_BLOCK) void k_resid_part(hb_dev d, const double *b, double *part)
+{
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * HB_BLOCK + threadIdx.x;
+    const double *x = d.x_set + off(j, d.n);
+    double e2 = 0.0, b2 = 0.0;
+    if (i < d.n) {
+        double t = 0.0;
+        for (unsigned e = d.ptr[i]; e < d.ptr[i + 1]; ++e) t += d.val[e] * x[d.col[e]];
+        t = t + d.sigma[j] * x[i];
+        const double e1 = t - b[i];
+        e2 = e1 * e1;
+        b2 = b[i] * b[i];
+    }
+    block_sum2(e2, b2);
+    if (threadIdx.x == 0) {
+        part[((size_t)j * gridDim.x + blockIdx.x) * 2] = e2;
+        part[((size_t)j * gridDim.x + blockIdx.x) * 2 + 1] = b2;
+    }
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void k_resid_final(const double *part, int nb, double *out)
+{
+    double a, b;
+    reduce_partials(part + (size_t)blockIdx.x * nb * 2, nb, a, b);
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(a) / sqrt(b);
+}
+
+/* ---- launch layer --------------------------------------------------------- */
+template <int WM>
+static void launch_shift(const hb_dev *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_shift<WM>, dim3(d->shift_gx, d->shift_gy), dim3(HB_BLOCK), 0, s, *d);
+}
+
+extern "C" {
+
+int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s)
+{
+    if (d->exact) hipLaunchKernelGGL(k_init<true>, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d, b, seed);
+    else hipLaunchKernelGGL(k_init<false>, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d, b, seed);
+    return (int)hipGetLastError();
+}
+
+int hb_k_iteration(const hb_dev *d, hipStream_t s)
+{
+    PlainArgs none = {nullptr, nullptr, 0.0};
+    const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK);
+    if (d->exact) {
+        hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d, none);
+        hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d, none);
+        hipLaunchKernelGGL(k_seed_update<true>, gv, bl, 0, s, *d);
+    } else {
+        hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d, none);
+        hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d, none);
+        hipLaunchKernelGGL(k_seed_update<false>, gv, bl, 0, s, *d);
+    }
+    if (d->W <= 1) launch_shift<1>(d, s);
+    else if (d->W <= 4) launch_shift<4>(d, s);
+    else if (d->W <= 8) launch_shift<8>(d, s);
+    else launch_shift<16>(d, s);
+    return (int)hipGetLastError();
+}
+
+int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, d->st, flag_dev);
+    return (int)hipGetLastError();
+}
+
+int hb_k_finalize(const hb_dev *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d);
+    return (int)hipGetLastError();
+}
+
+int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *out, hipStream_t s)
+{
+    const int nb = (d->n + HB_BLOCK - 1) / HB_BLOCK;
+    hipLaunchKernelGGL(k_resid_part, dim3(nb, d->M), dim3(HB_BLOCK), 0, s, *d, b, part);
+    hipLaunchKernelGGL(k_resid_final, dim3(d->M), dim3(HB_BLOCK), 0, s, part, nb, out);
+    return (int)hipGetLastError();
+}
+
+int hb_k_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s)
+{
+    PlainArgs pa = {x, y, shift};
+    if (d->exact) hipLaunchKernelGGL((k_spmv<SP_PLAIN, true>), dim3(d->nblk), dim3(HB_BLOCK), 0, s, *d, pa);
+    else hipLaunchKernelGGL((k_spmv<SP_PLAIN, false>), dim3(d->nblk), dim3(HB_BLOCK), 0, s, *d, pa);
+    return (int)hipGetLastError();
+}
+
+} /* extern "C" */

@@ -1,0 +1,41 @@
+/*
+ * hb_launch.h -- the thin launch layer between the C host driver and the
+ * gfx950 kernels in hb_kernels.hip.  Every function enqueues work on the
+ * given stream and returns the hipError_t of the launch (0 on success).
+ */
+#ifndef HB_LAUNCH_H
+#define HB_LAUNCH_H
+
+#include <hip/hip_runtime_api.h>
+#include "hb_shared.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* r^ = r_0 = p_j = b for every shift, the reference's scalar initialisation
+ * (src/solver_dynamic.c:101-123); (b,b) reduced on the device. */
+int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s);
+
+/* One iteration: seed SpMV 1, fused q + SpMV 2, seed update with the scalar
+ * stage (shift recurrences, convergence check, seed switch), and the shift
+ * window kernel.  Each kernel returns at once when the solve is done. */
+int hb_k_iteration(const hb_dev *d, hipStream_t s);
+
+/* Copies {done, k} of the device state into host-visible memory. */
+int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s);
+
+/* Applies a seed-switch rescale of r left pending by the last iteration. */
+int hb_k_finalize(const hb_dev *d, hipStream_t s);
+
+/* out[j] = ||b - (A + sigma_j I) x_j|| / ||b|| for every shift j.
+ * part must hold 2*M*ceil(n/HB_BLOCK) doubles. */
+int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *out, hipStream_t s);
+
+/* y = A x + shift*x through the row-binned SpMV (exact: row-sequential sums). */
+int hb_k_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

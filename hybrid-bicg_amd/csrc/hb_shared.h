@@ -1,0 +1,91 @@
+/*
+ * hb_shared.h -- data shared between the C host driver and the HIP kernels:
+ * the device-resident solver state and the bundle of device pointers one
+ * solve works on.  Plain C so both gcc (host) and hipcc (kernels) read it.
+ */
+#ifndef HB_SHARED_H
+#define HB_SHARED_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HB_BLOCK        256   /* threads per workgroup of every kernel        */
+#define HB_NNZ_CAP      2048  /* nonzeros staged in LDS by one SpMV workgroup */
+#define HB_ROWS_CAP     256   /* rows per SpMV workgroup                       */
+#define HB_WMAX         16    /* largest shift-update window                   */
+
+/* Error bits in hb_state.status */
+#define HB_ST_BREAKDOWN_RTS   1   /* (r^,s) == 0 or non-finite alpha          */
+#define HB_ST_BREAKDOWN_QTY   2   /* (q,y) == 0 or non-finite omega           */
+#define HB_ST_NO_SWITCH_TARGET 4  /* seed stopped but no shift qualified      */
+
+/*
+ * Device-resident state of one shifted solve.  Every scalar the reference
+ * keeps in host variables of shifted_lopbicg_* (src/solver_dynamic.c:22-46)
+ * lives here, written only by the "last" workgroup of a kernel, so no value
+ * crosses to the host during the iteration loop.
+ */
+typedef struct hb_state {
+    double rTr;        /* (r^, r)                                  */
+    double rTr_old;    /* (r^, r) of the previous iteration        */
+    double dot_r;      /* (r, r)                                   */
+    double dot_zero;   /* (b, b)                                   */
+    double rTs, qTq, qTy;
+    double zscale;     /* 1/(zeta_m pi_m[k]) to apply to r after a switch */
+    double pb, pbw;    /* beta~[k] and -beta~[k]*omega~[k] for the seed p update */
+    int k;             /* iteration about to run (1-based, reference k)   */
+    int kprev;         /* iteration the shift kernel works for            */
+    int seed;          /* current seed index                              */
+    int prev_seed;     /* seed of the iteration just finished             */
+    int stop_count;
+    int max_sigma;     /* arg-max shift, persistent like the reference's  */
+    int done;          /* 1 once stop_count == M or the cap is reached    */
+    int switched;      /* a switch fired: r must be rescaled, rTr recomputed */
+    int n_switch;
+    int win_k0;        /* first iteration of the pending shift window     */
+    int flush;         /* shift kernel applies the window this iteration  */
+    int flush_k0;      /* window the flush covers: iterations flush_k0 .. */
+    int flush_len;     /*   .. flush_k0 + flush_len - 1                   */
+    int flush_cnt;     /* shifts in the flush list                        */
+    int k4_todo;       /* shift kernel has work for the finished iteration */
+    int status;        /* HB_ST_* bits                                    */
+    unsigned cnt[6];   /* arrival counters of the last-workgroup pattern  */
+    int pad[2];
+} hb_state;
+
+/* Device pointers and sizes of one solve (passed by value to kernels). */
+typedef struct hb_dev {
+    int n, M, W, cap, nblk;    /* rows, shifts, window, archive length, SpMV workgroups */
+    int exact;                 /* 1: reference summation order everywhere  */
+    int vec_grid;              /* workgroups of the elementwise kernels    */
+    int shift_gx, shift_gy;    /* grid of the shift kernel                 */
+    int shift_group;           /* shifts per shift-kernel workgroup row    */
+    double tol;
+    const double   *val;
+    const unsigned *col;
+    const unsigned *ptr;
+    const unsigned *blk;       /* SpMV workgroup row starts [nblk+1]       */
+    const double   *sigma;     /* [M]                                      */
+    const double   *rhat;      /* [n]   shadow residual r^ = b             */
+    double *rring;             /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */
+    double *qring;             /* [W][n]     q of iteration k at slot k mod W            */
+    double *s, *y;             /* [n] seed SpMV results                    */
+    double *x_set, *p_set;     /* [M][n] shift-major                       */
+    double *aa, *ba, *wa;      /* seed archives alpha~, beta~, omega~ [cap] */
+    double *pi;                /* [M][cap]                                 */
+    double *eta, *zeta;        /* [M]                                      */
+    double *coef;              /* [W][M][6] per-shift vector coefficients  */
+    int    *stop;              /* [M]                                      */
+    int    *stop_iter;         /* [M]                                      */
+    int    *act_len;           /* [M] iterations of the window shift j was updated in */
+    int    *flist;             /* [M] shifts to flush, ascending           */
+    int    *switch_k, *switch_to; /* [M] switch log                        */
+    double *part;              /* workgroup partial sums [max(nblk,vec_grid)*2] */
+    hb_state *st;
+} hb_dev;
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+"""Python front end of the C-ABI library.
+
+* ``dropin_solve`` calls the reference entry points ``shifted_lopbicg_*``
+  exactly as a C driver would (host CSR_Matrix, host x_set and r).
+* ``Plan`` wraps the device-resident ``hbicg_plan`` for solves on torch
+  tensors that already live in HBM (benchmarks, PyTorch binding).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+VARIANTS = ("normal", "static", "dynamic")
+
+
+def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
+                device=0, exit_on_error=False, upload_x=True) -> _lib.Params:
+    p = _lib.default_params()
+    p.tol = float(tol)
+    p.max_iter = int(max_iter)
+    p.window = int(window)
+    p.exact_order = int(bool(exact_order))
+    p.graph = int(graph)
+    p.chunk = int(chunk)
+    p.device = int(device)
+    p.exit_on_error = int(bool(exit_on_error))
+    p.upload_x = int(bool(upload_x))
+    return p
+
+
+@dataclass
+class SolveResult:
+    k: int
+    x: object                 # [M, n] numpy (drop-in) or torch tensor (plan)
+    r: object
+    stop_iter: np.ndarray
+    switches: list            # [(k, new_seed)]
+    report: dict
+
+
+def _report_dict(rep: _lib.Report) -> dict:
+    return {name: getattr(rep, name) for name, _ in rep._fields_}
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def dropin_solve(A, b, sigma, seed, variant="dynamic", params=None, x0=None) -> SolveResult:
+    """Run shifted_lopbicg_<variant> on host arrays, like the reference's main.c."""
+    L = _lib.load()
+    if variant not in VARIANTS:
+        raise ValueError(variant)
+    p = params if params is not None else make_params()
+    _lib.check(L.hbicg_set_params(C.byref(p)), "hbicg_set_params")
+    n = A.n
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    M = sigma.shape[0]
+    val = np.ascontiguousarray(A.val, dtype=np.float64)
+    col = np.ascontiguousarray(A.col, dtype=np.uint32)
+    ptr = np.ascontiguousarray(A.ptr, dtype=np.uint32)
+    up = C.POINTER(C.c_uint)
+    diag = _lib.CSR_Matrix(_dp(val), col.ctypes.data_as(up), ptr.ctypes.data_as(up), int(ptr[-1]), n, n)
+    offd_ptr = np.zeros(n + 1, dtype=np.uint32)
+    offd = _lib.CSR_Matrix(None, None, offd_ptr.ctypes.data_as(up), 0, n, n)
+    rc = np.array([n], dtype=np.int32)
+    dp = np.array([0], dtype=np.int32)
+    info = _lib.INFO_Matrix(int(ptr[-1]), n, n, b"MCRG", rc.ctypes.data_as(C.POINTER(C.c_int)),
+                            dp.ctypes.data_as(C.POINTER(C.c_int)))
+    x = np.zeros((M, n)) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+    r = np.array(b, dtype=np.float64, copy=True)
+    fn = getattr(L, f"shifted_lopbicg_{variant}")
+    k = fn(C.byref(diag), C.byref(offd), C.byref(info), _dp(x), _dp(r), _dp(sigma), M, int(seed))
+    _lib.check(k, f"shifted_lopbicg_{variant}")
+    rep = _lib.Report()
+    stop = np.zeros(M, dtype=np.int32)
+    swk = np.zeros(M, dtype=np.int32)
+    swt = np.zeros(M, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    ns = L.hbicg_last_report(C.byref(rep), stop.ctypes.data_as(ip), M, swk.ctypes.data_as(ip),
+                             swt.ctypes.data_as(ip), M)
+    return SolveResult(k, x, r, stop, [(int(swk[i]), int(swt[i])) for i in range(ns)],
+                       _report_dict(rep))
+
+
+class Plan:
+    """Device-resident solver plan over a CSR matrix held in torch tensors."""
+
+    def __init__(self, A, M, params=None, device="cuda", stream=None):
+        import torch
+        self._torch = torch
+        L = _lib.load()
+        self.L = L
+        self.n = A.n
+        self.M = int(M)
+        dev = torch.device(device)
+        self.device = dev
+        self.val = torch.as_tensor(np.ascontiguousarray(A.val, dtype=np.float64)).to(dev)
+        self.col = torch.as_tensor(np.ascontiguousarray(A.col, dtype=np.uint32).view(np.int32)).to(dev)
+        self.ptr = torch.as_tensor(np.ascontiguousarray(A.ptr, dtype=np.uint32).view(np.int32)).to(dev)
+        self.nnz = int(A.ptr[-1])
+        self.params = params if params is not None else make_params()
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        self.stream = s
+        csr = _lib.Csr(self.val.data_ptr(), self.col.data_ptr(), self.ptr.data_ptr(), self.n, self.nnz)
+        torch.cuda.synchronize(dev)
+        self.h = L.hbicg_plan_create(C.byref(csr), 1, self.M, C.byref(self.params),
+                                     C.c_void_p(s.cuda_stream))
+        if not self.h:
+            _lib.check(L.hbicg_errno() or -4, "hbicg_plan_create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hbicg_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, sigma, seed, x=None, r_out=None):
+        """b: [n] device tensor; sigma: host array [M]; x: [M, n] device tensor
+        accumulated in place (zeros if None).  Returns (k, x, report)."""
+        torch = self._torch
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        assert sigma.shape[0] == self.M
+        if x is None:
+            x = torch.zeros((self.M, self.n), dtype=torch.float64, device=self.device)
+        assert x.is_contiguous() and x.dtype == torch.float64 and b.dtype == torch.float64
+        rep = _lib.Report()
+        k = self.L.hbicg_plan_solve(self.h, C.c_void_p(b.data_ptr()), _dp(sigma), int(seed),
+                                    C.c_void_p(x.data_ptr()),
+                                    C.c_void_p(r_out.data_ptr()) if r_out is not None else None,
+                                    1, C.byref(rep))
+        _lib.check(k, "hbicg_plan_solve")
+        return k, x, _report_dict(rep)
+
+    def report(self):
+        rep = _lib.Report()
+        stop = np.zeros(self.M, dtype=np.int32)
+        swk = np.zeros(self.M, dtype=np.int32)
+        swt = np.zeros(self.M, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        ns = self.L.hbicg_plan_report(self.h, C.byref(rep), stop.ctypes.data_as(ip),
+                                      swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), self.M)
+        return _report_dict(rep), stop, [(int(swk[i]), int(swt[i])) for i in range(ns)]
+
+    def true_residuals(self, b, sigma, x):
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        out = np.zeros(self.M)
+        rc = self.L.hbicg_plan_true_residuals(self.h, C.c_void_p(b.data_ptr()), _dp(sigma),
+                                              C.c_void_p(x.data_ptr()), 1, _dp(out))
+        _lib.check(rc, "hbicg_plan_true_residuals")
+        return out
+
+    def spmv(self, x, shift=0.0):
+        y = self._torch.empty_like(x)
+        _lib.check(self.L.hbicg_plan_spmv(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                                          float(shift)), "hbicg_plan_spmv")
+        return y

@@ -1,0 +1,110 @@
+/*
+ * hbicg.h -- extended C API of the MI355X shifted BiCG library: runtime
+ * parameters (the reference fixes them at compile time, src/solver.h:13-31),
+ * the per-solve report the reference only prints (src/solver_dynamic.c:
+ * 274-306, 397-452), and a device-resident plan used by the benchmarks and
+ * the PyTorch binding.  No function here has a reference counterpart except
+ * where a "Replaces" line says so.
+ */
+#ifndef HBICG_HBICG_H
+#define HBICG_HBICG_H
+
+#include "solver.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* negative return codes */
+#define HBICG_E_ARG        (-1)  /* bad argument (null pointer, M <= 0, seed out of range) */
+#define HBICG_E_NOTSQUARE  (-2)  /* A_info->rows != A_info->cols                        */
+#define HBICG_E_OFFD       (-3)  /* non-empty off-process block / row split             */
+#define HBICG_E_HIP        (-4)  /* HIP runtime error                                   */
+#define HBICG_E_NOMEM      (-5)  /* host or device allocation failed                    */
+#define HBICG_E_NODEVICE   (-6)  /* no HIP device                                       */
+
+/* status bits of hbicg_report.status (device-detected conditions) */
+#define HBICG_S_BREAKDOWN_RTS    1  /* (r^, s) = 0 or non-finite alpha~ */
+#define HBICG_S_BREAKDOWN_QTY    2  /* (q, y) = 0 or non-finite omega~  */
+#define HBICG_S_NO_SWITCH_TARGET 4  /* seed stopped, no shift to switch to; solve ended */
+
+typedef struct hbicg_params {
+    double tol;          /* stopping rule, default EPS (1e-12)                           */
+    int    max_iter;     /* iteration cap, default MAX_ITER (2000)                       */
+    int    window;       /* shift-update window 1..16; 0 = variant default (normal 1, else 8) */
+    int    exact_order;  /* 1: serial dots and row-sequential SpMV (reference order)     */
+    int    graph;        /* -1 variant default (dynamic on), 0 off, 1 on                  */
+    int    chunk;        /* iterations between host checks of the done flag (default 16) */
+    int    device;       /* HIP device ordinal                                            */
+    int    exit_on_error;/* 1 (default): non-square input prints and exit(1) like the reference */
+    int    upload_x;     /* 1 (default): x_loc_set copied to the GPU (honours x0 != 0)   */
+} hbicg_params;
+
+typedef struct hbicg_report {
+    int    k;            /* returned value, iterations + 1 */
+    int    stop_count;   /* shifts that met the stopping rule */
+    int    n_switch;     /* seed switches */
+    int    seed_final;   /* seed of the last iteration */
+    int    status;       /* HBICG_S_* bits */
+    int    M, n;
+    double dot_r, dot_zero, rTr;
+    double t_setup_ms;   /* plan creation (allocation, upload, SpMV binning) */
+    double t_solve_ms;   /* device time of init + iteration loop              */
+    double t_total_ms;   /* wall time of the whole call                        */
+} hbicg_report;
+
+void hbicg_default_params(hbicg_params *p);
+/* Process-wide parameters used by the shifted_lopbicg_* entry points. */
+int  hbicg_set_params(const hbicg_params *p);
+int  hbicg_get_params(hbicg_params *p);
+
+/* Report of the last solve on this process.  stop_iter[M] gets the
+ * iteration at which each shift stopped (0 = never); switch_k / switch_to
+ * (capacity max_switch) the switch log.  Any array may be NULL.
+ * Returns the number of switches or a negative code. */
+int  hbicg_last_report(hbicg_report *rep, int *stop_iter, int M,
+                       int *switch_k, int *switch_to, int max_switch);
+
+/* ---- device-resident plan ------------------------------------------------ */
+typedef struct hbicg_csr {
+    const double   *val;
+    const unsigned *col;
+    const unsigned *ptr;
+    unsigned        n;
+    unsigned        nz;
+} hbicg_csr;
+
+typedef struct hbicg_plan hbicg_plan;
+
+/* Uploads A (a_on_device = 0) or references caller device memory (1),
+ * bins the rows for the SpMV and allocates the workspace for M shifts.
+ * stream: a hipStream_t, or NULL for a stream of the plan's own.
+ * Returns NULL on failure (hbicg_errno() tells why). */
+hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,
+                              const hbicg_params *p, void *stream);
+void        hbicg_plan_destroy(hbicg_plan *pl);
+int         hbicg_errno(void);
+
+/* One solve with the plan.  on_device = 1: b, x_set and r_out are device
+ * pointers (x_set accumulated in place, r_out may be NULL); 0: host
+ * pointers (x_set uploaded when params.upload_x, else zeroed).  sigma is a
+ * host array of M shifts.  Returns k or a negative code; rep may be NULL. */
+int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int seed,
+                     double *x_set, double *r_out, int on_device, hbicg_report *rep);
+
+/* Per-shift true residual ||b - (A + sigma_j I) x_j|| / ||b|| in FP64 on the
+ * GPU (pointers as in hbicg_plan_solve; out is a host array of M). */
+int hbicg_plan_true_residuals(hbicg_plan *pl, const double *b, const double *sigma,
+                              const double *x_set, int on_device, double *out);
+
+/* y = (A + shift I) x with the plan's SpMV (device pointers). */
+int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
+
+/* Stop iterations / switch log of the plan's last solve (host arrays). */
+int hbicg_plan_report(hbicg_plan *pl, hbicg_report *rep, int *stop_iter,
+                      int *switch_k, int *switch_to, int max_switch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

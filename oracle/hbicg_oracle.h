@@ -1,0 +1,83 @@
+/*
+ * hbicg_oracle.h -- CPU restatement of the reference shifted BiCG solver with
+ * seed switching.  TEST INFRASTRUCTURE ONLY: this library is the checker that
+ * the HIP product path is compared against (and the timed CPU baseline).  No
+ * product code may link, load or call it.
+ *
+ * The arithmetic follows src/solver_dynamic.c of the reference op for op
+ * (same operation order, every axpy a separate rounding, dot products summed
+ * serially in ascending index order), so that a build with
+ * -ffp-contract=off reproduces the reference's FP64 results bit for bit.
+ * The three reference variants (normal/static/dynamic) differ only in MPI
+ * placement and OpenMP schedules and share this one restatement.
+ */
+#ifndef HBICG_ORACLE_H
+#define HBICG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Plain CSR view, 0-based, 32-bit indices: the reference CSR_Matrix fields
+ * (src/matrix.h:19-26) without ownership. */
+typedef struct {
+    const double   *val;
+    const unsigned *col;
+    const unsigned *ptr;
+    unsigned        n;
+    unsigned        nz;
+} orc_csr;
+
+/* Optional per-solve trace.  Any pointer may be NULL. */
+typedef struct {
+    int    *stop_iter;    /* [M]   iteration k at which shift j was stopped (0: never)      */
+    int    *switch_k;     /* [M]   iteration at which the i-th seed switch fired             */
+    int    *switch_to;    /* [M]   new seed chosen by the i-th switch                        */
+    int     n_switch;     /* out:  number of switches                                        */
+    double *alpha;        /* [max_iter+1] seed archive alpha~ as it stands on exit           */
+    double *beta;         /* [max_iter+1] seed archive beta~                                 */
+    double *omega;        /* [max_iter+1] seed archive omega~                                */
+    double *res_hist;     /* [(max_iter+1)*M] estimated residual |1/(zeta pi)|*sqrt(dot_r/dot_zero)
+                             of every unstopped shift at iteration k (row k), -1 when stopped */
+    int    *seed_hist;    /* [max_iter+1] seed index used by iteration k                     */
+    double *zeta;         /* [M]   zeta_j on exit                                            */
+    double *pi_k;         /* [M]   pi_j[k_last] on exit                                      */
+    double *dots;         /* [(max_iter+1)*5] rTs,qTq,qTy,dot_r,rTr of iteration k           */
+} orc_trace;
+
+/* One solve of (A + sigma_j I) x_j = b, j = 0..M-1.
+ *   x_set : [M][n] shift-major, accumulated in place (zero on entry for x0 = 0)
+ *   r     : b on entry, the last seed's residual on exit (reference r_loc)
+ *   tol   : relative-residual stopping rule (reference EPS)
+ *   max_iter : iteration cap (reference MAX_ITER); the loop runs while k < max_iter+1
+ * Returns k = iterations + 1 like the reference, or -1 on bad arguments. */
+int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
+                        const double *sigma, int M, int seed,
+                        double tol, int max_iter, orc_trace *tr);
+
+/* y[i] += sum_j val*x[col], rows summed sequentially from 0.0 (reference
+ * src/matrix.c:498-516 mult). */
+void orc_mult(const orc_csr *A, const double *x, double *y);
+
+/* True relative residual ||b - (A + sigma_j I) x_j|| / ||b|| of every shift,
+ * computed in FP64 the way the reference's DISPLAY_ERROR block does
+ * (src/solver_dynamic.c:405-421) but with the caller's b. */
+void orc_true_residuals(const orc_csr *A, const double *x_set, const double *b,
+                        const double *sigma, int M, double *out);
+
+/* Serial BLAS-1 of src/vector.c:3-27 and the product-vector dot of
+ * src/openmp_vector.c:60-76, exported so the tests can hold them against the
+ * reference sources compiled under oracle/_ref. */
+void   orc_daxpy(int n, double a, const double *x, double *y);
+double orc_ddot(int n, const double *x, const double *y);
+void   orc_dscal(int n, double a, double *x);
+void   orc_dcopy(int n, const double *x, double *y);
+double orc_ddot_v3(int n, const double *x, const double *y, double *tmp);
+
+/* Threads the OpenMP build uses (omp_get_max_threads), 1 without OpenMP. */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

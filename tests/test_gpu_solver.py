@@ -1,0 +1,79 @@
+"""GPU parity of the drop-in entry points against the oracle (the CPU
+restatement of src/solver_dynamic.c), called through the C-ABI.
+
+exact_order=1 runs every dot and SpMV row in the reference's summation
+order, so the whole GPU solve must equal the oracle bit for bit: returned k,
+per-shift stop iterations, seed-switch log, x_set and the returned r.
+The default (tree reductions) must stay within the parity bar: k and every
+switch iteration within +-2 of the oracle, and solutions within 1e-9
+relative (x) of the oracle's, with the true residual of every shift recomputed.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from hbicg import dropin_solve, make_params
+from hbicg import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return P.config0()
+
+
+@pytest.fixture(scope="module")
+def oracle_c0(c0):
+    return {s: O.solve(c0.A, c0.b, c0.sigma, s, tol=c0.tol) for s in (0, 8, 15)}
+
+
+@pytest.mark.parametrize("variant", ["normal", "static", "dynamic"])
+@pytest.mark.parametrize("seed", [0, 8, 15])
+def test_exact_order_bitwise_c0(c0, oracle_c0, variant, seed):
+    print(f"case exact c0 variant={variant} seed={seed}", flush=True)
+    ref = oracle_c0[seed]
+    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, make_params(tol=c0.tol, exact_order=True))
+    assert got.k == ref.k
+    np.testing.assert_array_equal(got.stop_iter, ref.stop_iter)
+    assert got.switches == ref.switches
+    assert np.array_equal(got.x, ref.x)
+    assert np.array_equal(got.r, ref.r)
+
+
+@pytest.mark.parametrize("variant", ["normal", "static", "dynamic"])
+@pytest.mark.parametrize("seed", [0, 8, 15])
+def test_tree_parity_c0(c0, oracle_c0, variant, seed):
+    print(f"case tree c0 variant={variant} seed={seed}", flush=True)
+    ref = oracle_c0[seed]
+    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, make_params(tol=c0.tol))
+    assert abs(got.k - ref.k) <= 2
+    assert np.all(np.abs(got.stop_iter - ref.stop_iter) <= 2)
+    assert len(got.switches) == len(ref.switches)
+    for (kg, sg), (kr, sr) in zip(got.switches, ref.switches):
+        assert abs(kg - kr) <= 2 and sg == sr
+    rel = np.linalg.norm(got.x - ref.x, axis=1) / np.linalg.norm(ref.x, axis=1)
+    assert rel.max() <= 1e-9, rel
+    tr = O.true_residuals(c0.A, got.x, c0.b, c0.sigma)
+    assert tr.max() <= 10 * c0.tol, tr
+
+
+def test_variants_bitwise_equal(c0):
+    """The three entry points compute the same result (src/solver_{normal,
+    static,dynamic}.c share their arithmetic)."""
+    outs = [dropin_solve(c0.A, c0.b, c0.sigma, 8, v, make_params(tol=c0.tol)) for v in
+            ("normal", "static", "dynamic")]
+    for o in outs[1:]:
+        assert o.k == outs[0].k
+        assert np.array_equal(o.x, outs[0].x)
+        assert np.array_equal(o.r, outs[0].r)
+
+
+@pytest.mark.parametrize("window", [1, 3, 8, 16])
+def test_window_bitwise_independent(c0, window):
+    """Deferring the shift updates over a window replays the same per-element
+    operations: x_set must not depend on the window length."""
+    base = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", make_params(tol=c0.tol, window=1))
+    got = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", make_params(tol=c0.tol, window=window))
+    assert got.k == base.k
+    assert np.array_equal(got.x, base.x)

@@ -1,0 +1,246 @@
+"""Pins the oracle (oracle/hbicg_oracle.c) before it is trusted as the
+checker.  The reference ships no tests or fixtures for this path, and its
+solver sources need a Fujitsu profiler header and MPI that this image lacks,
+so the oracle is pinned by:
+  * the reference's own BLAS-1 sources compiled under oracle/_ref (bitwise);
+  * mathematics that fixes the answer: dense brute-force SpMV, exact
+    integer dots, direct sparse solves, a diagonal matrix's closed form, a
+    manufactured solution, and the shifted-Krylov collinearity invariant
+    b - (A + sigma_j I) x_j = r_seed / (zeta_j pi_j) at every iteration,
+    which a dropped term, a wrong sign or a k/k-1 index slip breaks;
+  * seed independence and the polynomial-ratio property of seed switching.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import pyoracle as O
+from hbicg import problems as P
+
+_dp = C.POINTER(C.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return P.config0()
+
+
+def small3d(g=10, m=6):
+    A = P.conv_diff_3d(g, g, g, 6.0, 0.2)
+    b = P.uniform(7, 1, A.n, -1.0, 1.0)
+    return A, b, P.logspace_shifts(m, 1e-2, 1.0)
+
+
+# ---- BLAS-1 against the reference sources -----------------------------------
+@pytest.fixture(scope="module")
+def refblas():
+    if not os.path.exists(O.REF_BLAS_PATH):
+        pytest.skip("oracle/_ref/libref_blas.so not built (needs /root/reference at build time)")
+    L = C.CDLL(O.REF_BLAS_PATH)
+    L.my_daxpy.argtypes = [C.c_int, C.c_double, _dp, _dp]
+    L.my_ddot.argtypes = [C.c_int, _dp, _dp]
+    L.my_ddot.restype = C.c_double
+    L.my_dscal.argtypes = [C.c_int, C.c_double, _dp]
+    L.my_openmp_ddot_v3.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    return L
+
+
+def test_blas_matches_reference_sources(refblas):
+    L = O.lib()
+    rng = np.random.default_rng(0)
+    n = 10007
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    for a in (0.3, -1.7, 1e-5):
+        y1, y2 = y.copy(), y.copy()
+        L.orc_daxpy(n, a, _p(x), _p(y1))
+        refblas.my_daxpy(n, a, _p(x), _p(y2))
+        assert np.array_equal(y1, y2)
+        z1, z2 = x.copy(), x.copy()
+        L.orc_dscal(n, a, _p(z1))
+        refblas.my_dscal(n, a, _p(z2))
+        assert np.array_equal(z1, z2)
+    assert L.orc_ddot(n, _p(x), _p(y)) == refblas.my_ddot(n, _p(x), _p(y))
+    tmp1, tmp2 = np.zeros(n), np.zeros(n)
+    g = np.zeros(1)
+    v3 = L.orc_ddot_v3(n, _p(x), _p(y), _p(tmp1))
+    refblas.my_openmp_ddot_v3(n, _p(x), _p(y), _p(tmp2), _p(g))
+    assert v3 == g[0]
+    # serial ascending order, not numpy's pairwise sum
+    acc = 0.0
+    for i in range(n):
+        acc += float(x[i] * y[i])
+    assert v3 == acc
+
+
+def test_dot_exact_on_integers():
+    L = O.lib()
+    rng = np.random.default_rng(1)
+    x = rng.integers(-1000, 1000, 50000).astype(np.float64)
+    y = rng.integers(-1000, 1000, 50000).astype(np.float64)
+    assert L.orc_ddot(x.size, _p(x), _p(y)) == float(np.dot(x.astype(np.int64), y.astype(np.int64)))
+
+
+# ---- SpMV ----------------------------------------------------------------------
+def test_mult_dense_bruteforce():
+    rng = np.random.default_rng(2)
+    for n in (1, 5, 33, 64):
+        D = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.3)
+        rows, cols = np.nonzero(D)
+        ptr = np.zeros(n + 1, dtype=np.uint32)
+        ptr[1:] = np.cumsum(np.bincount(rows, minlength=n))
+        A = P.CSR(ptr, cols.astype(np.uint32), D[rows, cols], n)
+        x = rng.standard_normal(n)
+        np.testing.assert_allclose(O.mult(A, x), D @ x, rtol=1e-13, atol=1e-13)
+
+
+def test_mult_rowwise_sequential(c0):
+    """Each row summed from 0.0 in CSR order (src/matrix.c:506-515)."""
+    x = P.uniform(5, 5, c0.n, -1, 1)
+    y = O.mult(c0.A, x)
+    A = c0.A
+    for i in (0, 1, 63, 64, 2000, c0.n - 1):
+        t = 0.0
+        for e in range(A.ptr[i], A.ptr[i + 1]):
+            t += float(A.val[e] * x[A.col[e]])
+        assert y[i] == t
+
+
+def test_stencil_against_scipy(c0):
+    x = P.uniform(5, 6, c0.n, -1, 1)
+    np.testing.assert_allclose(O.mult(c0.A, x), c0.A.to_scipy() @ x, rtol=1e-14, atol=1e-14)
+    assert c0.A.nnz == 20224
+    row = 64 * 10 + 10
+    s, e = c0.A.ptr[row], c0.A.ptr[row + 1]
+    assert list(c0.A.col[s:e]) == [row - 64, row - 1, row, row + 1, row + 64]
+    assert list(c0.A.val[s:e]) == [-1.3, -1.3, 4.01, -0.7, -0.7]
+
+
+# ---- whole solve ------------------------------------------------------------------
+def test_c0_converges_true_residual(c0):
+    res = O.solve(c0.A, c0.b, c0.sigma, 0, tol=c0.tol)
+    assert res.k < 2001
+    assert np.all(res.stop_iter > 0)
+    tr = O.true_residuals(c0.A, res.x, c0.b, c0.sigma)
+    assert tr.max() < 5 * c0.tol
+
+
+def test_direct_solve_agreement(c0):
+    res = O.solve(c0.A, c0.b, c0.sigma, 0, tol=1e-12)
+    S = c0.A.to_scipy().tocsc()
+    import scipy.sparse as sp
+    for j in (0, 7, 15):
+        xe = spla.spsolve((S + c0.sigma[j] * sp.identity(c0.n, format="csc")).tocsc(), c0.b)
+        assert np.linalg.norm(res.x[j] - xe) / np.linalg.norm(xe) < 1e-9
+
+
+@pytest.mark.parametrize("seed,cap,tol", [(0, 5, 1e-13), (0, 37, 1e-9), (3, 60, 1e-9),
+                                          (15, 100, 1e-9)])
+def test_collinearity_invariant(c0, seed, cap, tol):
+    """After any number of iterations, with or without seed switches, the true
+    residual of every unstopped shift is the seed residual over zeta_j pi_j.
+    The deviation allowed is the residual gap roundoff builds up (about
+    2e-10*||b|| by iteration 100); a flipped sign, a pi[k]/pi[k-1] slip or a
+    dropped zeta in one vector update gives deviations of 40..8e5*||b||."""
+    res = O.solve(c0.A, c0.b, c0.sigma, seed, tol=1e-10, max_iter=cap)
+    s = res.seed_hist[res.k - 1]
+    S = c0.A.to_scipy()
+    nb = np.linalg.norm(c0.b)
+    active = [j for j in range(c0.M) if res.stop_iter[j] == 0]
+    assert len(active) > 0
+    biggest = 0.0
+    for j in active:
+        rj = c0.b - (S @ res.x[j] + c0.sigma[j] * res.x[j])
+        if j == s:
+            pred = res.r
+        else:
+            pred = res.r / (res.zeta[j] * res.pi_k[j])
+            biggest = max(biggest, np.linalg.norm(pred) / nb)
+        assert np.linalg.norm(rj - pred) / nb < tol, (j, s)
+    assert biggest > 1e3 * tol   # the check is not vacuous
+
+
+def test_switching_happens_and_is_seed_independent(c0):
+    base = O.solve(c0.A, c0.b, c0.sigma, 0, tol=c0.tol)
+    for seed in (5, 15):
+        res = O.solve(c0.A, c0.b, c0.sigma, seed, tol=c0.tol)
+        assert len(res.switches) >= 1
+        k_sw, m = res.switches[0]
+        # the new seed is the unconverged shift with the largest estimate
+        row = res.res_hist[k_sw]
+        cand = [j for j in range(c0.M) if res.stop_iter[j] == 0 or res.stop_iter[j] > k_sw]
+        assert m == max(cand, key=lambda j: (row[j], -j))
+        rel = np.linalg.norm(res.x - base.x, axis=1) / np.linalg.norm(base.x, axis=1)
+        assert rel.max() < 1e-8
+
+
+def test_diagonal_closed_form():
+    n = 7
+    d = np.array([1.0, 2.0, 3.5, 4.0, 5.25, 6.0, 8.0])
+    A = P.CSR(np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32), d, n)
+    b = np.array([1.0, -2.0, 0.5, 3.0, -1.0, 2.0, 0.25])
+    sigma = np.array([0.0, 0.5, 1.5])
+    res = O.solve(A, b, sigma, 1, tol=1e-13, max_iter=50)
+    for j in range(3):
+        np.testing.assert_allclose(res.x[j], b / (d + sigma[j]), rtol=1e-11)
+
+
+def test_manufactured_solution():
+    """b = (A + sigma_s I) 1 makes the seed solution all ones
+    (archive/main_shifted.c:143-151)."""
+    A, _, sigma = small3d()
+    s = 2
+    ones = np.ones(A.n)
+    b = O.mult(A, ones) + sigma[s] * ones
+    res = O.solve(A, b, sigma, s, tol=1e-13)
+    np.testing.assert_allclose(res.x[s], ones, rtol=0, atol=1e-10)
+
+
+def test_single_shift_is_plain_solve():
+    A, b, sigma = small3d()
+    res = O.solve(A, b, sigma[:1], 0, tol=1e-12)
+    import scipy.sparse as sp
+    xe = spla.spsolve((A.to_scipy() + sigma[0] * sp.identity(A.n)).tocsc(), b)
+    assert np.linalg.norm(res.x[0] - xe) / np.linalg.norm(xe) < 1e-10
+
+
+def test_switch_rescale_matches_polynomial_ratio(c0):
+    """A switch re-expresses the archives in the new seed's frame; rerunning
+    from the start with that seed reproduces the iterates to roundoff: the
+    final x of a run that switched 0 -> m agrees with the run seeded at m."""
+    res = O.solve(c0.A, c0.b, c0.sigma, 15, tol=c0.tol)
+    direct = O.solve(c0.A, c0.b, c0.sigma, res.switches[0][1], tol=c0.tol)
+    rel = np.linalg.norm(res.x - direct.x, axis=1) / np.linalg.norm(direct.x, axis=1)
+    assert rel.max() < 1e-8
+
+
+def test_cap_returns_max_iter_plus_one(c0):
+    res = O.solve(c0.A, c0.b, c0.sigma, 0, tol=1e-30, max_iter=10)
+    assert res.k == 11
+    assert np.all(res.stop_iter == 0)
+
+
+def test_thread_count_independent(c0):
+    """Parallel sections never change a rounding (src/openmp_vector.c:68-74
+    sums serially); the result is the same at any thread count."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path[:0]=['oracle','hybrid-bicg_amd'];"
+            "import numpy as np, pyoracle as O; from hbicg import problems as P;"
+            "p=P.config0(); r=O.solve(p.A,p.b,p.sigma,8,tol=p.tol);"
+            "np.save(sys.argv[1], r.x)")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    outs = []
+    for t in (1, 3):
+        f = f"/tmp/hbicg_thr{t}.npy"
+        env = dict(os.environ, OMP_NUM_THREADS=str(t))
+        subprocess.check_call([sys.executable, "-c", code, f], cwd=root, env=env)
+        outs.append(np.load(f))
+    assert np.array_equal(outs[0], outs[1])
