@@ -37,6 +37,7 @@ void hbicg_default_params(hbicg_params *p)
     p->device = 0;
     p->exit_on_error = 1;
     p->upload_x = 1;
+    p->profile = 0;
 }
 
 static const hbicg_params *params(void)
@@ -87,7 +88,7 @@ static double now_ms(void)
 
 /* ---- plan ------------------------------------------------------------------ */
 struct hbicg_plan {
-    int n, M, W, cap, exact, graph, chunk, upload_x, device;
+    int n, M, W, cap, exact, graph, chunk, upload_x, device, profile;
     double tol;
     unsigned nz;
     hipStream_t stream;
@@ -108,6 +109,11 @@ struct hbicg_plan {
     hipGraph_t graph_obj;
     double *graph_x;
     hb_dev dev;
+    /* per-kernel timing (params.profile) */
+    hipEvent_t *pev;          /* [2][chunk][HB_NSTAGE + 1] */
+    double *ptimes;           /* [iterations launched][HB_NSTAGE] */
+    int ptimes_cap;
+    hbicg_kernel_stat kstat[HB_NSTAGE];
     /* last report */
     hbicg_report rep;
     int *h_stop_iter, *h_switch_k, *h_switch_to;
@@ -166,6 +172,12 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     for (int i = 0; i < 2; ++i) if (pl->ev[i]) hipEventDestroy(pl->ev[i]);
     if (pl->t0) hipEventDestroy(pl->t0);
     if (pl->t1) hipEventDestroy(pl->t1);
+    if (pl->pev) {
+        for (int i = 0; i < 2 * pl->chunk * (HB_NSTAGE + 1); ++i)
+            if (pl->pev[i]) hipEventDestroy(pl->pev[i]);
+        free(pl->pev);
+    }
+    free(pl->ptimes);
     if (pl->own_stream && pl->stream) hipStreamDestroy(pl->stream);
     free(pl->h_stop_iter); free(pl->h_switch_k); free(pl->h_switch_to);
     free(pl);
@@ -202,6 +214,8 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->graph = p->graph < 0 ? (variant == 2) : (p->graph ? 1 : 0);
     pl->chunk = p->chunk > 0 ? p->chunk : 16;
     pl->upload_x = p->upload_x;
+    pl->profile = p->profile ? 1 : 0;
+    if (pl->profile) pl->graph = 0;
     if (stream) {
         pl->stream = (hipStream_t)stream;
     } else {
@@ -347,9 +361,68 @@ fail:
     return rc;
 }
 
+/* Direct launches with an event around every kernel; per-iteration
+ * durations land in pl->ptimes, summed once the final k is known. */
+static int run_loop_profiled(hbicg_plan *pl)
+{
+    int rc = 0;
+    const int C = pl->chunk, E = HB_NSTAGE + 1;
+    const int max_chunks = pl->cap / C + 3;
+    if (!pl->pev) {
+        pl->pev = (hipEvent_t *)calloc((size_t)2 * C * E, sizeof(hipEvent_t));
+        if (!pl->pev) return HBICG_E_NOMEM;
+        for (int i = 0; i < 2 * C * E; ++i) HIPCK(hipEventCreate(&pl->pev[i]));
+    }
+    if (pl->ptimes_cap < (max_chunks + 1) * C) {
+        free(pl->ptimes);
+        pl->ptimes_cap = (max_chunks + 1) * C;
+        pl->ptimes = (double *)calloc((size_t)pl->ptimes_cap * HB_NSTAGE, sizeof(double));
+        if (!pl->ptimes) { pl->ptimes_cap = 0; return HBICG_E_NOMEM; }
+    }
+    pl->flag_host[0] = 0;
+    int c = 0, stop = 0;
+    for (; c <= max_chunks && !stop; ++c) {
+        hipEvent_t *ev = pl->pev + (size_t)(c & 1) * C * E;
+        for (int it = 0; it < C; ++it) {
+            HIPCK(hipEventRecord(ev[it * E], pl->stream));
+            for (int sg = 0; sg < HB_NSTAGE; ++sg) {
+                if (hb_k_stage(&pl->dev, sg, pl->stream) != 0) return HBICG_E_HIP;
+                HIPCK(hipEventRecord(ev[it * E + sg + 1], pl->stream));
+            }
+        }
+        if (hb_k_publish(&pl->dev, pl->flag_host, pl->stream) != 0) return HBICG_E_HIP;
+        HIPCK(hipEventRecord(pl->ev[c & 1], pl->stream));
+        if (c > 0) {
+            HIPCK(hipEventSynchronize(pl->ev[(c - 1) & 1]));
+            hipEvent_t *pe = pl->pev + (size_t)((c - 1) & 1) * C * E;
+            for (int it = 0; it < C; ++it)
+                for (int sg = 0; sg < HB_NSTAGE; ++sg) {
+                    float ms = 0.0f;
+                    HIPCK(hipEventElapsedTime(&ms, pe[it * E + sg], pe[it * E + sg + 1]));
+                    pl->ptimes[((size_t)(c - 1) * C + it) * HB_NSTAGE + sg] = ms;
+                }
+            stop = __atomic_load_n(&pl->flag_host[0], __ATOMIC_ACQUIRE);
+        }
+    }
+    HIPCK(hipStreamSynchronize(pl->stream));
+    {
+        hipEvent_t *pe = pl->pev + (size_t)((c - 1) & 1) * C * E;
+        for (int it = 0; it < C; ++it)
+            for (int sg = 0; sg < HB_NSTAGE; ++sg) {
+                float ms = 0.0f;
+                HIPCK(hipEventElapsedTime(&ms, pe[it * E + sg], pe[it * E + sg + 1]));
+                pl->ptimes[((size_t)(c - 1) * C + it) * HB_NSTAGE + sg] = ms;
+            }
+    }
+    return 0;
+fail:
+    return rc;
+}
+
 static int run_loop(hbicg_plan *pl)
 {
     int rc = 0;
+    if (pl->profile) return run_loop_profiled(pl);
     if (pl->graph && (rc = capture_graph(pl)) != 0) return rc;
     pl->flag_host[0] = 0;
     pl->flag_host[1] = 0;
@@ -437,11 +510,34 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->rTr = st.rTr;
     R->t_solve_ms = ms;
     R->t_total_ms = now_ms() - t_start;
+    if (pl->profile) {
+        const double n = (double)pl->n, nz = (double)pl->nz;
+        const double spmv = 12.0 * nz + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
+        const double per_launch[HB_NSTAGE] = {spmv + 24.0 * n, spmv + 32.0 * n, 56.0 * n, 0.0};
+        const int iters = st.k - 1;
+        for (int sg = 0; sg < HB_NSTAGE; ++sg) {
+            hbicg_kernel_stat *K = &pl->kstat[sg];
+            K->launches = iters;
+            K->total_ms = 0.0;
+            for (int it = 0; it < iters && it < pl->ptimes_cap; ++it)
+                K->total_ms += pl->ptimes[(size_t)it * HB_NSTAGE + sg];
+            K->bytes = per_launch[sg] * iters;
+        }
+        pl->kstat[HB_STAGE_SHIFT].bytes = st.shift_bytes;
+    }
     if (rep) *rep = *R;
     return st.k;
 fail:
     hipStreamSynchronize(pl->stream);
     return rc;
+}
+
+int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax)
+{
+    if (!pl || !stats) return HBICG_E_ARG;
+    const int m = nmax < HB_NSTAGE ? nmax : HB_NSTAGE;
+    for (int i = 0; i < m; ++i) stats[i] = pl->kstat[i];
+    return m;
 }
 
 int hbicg_plan_report(hbicg_plan *pl, hbicg_report *rep, int *stop_iter,

@@ -22,8 +22,8 @@
  * Every floating-point operation is rounded separately (the library is
  * built with -ffp-contract=off) so that each elementwise step reproduces the
  * reference's own sequence of roundings.
- * Reductions: workgroup partials combined by the last workgroup to arrive,
- * always in the same order (deterministic).  With d.exact set, every dot is
+ * Reductions: workgroup partials, combined in a fixed order by a one-
+ * workgroup finalize kernel (deterministic).  With d.exact set, every dot is
  * summed serially from 0.0 in ascending index order and every SpMV row
  * sequentially, the reference's order (src/openmp_vector.c:68-74,
  * src/openmp_matrix.c:47-58).
@@ -58,24 +58,6 @@ DEV void block_sum2(double &a, double &b)
         for (int t = 1; t < HB_BLOCK / 64; ++t) { a += sh[0][t]; b += sh[1][t]; }
     }
     __syncthreads();
-}
-
-/* Last-workgroup detection: returns true in every thread of the workgroup
- * that arrives last; that workgroup then sees all other workgroups' writes. */
-DEV bool arrive_last(unsigned *cnt, unsigned total)
-{
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(cnt, 1u);
-        last = (prev == total - 1u);
-        if (last) atomicExch(cnt, 0u);
-    }
-    __syncthreads();
-    const bool l = last != 0;
-    if (l) __threadfence();
-    return l;
 }
 
 /* Combines per-workgroup partial pairs in a fixed order (thread 0 result). */
@@ -148,6 +130,9 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
     XFetch<MODE> xf;
     double *rprev = nullptr, *qout = nullptr;
     if (MODE != SP_PLAIN) {
+        /* the shift kernel runs once per scalar stage: it compares k with
+         * the epoch written here, after the previous shift kernel finished */
+        if (MODE == SP_SEED1 && blockIdx.x == 0 && threadIdx.x == 0) st->k4_epoch = st->k;
         if (st->done) return;
         k = st->k;
         seed = st->seed;
@@ -235,19 +220,34 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
 
     block_sum2(part0, part1);
     if (tid == 0) { d.part[2 * b] = part0; d.part[2 * b + 1] = part1; }
-    if (!arrive_last(&st->cnt[MODE], gridDim.x)) return;
+}
 
+/* One workgroup: combines the SpMV partials and forms alpha~[k] (after SpMV
+ * 1, src/solver_dynamic.c:163-167, with the post-switch (r^,r) of :323) or
+ * omega~[k] (after SpMV 2, :183-190).  A separate launch instead of a
+ * last-workgroup-to-arrive tail: on gfx950 every agent-scope release is an
+ * L2 write-back, far too costly to pay once per SpMV workgroup. */
+template <int MODE, bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_fin_spmv(hb_dev d)
+{
+    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
+    hb_state *st = d.st;
+    if (st->done) return;
+    const int k = st->k, tid = threadIdx.x;
+    const int switched = (MODE == SP_SEED1) ? st->switched : 0;
     double a0, a1;
     if (EXACT) {
+        const double *rprev = d.rring + off((k - 1) % (d.W + 1), d.n);
+        const double *q = d.qring + off(k % d.W, d.n);
         if (MODE == SP_SEED1) {
-            a0 = serial_dot(d.rhat, d.s, d.n, prod);
-            a1 = switched ? serial_dot(d.rhat, rprev, d.n, prod) : 0.0;
+            a0 = serial_dot(d.rhat, d.s, d.n, lds);
+            a1 = switched ? serial_dot(d.rhat, rprev, d.n, lds) : 0.0;
         } else {
-            a0 = serial_dot(qout, qout, d.n, prod);
-            a1 = serial_dot(qout, d.y, d.n, prod);
+            a0 = serial_dot(q, q, d.n, lds);
+            a1 = serial_dot(q, d.y, d.n, lds);
         }
     } else {
-        reduce_partials(d.part, gridDim.x, a0, a1);
+        reduce_partials(d.part, d.nblk, a0, a1);
     }
     if (tid == 0) {
         if (MODE == SP_SEED1) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
     }
 }
 
-/* ---- scalar stage (last workgroup of k_seed_update) ------------------------
+/* ---- scalar stage (one workgroup, after k_seed_update) ---------------------
  * src/solver_dynamic.c:204 (beta~), :243-253 (per-shift recurrences),
  * :278-303 (convergence check), :312-341 (seed switch). */
 DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
@@ -416,6 +416,8 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             st->seed = st->max_sigma;
             st->switched = 1;
         }
+        /* seed p update: read p, r, s, write p */
+        double bytes = 32.0 * (double)d.n;
         if (do_switch || done_next || len >= d.W) {
             int cnt = 0;
             for (int j = 0; j < M; ++j)
@@ -425,20 +427,20 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             st->flush_len = len;
             st->flush_cnt = cnt;
             st->win_k0 = k + 1;
+            /* window vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1}; x_j, p_j read and written */
+            bytes += (8.0 * (2 * len + 1) + 32.0 * cnt) * (double)d.n;
         } else {
             st->flush = 0;
         }
-        st->k4_todo = 1;
+        st->shift_bytes += bytes;
         st->k = k + 1;
         st->done = done_next;
     }
 }
 
 /* ---- seed update: x_s, r, (r,r), (r^,r) (src/solver_dynamic.c:194-204) --- */
-template <bool EXACT>
 __global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d)
 {
-    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     hb_state *st = d.st;
     if (st->done) return;
     const int k = st->k, s = st->seed, n = d.n;
@@ -461,15 +463,25 @@ __global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d)
     }
     block_sum2(part0, part1);
     if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = part0; d.part[2 * blockIdx.x + 1] = part1; }
-    if (!arrive_last(&st->cnt[2], gridDim.x)) return;
+}
+
+/* One workgroup: (r,r), (r^,r) from the seed-update partials, then the
+ * scalar stage. */
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_fin_seed(hb_dev d)
+{
+    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
+    __shared__ double sh[2];
+    hb_state *st = d.st;
+    if (st->done) return;
     double dr, rr;
     if (EXACT) {
-        dr = serial_dot(rn, rn, n, lds);
-        rr = serial_dot(d.rhat, rn, n, lds);
+        const double *rn = d.rring + off(st->k % (d.W + 1), d.n);
+        dr = serial_dot(rn, rn, d.n, lds);
+        rr = serial_dot(d.rhat, rn, d.n, lds);
     } else {
-        reduce_partials(d.part, gridDim.x, dr, rr);
+        reduce_partials(d.part, d.vec_grid, dr, rr);
     }
-    __shared__ double sh[2];
     if (threadIdx.x == 0) { sh[0] = dr; sh[1] = rr; }
     __syncthreads();
     scalar_stage(d, sh[0], sh[1]);
@@ -485,7 +497,7 @@ template <int WM>
 __global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d)
 {
     hb_state *st = d.st;
-    if (!st->k4_todo) return;
+    if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     const int n = d.n, W = d.W, M = d.M;
     const int i = blockIdx.x * HB_BLOCK + threadIdx.x;
     const int kp = st->kprev;
@@ -538,14 +550,11 @@ __global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d)
             *pp = p;
         }
     }
-    if (arrive_last(&st->cnt[3], gridDim.x * gridDim.y) && threadIdx.x == 0) st->k4_todo = 0;
 }
 
 /* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
-template <bool EXACT>
-__global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b, int seed)
+__global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b)
 {
-    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     const int n = d.n, M = d.M;
     double *rhat = (double *)d.rhat;
     double part0 = 0.0, part1 = 0.0;
@@ -558,10 +567,17 @@ __global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b, in
     }
     block_sum2(part0, part1);
     if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = part0; d.part[2 * blockIdx.x + 1] = 0.0; }
-    if (!arrive_last(&d.st->cnt[4], gridDim.x)) return;
+}
+
+/* One workgroup: (b,b) and the scalar initialisation. */
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_init_fin(hb_dev d, const double *b, int seed)
+{
+    __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
+    const int M = d.M;
     double bb, unused;
-    if (EXACT) bb = serial_dot(b, b, n, lds);
-    else reduce_partials(d.part, gridDim.x, bb, unused);
+    if (EXACT) bb = serial_dot(b, b, d.n, lds);
+    else reduce_partials(d.part, d.vec_grid, bb, unused);
     for (int j = threadIdx.x; j < M; j += HB_BLOCK) {
         d.eta[j] = 0.0;
         d.zeta[j] = 1.0;
@@ -590,7 +606,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b, in
         st->n_switch = 0;
         st->win_k0 = 1;
         st->flush = 0;
-        st->k4_todo = 0;
+        st->k4_epoch = 1;
         st->status = 0;
         st->zscale = 1.0;
     }
@@ -656,29 +672,57 @@ extern "C" {
 
 int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s)
 {
-    if (d->exact) hipLaunchKernelGGL(k_init<true>, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d, b, seed);
-    else hipLaunchKernelGGL(k_init<false>, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d, b, seed);
+    hipLaunchKernelGGL(k_init, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d, b);
+    if (d->exact) hipLaunchKernelGGL(k_init_fin<true>, dim3(1), dim3(HB_BLOCK), 0, s, *d, b, seed);
+    else hipLaunchKernelGGL(k_init_fin<false>, dim3(1), dim3(HB_BLOCK), 0, s, *d, b, seed);
+    return (int)hipGetLastError();
+}
+
+int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
+{
+    PlainArgs none = {nullptr, nullptr, 0.0};
+    const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK);
+    switch (stage) {
+    case HB_STAGE_SPMV1:
+        if (d->exact) {
+            hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d, none);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), dim3(1), bl, 0, s, *d);
+        } else {
+            hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d, none);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), dim3(1), bl, 0, s, *d);
+        }
+        break;
+    case HB_STAGE_SPMV2:
+        if (d->exact) {
+            hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d, none);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), dim3(1), bl, 0, s, *d);
+        } else {
+            hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d, none);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, false>), dim3(1), bl, 0, s, *d);
+        }
+        break;
+    case HB_STAGE_SEED:
+        hipLaunchKernelGGL(k_seed_update, gv, bl, 0, s, *d);
+        if (d->exact) hipLaunchKernelGGL(k_fin_seed<true>, dim3(1), bl, 0, s, *d);
+        else hipLaunchKernelGGL(k_fin_seed<false>, dim3(1), bl, 0, s, *d);
+        break;
+    default:
+        if (d->W <= 1) launch_shift<1>(d, s);
+        else if (d->W <= 4) launch_shift<4>(d, s);
+        else if (d->W <= 8) launch_shift<8>(d, s);
+        else launch_shift<16>(d, s);
+        break;
+    }
     return (int)hipGetLastError();
 }
 
 int hb_k_iteration(const hb_dev *d, hipStream_t s)
 {
-    PlainArgs none = {nullptr, nullptr, 0.0};
-    const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK);
-    if (d->exact) {
-        hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d, none);
-        hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d, none);
-        hipLaunchKernelGGL(k_seed_update<true>, gv, bl, 0, s, *d);
-    } else {
-        hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d, none);
-        hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d, none);
-        hipLaunchKernelGGL(k_seed_update<false>, gv, bl, 0, s, *d);
+    for (int st = 0; st < HB_NSTAGE; ++st) {
+        const int e = hb_k_stage(d, st, s);
+        if (e) return e;
     }
-    if (d->W <= 1) launch_shift<1>(d, s);
-    else if (d->W <= 4) launch_shift<4>(d, s);
-    else if (d->W <= 8) launch_shift<8>(d, s);
-    else launch_shift<16>(d, s);
-    return (int)hipGetLastError();
+    return 0;
 }
 
 int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s)

@@ -22,6 +22,14 @@ int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s);
  * window kernel.  Each kernel returns at once when the solve is done. */
 int hb_k_iteration(const hb_dev *d, hipStream_t s);
 
+/* The four kernels of an iteration, one at a time (for per-kernel timing). */
+#define HB_STAGE_SPMV1 0
+#define HB_STAGE_SPMV2 1
+#define HB_STAGE_SEED  2
+#define HB_STAGE_SHIFT 3
+#define HB_NSTAGE      4
+int hb_k_stage(const hb_dev *d, int stage, hipStream_t s);
+
 /* Copies {done, k} of the device state into host-visible memory. */
 int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s);
 

@@ -34,6 +34,7 @@ typedef struct hb_state {
     double rTs, qTq, qTy;
     double zscale;     /* 1/(zeta_m pi_m[k]) to apply to r after a switch */
     double pb, pbw;    /* beta~[k] and -beta~[k]*omega~[k] for the seed p update */
+    double shift_bytes;/* algorithmic HBM bytes the shift kernel moved (summed) */
     int k;             /* iteration about to run (1-based, reference k)   */
     int kprev;         /* iteration the shift kernel works for            */
     int seed;          /* current seed index                              */
@@ -48,7 +49,7 @@ typedef struct hb_state {
     int flush_k0;      /* window the flush covers: iterations flush_k0 .. */
     int flush_len;     /*   .. flush_k0 + flush_len - 1                   */
     int flush_cnt;     /* shifts in the flush list                        */
-    int k4_todo;       /* shift kernel has work for the finished iteration */
+    int k4_epoch;      /* k when the shift kernel last ran (runs while < k) */
     int status;        /* HB_ST_* bits                                    */
     unsigned cnt[6];   /* arrival counters of the last-workgroup pattern  */
     int pad[2];

@@ -43,7 +43,13 @@ class Params(C.Structure):
     """include/hbicg.h hbicg_params."""
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("window", C.c_int),
                 ("exact_order", C.c_int), ("graph", C.c_int), ("chunk", C.c_int),
-                ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int)]
+                ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int),
+                ("profile", C.c_int)]
+
+
+class KernelStat(C.Structure):
+    """include/hbicg.h hbicg_kernel_stat."""
+    _fields_ = [("launches", C.c_int), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
 
 class Report(C.Structure):
@@ -70,7 +76,7 @@ EXPORTS = {
     "hbicg.h": ["hbicg_default_params", "hbicg_set_params", "hbicg_get_params",
                 "hbicg_last_report", "hbicg_plan_create", "hbicg_plan_destroy", "hbicg_errno",
                 "hbicg_plan_solve", "hbicg_plan_true_residuals", "hbicg_plan_spmv",
-                "hbicg_plan_report"],
+                "hbicg_plan_report", "hbicg_plan_kernel_stats"],
 }
 
 _lib = None
@@ -127,6 +133,8 @@ def load():
     L.hbicg_plan_spmv.restype = C.c_int
     L.hbicg_plan_report.argtypes = [C.c_void_p, C.POINTER(Report), _ip, _ip, _ip, C.c_int]
     L.hbicg_plan_report.restype = C.c_int
+    L.hbicg_plan_kernel_stats.argtypes = [C.c_void_p, C.POINTER(KernelStat), C.c_int]
+    L.hbicg_plan_kernel_stats.restype = C.c_int
     _lib = L
     return L
 

@@ -18,7 +18,7 @@ VARIANTS = ("normal", "static", "dynamic")
 
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
-                device=0, exit_on_error=False, upload_x=True) -> _lib.Params:
+                device=0, exit_on_error=False, upload_x=True, profile=False) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -29,6 +29,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.device = int(device)
     p.exit_on_error = int(bool(exit_on_error))
     p.upload_x = int(bool(upload_x))
+    p.profile = int(bool(profile))
     return p
 
 
@@ -104,7 +105,9 @@ class Plan:
         self.ptr = torch.as_tensor(np.ascontiguousarray(A.ptr, dtype=np.uint32).view(np.int32)).to(dev)
         self.nnz = int(A.ptr[-1])
         self.params = params if params is not None else make_params()
-        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        # A stream of the plan's own (default priority); every call orders it
+        # after the caller's current stream and the caller's stream after it.
+        s = stream if stream is not None else torch.cuda.Stream(dev)
         self.stream = s
         csr = _lib.Csr(self.val.data_ptr(), self.col.data_ptr(), self.ptr.data_ptr(), self.n, self.nnz)
         torch.cuda.synchronize(dev)
@@ -134,12 +137,20 @@ class Plan:
             x = torch.zeros((self.M, self.n), dtype=torch.float64, device=self.device)
         assert x.is_contiguous() and x.dtype == torch.float64 and b.dtype == torch.float64
         rep = _lib.Report()
+        self._enter()
         k = self.L.hbicg_plan_solve(self.h, C.c_void_p(b.data_ptr()), _dp(sigma), int(seed),
                                     C.c_void_p(x.data_ptr()),
                                     C.c_void_p(r_out.data_ptr()) if r_out is not None else None,
                                     1, C.byref(rep))
+        self._leave()
         _lib.check(k, "hbicg_plan_solve")
         return k, x, _report_dict(rep)
+
+    def _enter(self):
+        self.stream.wait_stream(self._torch.cuda.current_stream(self.device))
+
+    def _leave(self):
+        self._torch.cuda.current_stream(self.device).wait_stream(self.stream)
 
     def report(self):
         rep = _lib.Report()
@@ -151,16 +162,35 @@ class Plan:
                                       swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), self.M)
         return _report_dict(rep), stop, [(int(swk[i]), int(swt[i])) for i in range(ns)]
 
+    KERNELS = ("spmv_seed", "spmv_q_fused", "seed_update", "shift_window")
+
+    def kernel_stats(self):
+        """Per-kernel {launches, total_ms, bytes, gbps} of the last profiled solve."""
+        arr = (_lib.KernelStat * 4)()
+        m = self.L.hbicg_plan_kernel_stats(self.h, arr, 4)
+        out = {}
+        for i in range(m):
+            st = arr[i]
+            gbps = st.bytes / (st.total_ms * 1e-3) / 1e9 if st.total_ms > 0 else 0.0
+            out[self.KERNELS[i]] = {"launches": st.launches, "total_ms": st.total_ms,
+                                    "bytes": st.bytes, "gbps": gbps}
+        return out
+
     def true_residuals(self, b, sigma, x):
         sigma = np.ascontiguousarray(sigma, dtype=np.float64)
         out = np.zeros(self.M)
+        self._enter()
         rc = self.L.hbicg_plan_true_residuals(self.h, C.c_void_p(b.data_ptr()), _dp(sigma),
                                               C.c_void_p(x.data_ptr()), 1, _dp(out))
+        self._leave()
         _lib.check(rc, "hbicg_plan_true_residuals")
         return out
 
     def spmv(self, x, shift=0.0):
         y = self._torch.empty_like(x)
-        _lib.check(self.L.hbicg_plan_spmv(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
-                                          float(shift)), "hbicg_plan_spmv")
+        self._enter()
+        rc = self.L.hbicg_plan_spmv(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()),
+                                    float(shift))
+        self._leave()
+        _lib.check(rc, "hbicg_plan_spmv")
         return y

@@ -38,6 +38,7 @@ typedef struct hbicg_params {
     int    device;       /* HIP device ordinal                                            */
     int    exit_on_error;/* 1 (default): non-square input prints and exit(1) like the reference */
     int    upload_x;     /* 1 (default): x_loc_set copied to the GPU (honours x0 != 0)   */
+    int    profile;      /* 1: time every kernel with HIP events (direct launches)       */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -99,6 +100,18 @@ int hbicg_plan_true_residuals(hbicg_plan *pl, const double *b, const double *sig
 
 /* y = (A + shift I) x with the plan's SpMV (device pointers). */
 int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
+
+/* Per-kernel statistics of the plan's last solve run with params.profile:
+ * stats[0] seed SpMV 1, [1] fused q + SpMV 2, [2] seed update + scalar
+ * stage, [3] shift window kernel.  launches counts executed iterations,
+ * total_ms the summed HIP-event durations, bytes the algorithmic HBM bytes
+ * those launches had to move.  Returns the number of entries filled. */
+typedef struct hbicg_kernel_stat {
+    int    launches;
+    double total_ms;
+    double bytes;
+} hbicg_kernel_stat;
+int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax);
 
 /* Stop iterations / switch log of the plan's last solve (host arrays). */
 int hbicg_plan_report(hbicg_plan *pl, hbicg_report *rep, int *stop_iter,

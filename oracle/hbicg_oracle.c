@@ -27,6 +27,15 @@
 #include <omp.h>
 #endif
 
+static double wtime(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP
@@ -203,6 +212,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
     if (ws_alloc(&w, n, M, cap) != 0) { ws_free(&w); return -1; }
     if (tr) {
         tr->n_switch = 0;
+        tr->t_seed = tr->t_shift = tr->t_check = 0.0;
         if (tr->stop_iter) memset(tr->stop_iter, 0, (size_t)M * sizeof(int));
         if (tr->res_hist) for (size_t t = 0; t < (size_t)cap * M; ++t) tr->res_hist[t] = -1.0;
         if (tr->seed_hist) memset(tr->seed_hist, 0xff, (size_t)cap * sizeof(int));
@@ -226,6 +236,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
         double *p_s = w.p_set + (size_t)seed * n;
         double *x_s = x_set + (size_t)seed * n;
         if (tr && tr->seed_hist) tr->seed_hist[k] = seed;
+        const double t0 = wtime();
 
         /* seed step (src/solver_dynamic.c:156-210) */
         par_copy(n, r, w.r_old);
@@ -260,6 +271,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
             dd[0] = rTs; dd[1] = qTq; dd[2] = qTy; dd[3] = dot_r; dd[4] = rTr;
         }
 
+        const double t1 = wtime();
         /* shifts (src/solver_dynamic.c:239-256) */
 #pragma omp parallel for schedule(dynamic, 1)
         for (int j = 0; j < M; ++j) {
@@ -267,6 +279,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
             shift_update(&w, sigma, seed, j, k, x_set + (size_t)j * n, r);
         }
 
+        const double t2 = wtime();
         /* convergence check (src/solver_dynamic.c:278-303) */
         double max_zp = 1.0;
         for (int j = 0; j < M; ++j) {
@@ -311,6 +324,12 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
             }
             if (tr) tr->n_switch++;
             seed = m;
+        }
+        if (tr) {
+            const double t3 = wtime();
+            tr->t_seed += t1 - t0;
+            tr->t_shift += t2 - t1;
+            tr->t_check += t3 - t2;
         }
         ++k;
     }

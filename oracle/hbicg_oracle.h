@@ -43,6 +43,9 @@ typedef struct {
     double *zeta;         /* [M]   zeta_j on exit                                            */
     double *pi_k;         /* [M]   pi_j[k_last] on exit                                      */
     double *dots;         /* [(max_iter+1)*5] rTs,qTq,qTy,dot_r,rTr of iteration k           */
+    double  t_seed;       /* out: seconds in seed steps (src/solver_dynamic.c:212-221 buckets) */
+    double  t_shift;      /* out: seconds in the shift loop                                  */
+    double  t_check;      /* out: seconds in convergence check and seed switches             */
 } orc_trace;
 
 /* One solve of (A + sigma_j I) x_j = b, j = 0..M-1.

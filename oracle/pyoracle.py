@@ -29,7 +29,8 @@ class _Csr(C.Structure):
 class _Trace(C.Structure):
     _fields_ = [("stop_iter", _ip), ("switch_k", _ip), ("switch_to", _ip), ("n_switch", C.c_int),
                 ("alpha", _dp), ("beta", _dp), ("omega", _dp), ("res_hist", _dp),
-                ("seed_hist", _ip), ("zeta", _dp), ("pi_k", _dp), ("dots", _dp)]
+                ("seed_hist", _ip), ("zeta", _dp), ("pi_k", _dp), ("dots", _dp),
+                ("t_seed", C.c_double), ("t_shift", C.c_double), ("t_check", C.c_double)]
 
 
 _lib = None
@@ -91,6 +92,9 @@ class OracleResult:
     zeta: np.ndarray
     pi_k: np.ndarray
     dots: np.ndarray         # [(max_iter+1), 5]
+    t_seed: float = 0.0      # seconds in seed steps
+    t_shift: float = 0.0     # seconds in the shift loop
+    t_check: float = 0.0     # seconds in check + switch
 
 
 def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
@@ -112,14 +116,16 @@ def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
     zeta = np.zeros(M); pi_k = np.zeros(M)
     dots = np.zeros(cap * 5)
     tr = _Trace(_p(stop_iter, _ip), _p(sw_k, _ip), _p(sw_to, _ip), 0, _p(alpha), _p(beta),
-                _p(omega), _p(res_hist), _p(seed_hist, _ip), _p(zeta), _p(pi_k), _p(dots))
+                _p(omega), _p(res_hist), _p(seed_hist, _ip), _p(zeta), _p(pi_k), _p(dots),
+                0.0, 0.0, 0.0)
     k = L.orc_shifted_lopbicg(C.byref(h.s), _p(x), _p(r), _p(sigma), M, int(seed), float(tol),
                               int(max_iter), C.byref(tr))
     if k < 0:
         raise ValueError("oracle rejected its arguments")
     sw = [(int(sw_k[i]), int(sw_to[i])) for i in range(tr.n_switch)]
     return OracleResult(k, x, r, stop_iter, sw, alpha, beta, omega, res_hist.reshape(cap, M),
-                        seed_hist, zeta, pi_k, dots.reshape(cap, 5))
+                        seed_hist, zeta, pi_k, dots.reshape(cap, 5), tr.t_seed, tr.t_shift,
+                        tr.t_check)
 
 
 def mult(A, x) -> np.ndarray:
