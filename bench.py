@@ -65,7 +65,7 @@ def main():
     dev = torch.device("cuda:0")
     M, n = prob.M, prob.n
     graph = 1 if args.variant == "dynamic" else 0
-    window = args.window if args.window > 0 else (1 if args.variant == "normal" else 8)
+    window = args.window if args.window > 0 else (1 if args.variant == "normal" else 16)
     params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph)
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)

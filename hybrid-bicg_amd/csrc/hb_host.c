@@ -88,7 +88,7 @@ static double now_ms(void)
 
 /* ---- plan ------------------------------------------------------------------ */
 struct hbicg_plan {
-    int n, M, W, cap, exact, graph, chunk, upload_x, device, profile;
+    int n, M, W, cap, exact, graph, chunk, upload_x, device, profile, vec2;
     double tol;
     unsigned nz;
     hipStream_t stream;
@@ -100,7 +100,7 @@ struct hbicg_plan {
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
     double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *part;
-    int *stop, *stop_iter, *act_len, *flist, *switch_k, *switch_to;
+    int *stop, *stop_iter, *act_len, *flist, *fpos, *switch_k, *switch_to;
     hb_state *st;
     int *flag_host;      /* pinned: {done, k} */
     hipEvent_t ev[2], t0, t1;
@@ -108,6 +108,7 @@ struct hbicg_plan {
     hipGraphExec_t gexec;
     hipGraph_t graph_obj;
     double *graph_x;
+    int graph_vec2;
     hb_dev dev;
     /* per-kernel timing (params.profile) */
     hipEvent_t *pev;          /* [2][chunk][HB_NSTAGE + 1] */
@@ -164,7 +165,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist,
+                    pl->coef, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -186,7 +187,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
 static int window_for(const hbicg_params *p, int variant)
 {
     if (p->window > 0) return p->window;
-    return variant == 0 ? 1 : 8;   /* variant 0: normal */
+    return variant == 0 ? 1 : 16;   /* variant 0: normal */
 }
 
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
@@ -257,6 +258,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     int vec_grid = (n + HB_BLOCK - 1) / HB_BLOCK;
     if (vec_grid > 2048) vec_grid = 2048;
     const int maxparts = pl->nblk > vec_grid ? pl->nblk : vec_grid;
+    const int vec2 = (n % 2) == 0;
     const int gx = (n + HB_BLOCK - 1) / HB_BLOCK;
     int per = 4096 / gx;
     if (per < 1) per = 1;
@@ -284,6 +286,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->stop_iter = (int *)dalloc((size_t)M * 4, &rc);
     pl->act_len = (int *)dalloc((size_t)M * 4, &rc);
     pl->flist = (int *)dalloc((size_t)M * 4, &rc);
+    pl->fpos = (int *)dalloc((size_t)M * 4, &rc);
     pl->switch_k = (int *)dalloc((size_t)M * 4, &rc);
     pl->switch_to = (int *)dalloc((size_t)M * 4, &rc);
     pl->st = (hb_state *)dalloc(sizeof(hb_state), &rc);
@@ -303,14 +306,15 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->n = n; d->M = M; d->W = W; d->cap = pl->cap; d->nblk = pl->nblk;
     d->exact = pl->exact;
     d->vec_grid = vec_grid;
-    d->shift_gx = gx; d->shift_gy = gy; d->shift_group = G;
+    d->shift_gx = gx; d->shift_gy = gy; d->shift_group = G; d->vec2 = vec2;
+    pl->vec2 = vec2;
     d->tol = pl->tol;
     d->val = pl->val; d->col = pl->col; d->ptr = pl->ptr; d->blk = pl->blk;
     d->sigma = pl->sigma; d->rhat = pl->rhat; d->rring = pl->rring; d->qring = pl->qring;
     d->s = pl->s; d->y = pl->y; d->p_set = pl->p_set; d->x_set = NULL;
     d->aa = pl->aa; d->ba = pl->ba; d->wa = pl->wa; d->pi = pl->pi; d->eta = pl->eta;
     d->zeta = pl->zeta; d->coef = pl->coef; d->stop = pl->stop; d->stop_iter = pl->stop_iter;
-    d->act_len = pl->act_len; d->flist = pl->flist; d->switch_k = pl->switch_k;
+    d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
     d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st;
 
     if (a_on_device) free(hptr);
@@ -343,7 +347,7 @@ static int enqueue_chunk(hbicg_plan *pl)
 static int capture_graph(hbicg_plan *pl)
 {
     int rc = 0;
-    if (pl->gexec && pl->graph_x == pl->dev.x_set) return 0;
+    if (pl->gexec && pl->graph_x == pl->dev.x_set && pl->graph_vec2 == pl->dev.vec2) return 0;
     if (pl->gexec) { hipGraphExecDestroy(pl->gexec); pl->gexec = NULL; }
     if (pl->graph_obj) { hipGraphDestroy(pl->graph_obj); pl->graph_obj = NULL; }
     HIPCK(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
@@ -356,6 +360,7 @@ static int capture_graph(hbicg_plan *pl)
     HIPCK(hipStreamEndCapture(pl->stream, &pl->graph_obj));
     HIPCK(hipGraphInstantiate(&pl->gexec, pl->graph_obj, NULL, NULL, 0));
     pl->graph_x = pl->dev.x_set;
+    pl->graph_vec2 = pl->dev.vec2;
     return 0;
 fail:
     return rc;
@@ -472,6 +477,8 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
         db = pl->b_buf;
     }
     pl->dev.x_set = dx;
+    /* 16-byte accesses in the shift kernel need an aligned x_set */
+    pl->dev.vec2 = pl->vec2 && ((((size_t)dx) & 15) == 0);
     HIPCK(hipMemcpyAsync(pl->sigma, sigma, (size_t)pl->M * 8, hipMemcpyHostToDevice, pl->stream));
     HIPCK(hipMemsetAsync(pl->st, 0, sizeof(hb_state), pl->stream));
     HIPCK(hipEventRecord(pl->t0, pl->stream));

@@ -61,12 +61,26 @@ DEV void block_sum2(double &a, double &b)
 }
 
 /* Combines per-workgroup partial pairs in a fixed order (thread 0 result). */
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+/* Eight 16-byte partial pairs in flight per thread before any is added, so
+ * the combine costs about one memory latency, not one per partial. */
 DEV void reduce_partials(const double *part, int nb, double &a, double &b)
 {
+    const dbl2 *p2 = (const dbl2 *)part;
     double sa = 0.0, sb = 0.0;
-    for (int i = threadIdx.x; i < nb; i += HB_BLOCK) {
-        sa += part[2 * i];
-        sb += part[2 * i + 1];
+    int i = threadIdx.x;
+    for (; i + 7 * HB_BLOCK < nb; i += 8 * HB_BLOCK) {
+        dbl2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p2[i + u * HB_BLOCK];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { sa += v[u].x; sb += v[u].y; }
+    }
+    for (; i < nb; i += HB_BLOCK) {
+        const dbl2 v = p2[i];
+        sa += v.x;
+        sb += v.y;
     }
     block_sum2(sa, sb);
     a = sa; b = sb;
@@ -101,96 +115,95 @@ DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
  * workgroup.  exact forces T = 1 and sequential long rows. */
 enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 
-struct PlainArgs {
-    const double *x;
-    double *y;
-    double shift;
-};
-
-template <int MODE>
-struct XFetch {
-    const double *a;      /* SEED1: p_s   SEED2: r_{k-1}   PLAIN: x */
-    const double *s;      /* SEED2: s                               */
-    double nalpha;        /* SEED2: -alpha~[k]                      */
-    DEV double operator()(unsigned c) const
-    {
-        if (MODE == SP_SEED2) return a[c] + nalpha * s[c];
-        return a[c];
-    }
-};
-
+/* The SpMV takes its arrays as __restrict__ arguments: the compiler then
+ * keeps several val/col/x loads in flight and reads uniform values through
+ * scalar loads.  SEED1 gathers p_set[seed], SEED2 gathers r_{k-1} (in
+ * rring) and s, PLAIN gathers x. */
 template <int MODE, bool EXACT>
-__global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
+__global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, const double *__restrict__ val,
+                                                   const unsigned *__restrict__ col,
+                                                   const unsigned *__restrict__ ptr,
+                                                   const unsigned *__restrict__ blk,
+                                                   const double *__restrict__ xbase,
+                                                   double *__restrict__ rring,
+                                                   const double *__restrict__ svec,
+                                                   const double *__restrict__ rhat,
+                                                   double *__restrict__ out1,
+                                                   double *__restrict__ qring,
+                                                   double *__restrict__ part, double shift)
 {
     __shared__ double prod[HB_NNZ_CAP];
     __shared__ unsigned rp[HB_ROWS_CAP + 1];
-    hb_state *st = d.st;
-    int k = 0, seed = 0, switched = 0;
-    double sig, zscale = 1.0;
-    XFetch<MODE> xf;
+    const int n = d.n;
+    int switched = 0;
+    double sig, zscale = 1.0, nalpha = 0.0;
+    const double *xa;
     double *rprev = nullptr, *qout = nullptr;
     if (MODE != SP_PLAIN) {
+        hb_state *st = d.st;
         /* the shift kernel runs once per scalar stage: it compares k with
          * the epoch written here, after the previous shift kernel finished */
         if (MODE == SP_SEED1 && blockIdx.x == 0 && threadIdx.x == 0) st->k4_epoch = st->k;
         if (st->done) return;
-        k = st->k;
-        seed = st->seed;
+        const int k = st->k, seed = st->seed;
         sig = d.sigma[seed];
-        rprev = d.rring + off((k - 1) % (d.W + 1), d.n);
+        rprev = rring + off((k - 1) % (d.W + 1), n);
         if (MODE == SP_SEED1) {
-            xf.a = d.p_set + off(seed, d.n);
+            xa = xbase + off(seed, n);
             switched = st->switched;
             zscale = st->zscale;
         } else {
-            xf.a = rprev;
-            xf.s = d.s;
-            xf.nalpha = -d.aa[k];
-            qout = d.qring + off(k % d.W, d.n);
+            xa = rprev;
+            nalpha = -d.aa[k];
+            qout = qring + off(k % d.W, n);
         }
     } else {
-        sig = pa.shift;
-        xf.a = pa.x;
+        sig = shift;
+        xa = xbase;
     }
+    auto xval = [&](unsigned c) -> double {
+        if (MODE == SP_SEED2) return xa[c] + nalpha * svec[c];
+        return xa[c];
+    };
 
     const unsigned b = blockIdx.x;
-    const unsigned r0 = d.blk[b], r1 = d.blk[b + 1];
+    const unsigned r0 = blk[b], r1 = blk[b + 1];
     const int R = (int)(r1 - r0);
-    const unsigned e0 = d.ptr[r0];
-    const int N = (int)(d.ptr[r1] - e0);
+    const unsigned e0 = ptr[r0];
+    const int N = (int)(ptr[r1] - e0);
     const int tid = threadIdx.x;
     double part0 = 0.0, part1 = 0.0;
 
     auto epilogue = [&](unsigned i, double t) {
         if (MODE == SP_SEED1) {
-            const double si = t + sig * xf.a[i];
-            d.s[i] = si;
-            part0 += d.rhat[i] * si;
+            const double si = t + sig * xa[i];
+            out1[i] = si;
+            part0 += rhat[i] * si;
             if (switched) {
                 const double rr = rprev[i] * zscale;
                 rprev[i] = rr;
-                part1 += d.rhat[i] * rr;
+                part1 += rhat[i] * rr;
             }
         } else if (MODE == SP_SEED2) {
-            const double qi = rprev[i] + xf.nalpha * d.s[i];
+            const double qi = rprev[i] + nalpha * svec[i];
             qout[i] = qi;
             const double yi = t + sig * qi;
-            d.y[i] = yi;
+            out1[i] = yi;
             part0 += qi * qi;
             part1 += qi * yi;
         } else {
-            pa.y[i] = t + sig * pa.x[i];
+            out1[i] = t + sig * xa[i];
         }
     };
 
     if (N <= HB_NNZ_CAP) {
-        for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = d.ptr[r0 + t] - e0;
-        for (int e = tid; e < N; e += HB_BLOCK) prod[e] = d.val[e0 + e] * xf(d.col[e0 + e]);
+        for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
+#pragma unroll 4
+        for (int e = tid; e < N; e += HB_BLOCK) prod[e] = val[e0 + e] * xval(col[e0 + e]);
         __syncthreads();
         int T = 1;
         if (!EXACT) {
-            int q = HB_BLOCK / R;
-            T = 1;
+            const int q = HB_BLOCK / R;
             while ((T << 1) <= q && (T << 1) <= 64) T <<= 1;
         }
         const int per_pass = HB_BLOCK / T;
@@ -207,9 +220,9 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
         double acc = 0.0;
         if (EXACT) {
             if (tid == 0)
-                for (int e = 0; e < N; ++e) acc += d.val[e0 + e] * xf(d.col[e0 + e]);
+                for (int e = 0; e < N; ++e) acc += val[e0 + e] * xval(col[e0 + e]);
         } else {
-            for (int e = tid; e < N; e += HB_BLOCK) acc += d.val[e0 + e] * xf(d.col[e0 + e]);
+            for (int e = tid; e < N; e += HB_BLOCK) acc += val[e0 + e] * xval(col[e0 + e]);
             double dummy = 0.0;
             block_sum2(acc, dummy);
         }
@@ -219,7 +232,25 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, PlainArgs pa)
     if (MODE == SP_PLAIN) return;
 
     block_sum2(part0, part1);
-    if (tid == 0) { d.part[2 * b] = part0; d.part[2 * b + 1] = part1; }
+    if (tid == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
+}
+
+template <int MODE, bool EXACT>
+static void launch_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s)
+{
+    const dim3 g(d->nblk), b(HB_BLOCK);
+    if (MODE == SP_SEED1)
+        hipLaunchKernelGGL((k_spmv<MODE, EXACT>), g, b, 0, s, *d, d->val, d->col, d->ptr, d->blk,
+                           (const double *)d->p_set, d->rring, (const double *)nullptr, d->rhat,
+                           d->s, (double *)nullptr, d->part, 0.0);
+    else if (MODE == SP_SEED2)
+        hipLaunchKernelGGL((k_spmv<MODE, EXACT>), g, b, 0, s, *d, d->val, d->col, d->ptr, d->blk,
+                           (const double *)nullptr, d->rring, (const double *)d->s, d->rhat,
+                           d->y, d->qring, d->part, 0.0);
+    else
+        hipLaunchKernelGGL((k_spmv<MODE, EXACT>), g, b, 0, s, *d, d->val, d->col, d->ptr, d->blk,
+                           x, (double *)nullptr, (const double *)nullptr, d->rhat, y,
+                           (double *)nullptr, (double *)nullptr, shift);
 }
 
 /* One workgroup: combines the SpMV partials and forms alpha~[k] (after SpMV
@@ -279,34 +310,62 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     __shared__ int sh_switch;
     hb_state *st = d.st;
     const int tid = threadIdx.x;
+    /* every value this stage reads, loaded before its first store */
     const int k = st->k, s = st->seed, M = d.M;
+    const int win_k0 = st->win_k0;
+    const double rTr_old = st->rTr_old, dot_zero = st->dot_zero;
+    const double ak = d.aa[k], wk = d.wa[k], bkm1 = d.ba[k - 1], akm1 = d.aa[k - 1];
+    const double sig_s = d.sigma[s];
 
     if (tid == 0) {
+        const double beta = (ak / wk) * (rTr / rTr_old);
         st->dot_r = dot_r;
-        const double rTr_old = st->rTr_old;
         st->rTr = rTr;
-        const double beta = (d.aa[k] / d.wa[k]) * (rTr / rTr_old);
         d.ba[k] = beta;
         st->pb = beta;
-        st->pbw = -beta * d.wa[k];
+        st->pbw = -beta * wk;
         sh_beta = beta;
     }
     __syncthreads();
     const double beta = sh_beta;
-    const double ak = d.aa[k], wk = d.wa[k];
-    const double ratio = d.ba[k - 1] / d.aa[k - 1];
-    const double thr = (d.tol * d.tol) * st->dot_zero;
-    const int slot = k - st->win_k0;
-    const double sig_s = d.sigma[s];
+    const double ratio = bkm1 / akm1;
+    const double thr = (d.tol * d.tol) * dot_zero;
+    const int slot = k - win_k0;
     double *coef = d.coef + (size_t)slot * (size_t)M * 6;
+
+    if (slot == 0) {
+        /* A window opens: its shift list is every shift updated in the
+         * window's first iteration (not stopped, not the seed), ascending.
+         * The set only shrinks inside a window (a seed switch closes it), so
+         * list positions stay valid until the flush; coefficients and
+         * update counts are stored by position, contiguous for the shift
+         * kernel's shifts in flight. */
+        int base = 0;
+        for (int c0 = 0; c0 < M; c0 += HB_BLOCK) {
+            const int j = c0 + tid;
+            const int f = (j < M && !d.stop[j] && j != s) ? 1 : 0;
+            sh_nc[tid] = f;
+            __syncthreads();
+            for (int o = 1; o < HB_BLOCK; o <<= 1) {
+                const int v = tid >= o ? sh_nc[tid - o] : 0;
+                __syncthreads();
+                sh_nc[tid] += v;
+                __syncthreads();
+            }
+            if (f) {
+                d.flist[base + sh_nc[tid] - 1] = j;
+                d.fpos[j] = base + sh_nc[tid] - 1;
+            }
+            base += sh_nc[HB_BLOCK - 1];
+            __syncthreads();
+        }
+        if (tid == 0) st->flush_cnt = base;
+    }
 
     double best_zp = 1.0;
     int best_j = -1, nconv = 0;
     for (int j = tid; j < M; j += HB_BLOCK) {
-        if (d.stop[j]) {
-            if (slot == 0) d.act_len[j] = 0;
-            continue;
-        }
+        if (d.stop[j]) continue;
         double zp;
         if (j != s) {
             const double dj = sig_s - d.sigma[j];
@@ -328,12 +387,12 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             d.eta[j] = eta;
             PI(d, j, k) = pik;
             d.zeta[j] = znew;
-            double *c = coef + (size_t)j * 6;
+            const int pos = d.fpos[j];
+            double *c = coef + (size_t)pos * 6;
             c[0] = c1; c[1] = aj; c[2] = c2; c[3] = c3; c[4] = bj; c[5] = c4;
-            d.act_len[j] = slot + 1;
+            d.act_len[pos] = slot + 1;
             zp = fabs(1.0 / (znew * pik));
         } else {
-            if (slot == 0) d.act_len[j] = 0;
             zp = 1.0;
         }
         if (zp * zp * dot_r <= thr) {
@@ -403,10 +462,11 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         __syncthreads();
     }
 
+    __shared__ int sh_flush;
+    const int len = slot + 1;
     if (tid == 0) {
         const int done_next = (st->stop_count >= M) || (k + 1 >= d.cap) ||
                               (st->status & HB_ST_NO_SWITCH_TARGET);
-        const int len = slot + 1;
         st->kprev = k;
         st->prev_seed = s;
         if (do_switch) {
@@ -416,53 +476,59 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             st->seed = st->max_sigma;
             st->switched = 1;
         }
-        /* seed p update: read p, r, s, write p */
-        double bytes = 32.0 * (double)d.n;
-        if (do_switch || done_next || len >= d.W) {
-            int cnt = 0;
-            for (int j = 0; j < M; ++j)
-                if (d.act_len[j] > 0) d.flist[cnt++] = j;
-            st->flush = 1;
-            st->flush_k0 = st->win_k0;
+        const int fl = do_switch || done_next || len >= d.W;
+        sh_flush = fl;
+        st->flush = fl;
+        if (fl) {
+            st->flush_k0 = win_k0;
             st->flush_len = len;
-            st->flush_cnt = cnt;
             st->win_k0 = k + 1;
-            /* window vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1}; x_j, p_j read and written */
-            bytes += (8.0 * (2 * len + 1) + 32.0 * cnt) * (double)d.n;
-        } else {
-            st->flush = 0;
         }
-        st->shift_bytes += bytes;
         st->k = k + 1;
         st->done = done_next;
+    }
+    if (tid == 0) {
+        /* seed p update: p, r, s in, p out; at a flush also the window
+         * vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1} and x_j, p_j of every
+         * listed shift, read and written once */
+        double bytes = 32.0;
+        if (sh_flush) bytes += 8.0 * (2 * len + 1) + 32.0 * st->flush_cnt;
+        st->shift_bytes += bytes * (double)d.n;
     }
 }
 
 /* ---- seed update: x_s, r, (r,r), (r^,r) (src/solver_dynamic.c:194-204) --- */
-__global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d)
+__global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d, const double *__restrict__ qring,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ pset,
+                                                          double *__restrict__ xset,
+                                                          const double *__restrict__ rhat,
+                                                          double *__restrict__ rring,
+                                                          double *__restrict__ part)
 {
-    hb_state *st = d.st;
+    const hb_state *st = d.st;
     if (st->done) return;
     const int k = st->k, s = st->seed, n = d.n;
     const double a = d.aa[k], w = d.wa[k], nw = -d.wa[k];
-    const double *q = d.qring + off(k % d.W, n);
-    double *rn = d.rring + off(k % (d.W + 1), n);
-    const double *ps = d.p_set + off(s, n);
-    double *xs = d.x_set + off(s, n);
+    const double *q = qring + off(k % d.W, n);
+    double *rn = rring + off(k % (d.W + 1), n);
+    const double *ps = pset + off(s, n);
+    double *xs = xset + off(s, n);
     double part0 = 0.0, part1 = 0.0;
+#pragma unroll 2
     for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < n; i += gridDim.x * HB_BLOCK) {
         const double qi = q[i];
         double xi = xs[i];
         xi = xi + a * ps[i];
         xi = xi + w * qi;
         xs[i] = xi;
-        const double ri = qi + nw * d.y[i];
+        const double ri = qi + nw * y[i];
         rn[i] = ri;
         part0 += ri * ri;
-        part1 += d.rhat[i] * ri;
+        part1 += rhat[i] * ri;
     }
     block_sum2(part0, part1);
-    if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = part0; d.part[2 * blockIdx.x + 1] = part1; }
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = part0; part[2 * blockIdx.x + 1] = part1; }
 }
 
 /* One workgroup: (r,r), (r^,r) from the seed-update partials, then the
@@ -487,68 +553,163 @@ __global__ __launch_bounds__(HB_BLOCK) void k_fin_seed(hb_dev d)
     scalar_stage(d, sh[0], sh[1]);
 }
 
-/* ---- shift window kernel ---------------------------------------------------
- * Row 0 of the grid applies the seed p update of the finished iteration
- * (src/solver_dynamic.c:208-210).  When a flush is due, workgroup row y
- * applies the pending window to shifts flist[y*G .. y*G+G): for each window
- * iteration b the six vector operations of src/solver_dynamic.c:248-255 in
- * their order, on values held in registers. */
-template <int WM>
-__global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d)
+/* V consecutive doubles; V = 2 moves them with one 16-byte access. */
+
+template <int V>
+struct Vd {
+    double v[V];
+};
+
+template <int V, bool NT>
+DEV Vd<V> vload(const double *p)
 {
-    hb_state *st = d.st;
+    Vd<V> r;
+    if (V == 2) {
+        const dbl2 t = NT ? __builtin_nontemporal_load((const dbl2 *)p) : *(const dbl2 *)p;
+        r.v[0] = t.x;
+        r.v[V - 1] = t.y;
+    } else {
+        r.v[0] = NT ? __builtin_nontemporal_load(p) : *p;
+    }
+    return r;
+}
+
+template <int V, bool NT>
+DEV void vstore(double *p, const Vd<V> &r)
+{
+    if (V == 2) {
+        dbl2 t;
+        t.x = r.v[0];
+        t.y = r.v[V - 1];
+        if (NT) __builtin_nontemporal_store(t, (dbl2 *)p);
+        else *(dbl2 *)p = t;
+    } else {
+        if (NT) __builtin_nontemporal_store(r.v[0], p);
+        else *p = r.v[0];
+    }
+}
+
+/* The six vector operations of one shift for window iteration b, in the
+ * reference's order (src/solver_dynamic.c:248-255). */
+template <int V>
+DEV void shift_step(Vd<V> &x, Vd<V> &p, const double *c, const Vd<V> &q, const Vd<V> &ro,
+                    const Vd<V> &rn)
+{
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        double xe = x.v[e], pe = p.v[e];
+        xe = xe + c[0] * q.v[e];
+        xe = xe + c[1] * pe;
+        pe = pe + c[2] * q.v[e];
+        pe = pe + c[3] * ro.v[e];
+        pe = c[4] * pe;
+        pe = pe + c[5] * rn.v[e];
+        x.v[e] = xe;
+        p.v[e] = pe;
+    }
+}
+
+/* ---- shift window kernel ---------------------------------------------------
+ * Each thread owns V consecutive elements.  Row 0 of the grid applies the
+ * seed p update of the finished iteration (src/solver_dynamic.c:208-210).
+ * When a flush is due, workgroup row y applies the pending window to shifts
+ * flist[y*G .. y*G+G): the window's r and q values of the thread's elements
+ * stay in registers, x_j and p_j are read and written once per flush (U
+ * shifts in flight per thread), and every window iteration replays the six
+ * vector operations of src/solver_dynamic.c:248-255 in their order.
+ * The pointer arguments are __restrict__ so that the per-shift
+ * coefficients, uniform across a wavefront, come through scalar loads. */
+template <int WM, int V, int U>
+__global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d, double *__restrict__ xset,
+                                                    double *__restrict__ pset,
+                                                    const double *__restrict__ rring,
+                                                    const double *__restrict__ qring,
+                                                    const double *__restrict__ svec,
+                                                    const double *__restrict__ coef,
+                                                    const int *__restrict__ flist,
+                                                    const int *__restrict__ act_len)
+{
+    const hb_state *st = d.st;
     if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     const int n = d.n, W = d.W, M = d.M;
-    const int i = blockIdx.x * HB_BLOCK + threadIdx.x;
+    const int i = (blockIdx.x * HB_BLOCK + threadIdx.x) * V;
+    if (i >= n) return;
     const int kp = st->kprev;
-    const double *rk = d.rring + off(kp % (W + 1), n);
-    if (blockIdx.y == 0 && i < n) {
-        double *ps = d.p_set + off(st->prev_seed, n);
+    if (blockIdx.y == 0) {
+        double *ps = pset + off(st->prev_seed, n) + i;
         const double pb = st->pb, pbw = st->pbw;
-        double p = ps[i];
-        p = pb * p;
-        p = p + rk[i];
-        p = p + pbw * d.s[i];
-        ps[i] = p;
-    }
-    if (st->flush && i < n) {
-        const int L = st->flush_len, k0 = st->flush_k0, cnt = st->flush_cnt;
-        double R[WM + 1], Q[WM];
-        R[0] = d.rring[off((k0 - 1) % (W + 1), n) + i];
+        Vd<V> p = vload<V, false>(ps);
+        const Vd<V> r = vload<V, false>(rring + off(kp % (W + 1), n) + i);
+        const Vd<V> sv = vload<V, false>(svec + i);
 #pragma unroll
-        for (int b = 0; b < WM; ++b) {
-            if (b < L) {
-                Q[b] = d.qring[off((k0 + b) % W, n) + i];
-                R[b + 1] = d.rring[off((k0 + b) % (W + 1), n) + i];
-            } else {
-                Q[b] = 0.0;
-                R[b + 1] = 0.0;
-            }
+        for (int e = 0; e < V; ++e) {
+            double pe = pb * p.v[e];
+            pe = pe + r.v[e];
+            p.v[e] = pe + pbw * sv.v[e];
         }
-        const int G = d.shift_group;
-        const int j0 = blockIdx.y * G;
-        const int j1 = min(cnt, j0 + G);
-        for (int jj = j0; jj < j1; ++jj) {
-            const int j = d.flist[jj];
-            const int len = d.act_len[j];
-            double *xp = d.x_set + off(j, n) + i;
-            double *pp = d.p_set + off(j, n) + i;
-            double x = *xp, p = *pp;
+        vstore<V, false>(ps, p);
+    }
+    if (!st->flush) return;
+    const int L = st->flush_len, k0 = st->flush_k0, cnt = st->flush_cnt;
+    Vd<V> R[WM + 1], Q[WM];
+    R[0] = vload<V, false>(rring + off((k0 - 1) % (W + 1), n) + i);
+#pragma unroll
+    for (int b = 0; b < WM; ++b) {
+        if (b < L) {
+            Q[b] = vload<V, false>(qring + off((k0 + b) % W, n) + i);
+            R[b + 1] = vload<V, false>(rring + off((k0 + b) % (W + 1), n) + i);
+        }
+    }
+    const int G = d.shift_group;
+    const int j0 = blockIdx.y * G;
+    const int j1 = min(cnt, j0 + G);
+    int jj = j0;
+    for (; jj + U <= j1; jj += U) {
+        int jv[U], lv[U];
+        Vd<V> x[U], p[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            jv[u] = flist[jj + u];
+            lv[u] = act_len[jj + u];
+            x[u] = vload<V, true>(xset + off(jv[u], n) + i);
+            p[u] = vload<V, true>(pset + off(jv[u], n) + i);
+        }
+        bool whole = true;   /* every shift in flight active for the whole window */
+#pragma unroll
+        for (int u = 0; u < U; ++u) whole = whole && (lv[u] == L);
+        if (whole) {
 #pragma unroll
             for (int b = 0; b < WM; ++b) {
-                if (b < len) {
-                    const double *c = d.coef + ((size_t)b * M + j) * 6;
-                    x = x + c[0] * Q[b];
-                    x = x + c[1] * p;
-                    p = p + c[2] * Q[b];
-                    p = p + c[3] * R[b];
-                    p = c[4] * p;
-                    p = p + c[5] * R[b + 1];
+                if (b < L) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        shift_step<V>(x[u], p[u], coef + ((size_t)b * M + jj + u) * 6, Q[b], R[b], R[b + 1]);
                 }
             }
-            *xp = x;
-            *pp = p;
+        } else {
+#pragma unroll
+            for (int b = 0; b < WM; ++b) {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (b < lv[u]) shift_step<V>(x[u], p[u], coef + ((size_t)b * M + jj + u) * 6, Q[b], R[b], R[b + 1]);
+            }
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            vstore<V, true>(xset + off(jv[u], n) + i, x[u]);
+            vstore<V, true>(pset + off(jv[u], n) + i, p[u]);
+        }
+    }
+    for (; jj < j1; ++jj) {
+        const int ja = flist[jj];
+        const int la = act_len[jj];
+        double *xa = xset + off(ja, n) + i, *pa = pset + off(ja, n) + i;
+        Vd<V> x0 = vload<V, true>(xa), p0 = vload<V, true>(pa);
+#pragma unroll
+        for (int b = 0; b < WM; ++b)
+            if (b < la) shift_step<V>(x0, p0, coef + ((size_t)b * M + jj) * 6, Q[b], R[b], R[b + 1]);
+        vstore<V, true>(xa, x0);
+        vstore<V, true>(pa, p0);
     }
 }
 
@@ -584,6 +745,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_init_fin(hb_dev d, const double *b
         d.stop[j] = 0;
         d.stop_iter[j] = 0;
         d.act_len[j] = 0;
+        d.fpos[j] = -1;
         PI(d, j, 0) = 1.0;
         PI(d, j, 1) = 1.0;
     }
@@ -662,10 +824,13 @@ __global__ __launch_bounds__(HB_BLOCK) void k_resid_final(const double *part, in
 }
 
 /* ---- launch layer --------------------------------------------------------- */
-template <int WM>
+template <int WM, int V, int U>
 static void launch_shift(const hb_dev *d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shift<WM>, dim3(d->shift_gx, d->shift_gy), dim3(HB_BLOCK), 0, s, *d);
+    const int per = HB_BLOCK * V;
+    const dim3 g((d->n + per - 1) / per, d->shift_gy), b(HB_BLOCK);
+    hipLaunchKernelGGL((k_shift<WM, V, U>), g, b, 0, s, *d, d->x_set, d->p_set, d->rring,
+                       d->qring, d->s, d->coef, d->flist, d->act_len);
 }
 
 extern "C" {
@@ -680,37 +845,48 @@ int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s)
 
 int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
 {
-    PlainArgs none = {nullptr, nullptr, 0.0};
-    const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK);
+    const dim3 gv(d->vec_grid), bl(HB_BLOCK);
     switch (stage) {
     case HB_STAGE_SPMV1:
         if (d->exact) {
-            hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d, none);
+            launch_spmv<SP_SEED1, true>(d, nullptr, nullptr, 0.0, s);
             hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), dim3(1), bl, 0, s, *d);
         } else {
-            hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d, none);
+            launch_spmv<SP_SEED1, false>(d, nullptr, nullptr, 0.0, s);
             hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), dim3(1), bl, 0, s, *d);
         }
         break;
     case HB_STAGE_SPMV2:
         if (d->exact) {
-            hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d, none);
+            launch_spmv<SP_SEED2, true>(d, nullptr, nullptr, 0.0, s);
             hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), dim3(1), bl, 0, s, *d);
         } else {
-            hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d, none);
+            launch_spmv<SP_SEED2, false>(d, nullptr, nullptr, 0.0, s);
             hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, false>), dim3(1), bl, 0, s, *d);
         }
         break;
     case HB_STAGE_SEED:
-        hipLaunchKernelGGL(k_seed_update, gv, bl, 0, s, *d);
+        hipLaunchKernelGGL(k_seed_update, gv, bl, 0, s, *d, (const double *)d->qring,
+                           (const double *)d->y, (const double *)d->p_set, d->x_set, d->rhat,
+                           d->rring, d->part);
         if (d->exact) hipLaunchKernelGGL(k_fin_seed<true>, dim3(1), bl, 0, s, *d);
         else hipLaunchKernelGGL(k_fin_seed<false>, dim3(1), bl, 0, s, *d);
         break;
     default:
-        if (d->W <= 1) launch_shift<1>(d, s);
-        else if (d->W <= 4) launch_shift<4>(d, s);
-        else if (d->W <= 8) launch_shift<8>(d, s);
-        else launch_shift<16>(d, s);
+        /* windows up to 4 move 16-byte pairs; longer windows hold one
+         * element per thread (the window's r and q stay in registers) with
+         * four shifts in flight */
+        if (d->W <= 1) {
+            if (d->vec2) launch_shift<1, 2, 2>(d, s);
+            else launch_shift<1, 1, 4>(d, s);
+        } else if (d->W <= 4) {
+            if (d->vec2) launch_shift<4, 2, 2>(d, s);
+            else launch_shift<4, 1, 4>(d, s);
+        } else if (d->W <= 8) {
+            launch_shift<8, 1, 4>(d, s);
+        } else {
+            launch_shift<16, 1, 4>(d, s);
+        }
         break;
     }
     return (int)hipGetLastError();
@@ -747,9 +923,8 @@ int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *o
 
 int hb_k_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s)
 {
-    PlainArgs pa = {x, y, shift};
-    if (d->exact) hipLaunchKernelGGL((k_spmv<SP_PLAIN, true>), dim3(d->nblk), dim3(HB_BLOCK), 0, s, *d, pa);
-    else hipLaunchKernelGGL((k_spmv<SP_PLAIN, false>), dim3(d->nblk), dim3(HB_BLOCK), 0, s, *d, pa);
+    if (d->exact) launch_spmv<SP_PLAIN, true>(d, x, y, shift, s);
+    else launch_spmv<SP_PLAIN, false>(d, x, y, shift, s);
     return (int)hipGetLastError();
 }
 

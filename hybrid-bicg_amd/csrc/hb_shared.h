@@ -51,8 +51,7 @@ typedef struct hb_state {
     int flush_cnt;     /* shifts in the flush list                        */
     int k4_epoch;      /* k when the shift kernel last ran (runs while < k) */
     int status;        /* HB_ST_* bits                                    */
-    unsigned cnt[6];   /* arrival counters of the last-workgroup pattern  */
-    int pad[2];
+    int pad;
 } hb_state;
 
 /* Device pointers and sizes of one solve (passed by value to kernels). */
@@ -62,6 +61,7 @@ typedef struct hb_dev {
     int vec_grid;              /* workgroups of the elementwise kernels    */
     int shift_gx, shift_gy;    /* grid of the shift kernel                 */
     int shift_group;           /* shifts per shift-kernel workgroup row    */
+    int vec2;                  /* 1: shift kernel moves 2 doubles per access (n even) */
     double tol;
     const double   *val;
     const unsigned *col;
@@ -76,11 +76,12 @@ typedef struct hb_dev {
     double *aa, *ba, *wa;      /* seed archives alpha~, beta~, omega~ [cap] */
     double *pi;                /* [M][cap]                                 */
     double *eta, *zeta;        /* [M]                                      */
-    double *coef;              /* [W][M][6] per-shift vector coefficients  */
+    double *coef;              /* [W][M][6] vector coefficients by window iteration and list position */
     int    *stop;              /* [M]                                      */
     int    *stop_iter;         /* [M]                                      */
-    int    *act_len;           /* [M] iterations of the window shift j was updated in */
-    int    *flist;             /* [M] shifts to flush, ascending           */
+    int    *act_len;           /* [M] by list position: window iterations that shift was updated in */
+    int    *flist;             /* [M] the window's shift list, ascending   */
+    int    *fpos;              /* [M] position of shift j in the list      */
     int    *switch_k, *switch_to; /* [M] switch log                        */
     double *part;              /* workgroup partial sums [max(nblk,vec_grid)*2] */
     hb_state *st;
