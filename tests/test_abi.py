@@ -1,0 +1,236 @@
+"""C-ABI checks that need no GPU: libhbicg.so loads, exports every function
+include/*.h declares, its host helpers (src/vector.c, src/matrix.c
+replacements) behave like the reference's, and the entry points reject
+input the GPU path cannot take before touching a device."""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from hbicg import _lib, make_params
+from hbicg import problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+_dp = C.POINTER(C.c_double)
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(INCLUDE):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, re.M):
+            name = m.group(1)
+            if name in ("if", "while", "for", "return", "sizeof", "defined"):
+                continue
+            names.add(name)
+    return names
+
+
+def test_exports_every_declared_symbol():
+    L = _lib.load()
+    declared = declared_functions()
+    assert "shifted_lopbicg_dynamic" in declared and "hbicg_plan_solve" in declared
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    listed = {n for v in _lib.EXPORTS.values() for n in v}
+    assert listed == declared, (listed ^ declared)
+
+
+def test_params_roundtrip():
+    L = _lib.load()
+    p = _lib.default_params()
+    assert p.tol == 1e-12 and p.max_iter == 2000 and p.exit_on_error == 1
+    q = make_params(tol=1e-10, max_iter=77, window=4, exact_order=True)
+    assert L.hbicg_set_params(C.byref(q)) == 0
+    r = _lib.Params()
+    assert L.hbicg_get_params(C.byref(r)) == 0
+    assert (r.tol, r.max_iter, r.window, r.exact_order) == (1e-10, 77, 4, 1)
+    bad = make_params(window=17)
+    assert L.hbicg_set_params(C.byref(bad)) == -1
+    bad = make_params(tol=-1.0)
+    assert L.hbicg_set_params(C.byref(bad)) == -1
+
+
+def test_host_blas_matches_oracle_bitwise():
+    L = _lib.load()
+    rng = np.random.default_rng(3)
+    n = 5003
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    assert L.my_ddot(n, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp)) == \
+        O.lib().orc_ddot(n, x.ctypes.data_as(_dp), y.ctypes.data_as(_dp))
+    y1, y2 = y.copy(), y.copy()
+    L.my_daxpy(n, -0.37, x.ctypes.data_as(_dp), y1.ctypes.data_as(_dp))
+    O.lib().orc_daxpy(n, -0.37, x.ctypes.data_as(_dp), y2.ctypes.data_as(_dp))
+    assert np.array_equal(y1, y2)
+    z = x.copy()
+    L.my_dscal(n, 3.5, z.ctypes.data_as(_dp))
+    assert np.array_equal(z, x * 3.5)
+    w = np.zeros(n)
+    L.my_dcopy(n, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    assert np.array_equal(w, x)
+
+
+def _write_mtx(path, header, entries, size):
+    with open(path, "w") as f:
+        f.write(header + "\n% a comment\n")
+        f.write(f"{size[0]} {size[1]} {len(entries)}\n")
+        for e in entries:
+            f.write(" ".join(str(v) for v in e) + "\n")
+
+
+def _csr_arrays(m):
+    n = m.rows
+    ptr = np.ctypeslib.as_array(m.ptr, shape=(n + 1,)).copy()
+    nz = int(ptr[-1])
+    col = np.ctypeslib.as_array(m.col, shape=(nz,)).copy() if nz else np.zeros(0, np.uint32)
+    val = np.ctypeslib.as_array(m.val, shape=(nz,)).copy() if nz else np.zeros(0)
+    return ptr, col, val
+
+
+def test_matrix_market_loaders(tmp_path):
+    """Row-grouped CSR with the file's order kept inside a row (the
+    reference's merge sort is stable, src/matrix.c:135-183)."""
+    L = _lib.load()
+    path = str(tmp_path / "a.mtx")
+    entries = [(2, 1, 0.5), (1, 3, -1.25), (1, 1, 4.0), (3, 3, 2.0), (2, 2, 3.0), (1, 2, 7.0)]
+    _write_mtx(path, "%%MatrixMarket matrix coordinate real general", entries, (3, 3))
+    coo = _lib.COO_Matrix()
+    L.coo_init_matrix(C.byref(coo))
+    assert L.coo_load_matrix(path.encode(), C.byref(coo)) == 0
+    assert (coo.rows, coo.cols, coo.nz) == (3, 3, 6)
+    csr = _lib.CSR_Matrix()
+    L.csr_init_matrix(C.byref(csr))
+    L.coo2csr(C.byref(coo), C.byref(csr))
+    ptr, col, val = _csr_arrays(csr)
+    assert list(ptr) == [0, 3, 5, 6]
+    assert list(col) == [2, 0, 1, 0, 1, 2]
+    assert list(val) == [-1.25, 4.0, 7.0, 0.5, 3.0, 2.0]
+    L.coo_free_matrix(C.byref(coo))
+    L.csr_free_matrix(C.byref(csr))
+    # symmetric flag reported, not expanded (src/matrix.c:93)
+    spath = str(tmp_path / "s.mtx")
+    _write_mtx(spath, "%%MatrixMarket matrix coordinate real symmetric", [(1, 1, 1.0), (2, 1, 2.0)], (2, 2))
+    csr = _lib.CSR_Matrix()
+    assert L.csr_load_matrix(spath.encode(), C.byref(csr)) == 1
+    assert csr.nz == 2
+    L.csr_free_matrix(C.byref(csr))
+    # pattern and integer files get values (1.0 / the integer)
+    ppath = str(tmp_path / "p.mtx")
+    _write_mtx(ppath, "%%MatrixMarket matrix coordinate pattern general", [(1, 2), (2, 1)], (2, 2))
+    csr = _lib.CSR_Matrix()
+    assert L.csr_load_matrix(ppath.encode(), C.byref(csr)) == 0
+    assert list(_csr_arrays(csr)[2]) == [1.0, 1.0]
+    L.csr_free_matrix(C.byref(csr))
+    ipath = str(tmp_path / "i.mtx")
+    _write_mtx(ipath, "%%MatrixMarket matrix coordinate integer general", [(1, 1, 3), (2, 2, -4)], (2, 2))
+    csr = _lib.CSR_Matrix()
+    assert L.csr_load_matrix(ipath.encode(), C.byref(csr)) == 0
+    assert list(_csr_arrays(csr)[2]) == [3.0, -4.0]
+    L.csr_free_matrix(C.byref(csr))
+    # malformed input returns -1 instead of exiting
+    bpath = str(tmp_path / "b.mtx")
+    with open(bpath, "w") as f:
+        f.write("not a matrix market file\n")
+    coo = _lib.COO_Matrix()
+    assert L.coo_load_matrix(bpath.encode(), C.byref(coo)) == -1
+
+
+def test_block_loader_single_rank(tmp_path):
+    L = _lib.load()
+    A = P.conv_diff_2d(6, 5, 4.01, 0.3)
+    path = str(tmp_path / "c.mtx")
+    rows = np.repeat(np.arange(A.n), np.diff(A.ptr.astype(np.int64)))
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{A.n} {A.n} {A.nnz}\n")
+        for r, c, v in zip(rows, A.col, A.val):
+            f.write(f"{r + 1} {c + 1} {v:.17g}\n")
+    diag, offd = _lib.CSR_Matrix(), _lib.CSR_Matrix()
+    rc = (C.c_int * 1)()
+    dp = (C.c_int * 1)()
+    info = _lib.INFO_Matrix(0, 0, 0, b"", rc, dp)
+    L.MPI_csr_load_matrix_block(path.encode(), C.byref(diag), C.byref(offd), C.byref(info))
+    assert (info.rows, info.cols, info.nz) == (A.n, A.n, A.nnz)
+    assert rc[0] == A.n and dp[0] == 0
+    assert offd.nz == 0
+    ptr, col, val = _csr_arrays(diag)
+    assert np.array_equal(ptr, A.ptr) and np.array_equal(col, A.col) and np.array_equal(val, A.val)
+    x = P.uniform(9, 9, A.n, -1, 1)
+    y = np.zeros(A.n)
+    L.mult(C.byref(diag), x.ctypes.data_as(_dp), y.ctypes.data_as(_dp))
+    assert np.array_equal(y, O.mult(A, x))
+    L.csr_free_matrix(C.byref(diag))
+    L.csr_free_matrix(C.byref(offd))
+
+
+def _dropin_args(n=4, offd_nz=0, cols=None):
+    val = np.ones(n)
+    col = np.arange(n, dtype=np.uint32)
+    ptr = np.arange(n + 1, dtype=np.uint32)
+    up = C.POINTER(C.c_uint)
+    diag = _lib.CSR_Matrix(val.ctypes.data_as(_dp), col.ctypes.data_as(up), ptr.ctypes.data_as(up), n, n, n)
+    optr = np.zeros(n + 1, dtype=np.uint32)
+    offd = _lib.CSR_Matrix(None, None, optr.ctypes.data_as(up), offd_nz, n, n)
+    info = _lib.INFO_Matrix(n, n, n if cols is None else cols, b"MCRG", None, None)
+    keep = (val, col, ptr, optr)
+    return diag, offd, info, keep
+
+
+def test_dropin_rejects_bad_input_before_gpu():
+    L = _lib.load()
+    p = make_params(exit_on_error=False)
+    L.hbicg_set_params(C.byref(p))
+    n, M = 4, 2
+    x = np.zeros(M * n)
+    r = np.ones(n)
+    sig = np.array([0.0, 1.0])
+    args = lambda d, o, i: (C.byref(d), C.byref(o), C.byref(i), x.ctypes.data_as(_dp),
+                            r.ctypes.data_as(_dp), sig.ctypes.data_as(_dp))
+    d, o, i, keep = _dropin_args(cols=5)
+    assert L.shifted_lopbicg_static(*args(d, o, i), M, 0) == -2
+    d, o, i, keep = _dropin_args(offd_nz=3)
+    assert L.shifted_lopbicg_dynamic(*args(d, o, i), M, 0) == -3
+    d, o, i, keep = _dropin_args()
+    assert L.shifted_lopbicg_normal(*args(d, o, i), M, 2) == -1
+    assert L.shifted_lopbicg_normal(*args(d, o, i), 0, 0) == -1
+
+
+def test_nonsquare_exits_like_reference():
+    """src/solver_dynamic.c:11-14: message and exit(1) (the default)."""
+    code = (
+        "import sys; sys.path[:0]=['hybrid-bicg_amd','oracle','tests'];"
+        "import ctypes as C, numpy as np; from hbicg import _lib;"
+        "from test_abi import _dropin_args; L=_lib.load();"
+        "d,o,i,k=_dropin_args(cols=7); x=np.zeros(8); r=np.ones(4); s=np.array([0.,1.]);"
+        "dp=C.POINTER(C.c_double);"
+        "L.shifted_lopbicg_dynamic(C.byref(d),C.byref(o),C.byref(i),x.ctypes.data_as(dp),"
+        "r.ctypes.data_as(dp),s.ctypes.data_as(dp),2,0); print('returned')")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True)
+    assert out.returncode == 1
+    assert "Error: matrix is not square." in out.stdout
+    assert "returned" not in out.stdout
+
+
+def test_plan_without_device_reports_nodevice():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    L = _lib.load()
+    A = P.conv_diff_2d(4, 4)
+    csr = _lib.Csr(A.val.ctypes.data, A.col.ctypes.data, A.ptr.ctypes.data, A.n, A.nnz)
+    p = make_params()
+    h = L.hbicg_plan_create(C.byref(csr), 0, 2, C.byref(p), None)
+    assert not h
+    assert L.hbicg_errno() == -6
