@@ -191,7 +191,7 @@ static int window_for(const hbicg_params *p, int variant)
 }
 
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
-                                 const hbicg_params *p, void *stream, int variant)
+                                 const hbicg_params *p, void *stream, int variant, int w_alloc)
 {
     int rc = 0;
     unsigned *hptr = NULL, *hblk = NULL;
@@ -254,7 +254,8 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
 
     /* workspace */
     const size_t nb = (size_t)n * 8;
-    const int W = pl->W;
+    /* ring and coefficient storage for the largest window this plan may run */
+    const int W = w_alloc > pl->W ? w_alloc : pl->W;
     int vec_grid = (n + HB_BLOCK - 1) / HB_BLOCK;
     if (vec_grid > 2048) vec_grid = 2048;
     const int maxparts = pl->nblk > vec_grid ? pl->nblk : vec_grid;
@@ -303,7 +304,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
 
     hb_dev *d = &pl->dev;
     memset(d, 0, sizeof(*d));
-    d->n = n; d->M = M; d->W = W; d->cap = pl->cap; d->nblk = pl->nblk;
+    d->n = n; d->M = M; d->W = pl->W; d->cap = pl->cap; d->nblk = pl->nblk;
     d->exact = pl->exact;
     d->vec_grid = vec_grid;
     d->shift_gx = gx; d->shift_gy = gy; d->shift_group = G; d->vec2 = vec2;
@@ -331,7 +332,7 @@ fail:
 hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,
                               const hbicg_params *p, void *stream)
 {
-    return plan_create_v(A, a_on_device, M, p ? p : params(), stream, 1);
+    return plan_create_v(A, a_on_device, M, p ? p : params(), stream, 1, 0);
 }
 
 /* Enqueues one chunk of iterations followed by the done-flag publication. */
@@ -482,7 +483,9 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     HIPCK(hipMemcpyAsync(pl->sigma, sigma, (size_t)pl->M * 8, hipMemcpyHostToDevice, pl->stream));
     HIPCK(hipMemsetAsync(pl->st, 0, sizeof(hb_state), pl->stream));
     HIPCK(hipEventRecord(pl->t0, pl->stream));
-    if (hb_k_init(&pl->dev, db, seed, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    pl->dev.b_in = db;
+    pl->dev.seed_in = seed;
+    if (hb_k_init(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
     if ((rc = run_loop(pl)) != 0) goto fail;
     if (hb_k_finalize(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
     HIPCK(hipEventRecord(pl->t1, pl->stream));
@@ -638,7 +641,7 @@ static int dropin(CSR_Matrix *A_diag, CSR_Matrix *A_offd, INFO_Matrix *A_info, d
     if (!A_diag->ptr || (A_diag->nz > 0 && (!A_diag->val || !A_diag->col))) return HBICG_E_ARG;
     const double t0 = now_ms();
     hbicg_csr A = {A_diag->val, A_diag->col, A_diag->ptr, A_diag->rows, A_diag->nz};
-    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant);
+    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0);
     if (!pl) return g_errno ? g_errno : HBICG_E_HIP;
     const double t1 = now_ms();
     const int k = hbicg_plan_solve(pl, r, sigma, seed, x_set, r, 0, NULL);
@@ -677,4 +680,338 @@ int shifted_lopbicg_normal(CSR_Matrix *A_loc_diag, CSR_Matrix *A_loc_offd, INFO_
                            double *x_loc_set, double *r_loc, double *sigma, int sigma_len, int seed)
 {
     return dropin(A_loc_diag, A_loc_offd, A_info, x_loc_set, r_loc, sigma, sigma_len, seed, 0);
+}
+
+/* ---- batched solver -----------------------------------------------------------
+ * Many independent problems (different matrices, sizes, shift counts) solved
+ * together: each phase of the iteration is one launch whose workgroups are
+ * the tiles of every problem, found through (problem, tile) tables; done
+ * problems' tiles return at once.  One chunk of iterations is replayed as a
+ * hipGraph, and one device flag reports when every problem is done. */
+struct hbicg_batch {
+    int nprob, device, exact, chunk, graph, W, group_cap;
+    hipStream_t stream;
+    int own_stream;
+    hbicg_plan **plans;
+    hb_dev *hdevs;             /* host copies of the per-problem records */
+    hb_dev *ddevs;             /* device array                           */
+    int *t_spmv, *t_vec, *t_shift;
+    hb_batch B;
+    int *flag_host;
+    hipEvent_t ev[2], t0, t1;
+    hipGraphExec_t gexec;
+    hipGraph_t graph_obj;
+    int graph_dirty;
+    double last_ms;
+};
+
+void hbicg_batch_destroy(hbicg_batch *bt)
+{
+    if (!bt) return;
+    hipSetDevice(bt->device);
+    if (bt->stream) hipStreamSynchronize(bt->stream);
+    if (bt->gexec) hipGraphExecDestroy(bt->gexec);
+    if (bt->graph_obj) hipGraphDestroy(bt->graph_obj);
+    if (bt->plans)
+        for (int p = 0; p < bt->nprob; ++p) hbicg_plan_destroy(bt->plans[p]);
+    free(bt->plans);
+    free(bt->hdevs);
+    if (bt->ddevs) hipFree(bt->ddevs);
+    if (bt->t_spmv) hipFree(bt->t_spmv);
+    if (bt->t_vec) hipFree(bt->t_vec);
+    if (bt->t_shift) hipFree(bt->t_shift);
+    if (bt->flag_host) hipHostFree(bt->flag_host);
+    for (int i = 0; i < 2; ++i) if (bt->ev[i]) hipEventDestroy(bt->ev[i]);
+    if (bt->t0) hipEventDestroy(bt->t0);
+    if (bt->t1) hipEventDestroy(bt->t1);
+    if (bt->own_stream && bt->stream) hipStreamDestroy(bt->stream);
+    free(bt);
+}
+
+/* shift tiles of problem p with at most G shifts per workgroup row */
+static void shift_grid(const hbicg_plan *pl, int G, int *gx, int *gy)
+{
+    *gx = (pl->n + HB_BLOCK - 1) / HB_BLOCK;
+    *gy = (pl->M + G - 1) / G;
+}
+
+/* (Re)builds the shift tile table for group cap G (0: all shifts of a
+ * problem in one workgroup row) and uploads the per-problem records. */
+static int batch_layout(hbicg_batch *bt, int W, int group_cap)
+{
+    int rc = 0, *tab = NULL;
+    size_t tiles = 0;
+    for (int p = 0; p < bt->nprob; ++p) {
+        const hbicg_plan *pl = bt->plans[p];
+        const int G = group_cap > 0 && group_cap < pl->M ? group_cap : pl->M;
+        int gx, gy;
+        shift_grid(pl, G, &gx, &gy);
+        tiles += (size_t)gx * gy;
+        hb_dev *d = &bt->hdevs[p];
+        d->W = W;
+        d->shift_group = G;
+        d->shift_gx = gx;
+        d->shift_gy = gy;
+        d->vec2 = 0;
+    }
+    tab = (int *)malloc(tiles * 2 * sizeof(int));
+    if (!tab) return HBICG_E_NOMEM;
+    size_t t = 0;
+    for (int p = 0; p < bt->nprob; ++p) {
+        const hb_dev *d = &bt->hdevs[p];
+        for (int i = 0; i < d->shift_gx * d->shift_gy; ++i) { tab[2 * t] = p; tab[2 * t + 1] = i; ++t; }
+    }
+    if (bt->t_shift) hipFree(bt->t_shift);
+    bt->t_shift = (int *)dalloc(tiles * 2 * sizeof(int), &rc);
+    if (rc) { free(tab); return rc; }
+    HIPCK(hipMemcpy(bt->t_shift, tab, tiles * 2 * sizeof(int), hipMemcpyHostToDevice));
+    bt->B.t_shift = bt->t_shift;
+    bt->B.n_shift_tiles = (int)tiles;
+    bt->B.W = W;
+    bt->W = W;
+    bt->group_cap = group_cap;
+    bt->graph_dirty = 1;
+    for (int p = 0; p < bt->nprob; ++p) bt->plans[p]->W = W;
+fail:
+    free(tab);
+    return rc;
+}
+
+hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, const int *M,
+                                const hbicg_params *p, void *stream)
+{
+    int rc = 0, *tab = NULL;
+    g_errno = 0;
+    if (!A || !M || nprob <= 0) { g_errno = HBICG_E_ARG; return NULL; }
+    if (!p) p = params();
+    hbicg_batch *bt = (hbicg_batch *)calloc(1, sizeof(*bt));
+    if (!bt) { g_errno = HBICG_E_NOMEM; return NULL; }
+    bt->nprob = nprob;
+    bt->device = p->device;
+    bt->exact = p->exact_order ? 1 : 0;
+    bt->chunk = p->chunk > 0 ? p->chunk : 16;
+    bt->graph = p->graph != 0;
+    HIPCK(hipSetDevice(p->device));
+    if (stream) {
+        bt->stream = (hipStream_t)stream;
+    } else {
+        HIPCK(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking));
+        bt->own_stream = 1;
+    }
+    bt->plans = (hbicg_plan **)calloc((size_t)nprob, sizeof(hbicg_plan *));
+    bt->hdevs = (hb_dev *)calloc((size_t)nprob, sizeof(hb_dev));
+    if (!bt->plans || !bt->hdevs) { rc = HBICG_E_NOMEM; goto fail; }
+    size_t nsp = 0, nvec = 0;
+    for (int q = 0; q < nprob; ++q) {
+        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, HB_WMAX);
+        if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
+        bt->hdevs[q] = bt->plans[q]->dev;
+        nsp += (size_t)bt->plans[q]->nblk;
+        nvec += (size_t)bt->plans[q]->dev.vec_grid;
+    }
+    tab = (int *)malloc((nsp > nvec ? nsp : nvec) * 2 * sizeof(int));
+    if (!tab) { rc = HBICG_E_NOMEM; goto fail; }
+    {
+        size_t t = 0;
+        for (int q = 0; q < nprob; ++q)
+            for (int b = 0; b < bt->plans[q]->nblk; ++b) { tab[2 * t] = q; tab[2 * t + 1] = b; ++t; }
+        bt->t_spmv = (int *)dalloc(nsp * 2 * sizeof(int), &rc);
+        if (rc) goto fail;
+        HIPCK(hipMemcpy(bt->t_spmv, tab, nsp * 2 * sizeof(int), hipMemcpyHostToDevice));
+        t = 0;
+        for (int q = 0; q < nprob; ++q)
+            for (int b = 0; b < bt->plans[q]->dev.vec_grid; ++b) { tab[2 * t] = q; tab[2 * t + 1] = b; ++t; }
+        bt->t_vec = (int *)dalloc(nvec * 2 * sizeof(int), &rc);
+        if (rc) goto fail;
+        HIPCK(hipMemcpy(bt->t_vec, tab, nvec * 2 * sizeof(int), hipMemcpyHostToDevice));
+    }
+    bt->ddevs = (hb_dev *)dalloc((size_t)nprob * sizeof(hb_dev), &rc);
+    if (rc) goto fail;
+    HIPCK(hipHostMalloc((void **)&bt->flag_host, 2 * sizeof(int), hipHostMallocDefault));
+    HIPCK(hipEventCreateWithFlags(&bt->ev[0], hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&bt->ev[1], hipEventDisableTiming));
+    HIPCK(hipEventCreate(&bt->t0));
+    HIPCK(hipEventCreate(&bt->t1));
+    bt->B.nprob = nprob;
+    bt->B.exact = bt->exact;
+    bt->B.devs = bt->ddevs;
+    bt->B.t_spmv = bt->t_spmv;
+    bt->B.t_vec = bt->t_vec;
+    bt->B.n_spmv_tiles = (int)nsp;
+    bt->B.n_vec_tiles = (int)nvec;
+    if ((rc = batch_layout(bt, p->window > 0 ? p->window : HB_WMAX, 0)) != 0) goto fail;
+    free(tab);
+    return bt;
+fail:
+    free(tab);
+    hbicg_batch_destroy(bt);
+    g_errno = rc ? rc : HBICG_E_HIP;
+    return NULL;
+}
+
+static int batch_enqueue_chunk(hbicg_batch *bt)
+{
+    for (int c = 0; c < bt->chunk; ++c) {
+        const int e = hb_kb_iteration(&bt->B, bt->stream);
+        if (e) return e;
+    }
+    return hb_kb_publish(&bt->B, bt->flag_host, bt->stream);
+}
+
+static int batch_capture(hbicg_batch *bt)
+{
+    int rc = 0;
+    if (bt->gexec && !bt->graph_dirty) return 0;
+    if (bt->gexec) { hipGraphExecDestroy(bt->gexec); bt->gexec = NULL; }
+    if (bt->graph_obj) { hipGraphDestroy(bt->graph_obj); bt->graph_obj = NULL; }
+    HIPCK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+    if (batch_enqueue_chunk(bt) != 0) {
+        hipGraph_t g = NULL;
+        hipStreamEndCapture(bt->stream, &g);
+        if (g) hipGraphDestroy(g);
+        return HBICG_E_HIP;
+    }
+    HIPCK(hipStreamEndCapture(bt->stream, &bt->graph_obj));
+    HIPCK(hipGraphInstantiate(&bt->gexec, bt->graph_obj, NULL, NULL, 0));
+    bt->graph_dirty = 0;
+    return 0;
+fail:
+    return rc;
+}
+
+/* Runs the batch with the current layout, max_iter_override > 0 capping
+ * every problem's iterations (tuning runs).  b, x: per-problem pointers,
+ * device memory; sigma: host arrays. */
+static int batch_run(hbicg_batch *bt, const double *const *b, const double *const *sigma,
+                     const int *seed, double *const *x, int max_iter_override)
+{
+    int rc = 0;
+    HIPCK(hipSetDevice(bt->device));
+    int max_cap = 0;
+    for (int p = 0; p < bt->nprob; ++p) {
+        hbicg_plan *pl = bt->plans[p];
+        hb_dev *d = &bt->hdevs[p];
+        if (!b[p] || !sigma[p] || !x[p] || seed[p] < 0 || seed[p] >= pl->M) return HBICG_E_ARG;
+        d->x_set = x[p];
+        d->b_in = b[p];
+        d->seed_in = seed[p];
+        d->cap = max_iter_override > 0 ? max_iter_override + 1 : pl->cap;
+        if (d->cap > pl->cap) d->cap = pl->cap;
+        if (d->cap > max_cap) max_cap = d->cap;
+        HIPCK(hipMemcpyAsync(pl->sigma, sigma[p], (size_t)pl->M * 8, hipMemcpyHostToDevice, bt->stream));
+        HIPCK(hipMemsetAsync(pl->st, 0, sizeof(hb_state), bt->stream));
+    }
+    HIPCK(hipMemcpyAsync(bt->ddevs, bt->hdevs, (size_t)bt->nprob * sizeof(hb_dev), hipMemcpyHostToDevice,
+                         bt->stream));
+    HIPCK(hipEventRecord(bt->t0, bt->stream));
+    if (hb_kb_init(&bt->B, bt->stream) != 0) return HBICG_E_HIP;
+    if (bt->graph && (rc = batch_capture(bt)) != 0) return rc;
+    bt->flag_host[0] = 0;
+    const int max_chunks = max_cap / bt->chunk + 3;
+    int launched = 0, prev = -1;
+    for (;;) {
+        if (bt->graph) {
+            HIPCK(hipGraphLaunch(bt->gexec, bt->stream));
+        } else if (batch_enqueue_chunk(bt) != 0) {
+            return HBICG_E_HIP;
+        }
+        const int cur = launched & 1;
+        HIPCK(hipEventRecord(bt->ev[cur], bt->stream));
+        ++launched;
+        if (prev >= 0) {
+            HIPCK(hipEventSynchronize(bt->ev[prev]));
+            if (__atomic_load_n(&bt->flag_host[0], __ATOMIC_ACQUIRE)) break;
+        }
+        prev = cur;
+        if (launched > max_chunks) break;
+    }
+    if (hb_kb_finalize(&bt->B, bt->stream) != 0) return HBICG_E_HIP;
+    HIPCK(hipEventRecord(bt->t1, bt->stream));
+    HIPCK(hipStreamSynchronize(bt->stream));
+    float ms = 0.0f;
+    HIPCK(hipEventElapsedTime(&ms, bt->t0, bt->t1));
+    bt->last_ms = ms;
+    return 0;
+fail:
+    hipStreamSynchronize(bt->stream);
+    return rc;
+}
+
+int hbicg_batch_solve(hbicg_batch *bt, const double *const *b, const double *const *sigma,
+                      const int *seed, double *const *x, int *k_out, double *solve_ms)
+{
+    int rc = 0;
+    if (!bt || !b || !sigma || !seed || !x) return HBICG_E_ARG;
+    if ((rc = batch_run(bt, b, sigma, seed, x, 0)) != 0) return rc;
+    for (int p = 0; p < bt->nprob; ++p) {
+        hbicg_plan *pl = bt->plans[p];
+        hb_state st;
+        HIPCK(hipMemcpyAsync(&st, pl->st, sizeof(st), hipMemcpyDeviceToHost, bt->stream));
+        HIPCK(hipMemcpyAsync(pl->h_stop_iter, pl->stop_iter, (size_t)pl->M * 4, hipMemcpyDeviceToHost, bt->stream));
+        HIPCK(hipMemcpyAsync(pl->h_switch_k, pl->switch_k, (size_t)pl->M * 4, hipMemcpyDeviceToHost, bt->stream));
+        HIPCK(hipMemcpyAsync(pl->h_switch_to, pl->switch_to, (size_t)pl->M * 4, hipMemcpyDeviceToHost, bt->stream));
+        HIPCK(hipStreamSynchronize(bt->stream));
+        hbicg_report *R = &pl->rep;
+        memset(R, 0, sizeof(*R));
+        R->k = st.k;
+        R->stop_count = st.stop_count;
+        R->n_switch = st.n_switch;
+        R->seed_final = st.seed;
+        R->status = st.status;
+        R->M = pl->M;
+        R->n = pl->n;
+        R->dot_r = st.dot_r;
+        R->dot_zero = st.dot_zero;
+        R->rTr = st.rTr;
+        R->t_solve_ms = bt->last_ms;
+        if (k_out) k_out[p] = st.k;
+    }
+    if (solve_ms) *solve_ms = bt->last_ms;
+    return 0;
+fail:
+    return rc;
+}
+
+int hbicg_batch_report(hbicg_batch *bt, int p, hbicg_report *rep, int *stop_iter,
+                       int *switch_k, int *switch_to, int max_switch)
+{
+    if (!bt || p < 0 || p >= bt->nprob) return HBICG_E_ARG;
+    return hbicg_plan_report(bt->plans[p], rep, stop_iter, switch_k, switch_to, max_switch);
+}
+
+int hbicg_batch_true_residuals(hbicg_batch *bt, int p, const double *b, const double *sigma,
+                               const double *x_set, double *out)
+{
+    if (!bt || p < 0 || p >= bt->nprob) return HBICG_E_ARG;
+    return hbicg_plan_true_residuals(bt->plans[p], b, sigma, x_set, 1, out);
+}
+
+/* Candidate layouts are timed on the batch itself for `iters` iterations
+ * (x is overwritten); the fastest per-iteration layout is kept. */
+int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *const *sigma,
+                         const int *seed, double *const *x, int iters, hbicg_tuning *best)
+{
+    static const int windows[] = {4, 8, 16};
+    static const int groups[] = {0, 16, 4};
+    int rc = 0;
+    if (!bt || iters < 1) return HBICG_E_ARG;
+    double best_ms = 1e300;
+    int bw = bt->W, bg = bt->group_cap;
+    for (size_t wi = 0; wi < sizeof(windows) / sizeof(windows[0]); ++wi)
+        for (size_t gi = 0; gi < sizeof(groups) / sizeof(groups[0]); ++gi) {
+            if ((rc = batch_layout(bt, windows[wi], groups[gi])) != 0) return rc;
+            double ms = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {   /* first run warms the graph */
+                if ((rc = batch_run(bt, b, sigma, seed, x, iters)) != 0) return rc;
+                ms = bt->last_ms;
+            }
+            if (ms < best_ms) { best_ms = ms; bw = windows[wi]; bg = groups[gi]; }
+        }
+    if ((rc = batch_layout(bt, bw, bg)) != 0) return rc;
+    if (best) {
+        best->window = bw;
+        best->group_cap = bg;
+        best->ms_per_iter = best_ms / iters;
+    }
+    return 0;
 }

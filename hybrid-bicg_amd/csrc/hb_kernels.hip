@@ -2,31 +2,35 @@
  * hb_kernels.hip -- gfx950 kernels of the shifted BiCG solver with seed
  * switching, and the C-callable launch layer declared in hb_launch.h.
  *
- * One iteration of the reference loop (src/solver_dynamic.c:147-379) is four
- * kernels that keep every scalar on the device:
- *   k_spmv<SEED1>  s = (A + sigma_s I) p_s, (r^,s) -> alpha~[k]      (:156-167)
- *   k_spmv<SEED2>  q = r - alpha~ s fused into y = (A + sigma_s I) q,
- *                  (q,q), (q,y) -> omega~[k]                           (:171-189)
- *   k_seed_update  x_s, r update, (r,r), (r^,r) -> beta~[k]; then the
- *                  scalar stage: per-shift recurrences, convergence
- *                  check, seed switch with archive replay              (:194-204,
+ * One iteration of the reference loop (src/solver_dynamic.c:147-379) is
+ * seven launches that keep every scalar on the device:
+ *   spmv<SEED1> + fin   s = (A + sigma_s I) p_s; (r^,s) -> alpha~[k]   (:156-167)
+ *   spmv<SEED2> + fin   q = r - alpha~ s fused into y = (A + sigma_s I) q;
+ *                       (q,q), (q,y) -> omega~[k]                      (:171-190)
+ *   seed_update + fin   x_s and r updates, (r,r), (r^,r) -> beta~[k];
+ *                       then the scalar stage: per-shift recurrences,
+ *                       convergence check, seed switch with replay     (:194-204,
  *                                                                       :239-359)
- *   k_shift        seed p update and the windowed x_j / p_j update of
- *                  every active shift                                  (:208-210,
+ *   shift               seed p update, and every W iterations the
+ *                       deferred x_j / p_j updates of all shifts       (:208-210,
  *                                                                       :248-255)
  * Per-shift vector updates are deferred over a window of up to W iterations
  * and applied in one pass that reads and writes x_j and p_j once, replaying
  * the reference's six per-iteration vector operations in their original
- * order, so x_j and p_j are bit-identical to the per-iteration updates.
+ * order, so x_j and p_j are bit-identical to per-iteration updates.
+ *
+ * Every kernel body is a device function of (problem, tile).  The plain
+ * solver launches one problem's tiles; the batched solver launches the
+ * tiles of all its problems in the same launch, looked up through a
+ * tile -> (problem, tile) table, so each phase is one launch per batch.
  *
  * Every floating-point operation is rounded separately (the library is
  * built with -ffp-contract=off) so that each elementwise step reproduces the
- * reference's own sequence of roundings.
- * Reductions: workgroup partials, combined in a fixed order by a one-
- * workgroup finalize kernel (deterministic).  With d.exact set, every dot is
- * summed serially from 0.0 in ascending index order and every SpMV row
- * sequentially, the reference's order (src/openmp_vector.c:68-74,
- * src/openmp_matrix.c:47-58).
+ * reference's own sequence of roundings.  Reductions: workgroup partials,
+ * combined in a fixed order by a one-workgroup finalize (deterministic).
+ * With d.exact set, every dot is summed serially from 0.0 in ascending index
+ * order and every SpMV row sequentially, the reference's order
+ * (src/openmp_vector.c:68-74, src/openmp_matrix.c:47-58).
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -35,11 +39,13 @@
 
 #define DEV __device__ __forceinline__
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 /* ---- reductions ----------------------------------------------------------- */
 DEV double wave_sum(double v)
 {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
@@ -60,11 +66,9 @@ DEV void block_sum2(double &a, double &b)
     __syncthreads();
 }
 
-/* Combines per-workgroup partial pairs in a fixed order (thread 0 result). */
-typedef double dbl2 __attribute__((ext_vector_type(2)));
-
-/* Eight 16-byte partial pairs in flight per thread before any is added, so
- * the combine costs about one memory latency, not one per partial. */
+/* Combines per-workgroup partial pairs in a fixed order (thread 0 result).
+ * Eight 16-byte pairs are in flight per thread before any is added, so the
+ * combine costs about one memory latency, not one per partial. */
 DEV void reduce_partials(const double *part, int nb, double &a, double &b)
 {
     const dbl2 *p2 = (const dbl2 *)part;
@@ -106,31 +110,26 @@ DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
 #define PI(d, j, i) ((d).pi[(size_t)(j) * (size_t)(d).cap + (size_t)(i)])
 
 /* ---- SpMV ------------------------------------------------------------------
- * Workgroup b owns rows [blk[b], blk[b+1]).  When their nonzeros fit in LDS
+ * Tile b owns rows [blk[b], blk[b+1]).  When their nonzeros fit in LDS
  * (<= HB_NNZ_CAP, <= HB_ROWS_CAP rows), the workgroup loads val/col
  * coalesced, gathers x, stores the products in LDS and reduces each row with
  * T lanes, T the largest power of two with T*rows <= 256 (at most 64).
- * Stencil rows give 256 rows per workgroup and T = 1: each row is summed
+ * Stencil rows give 256 rows per tile and T = 1: each row is summed
  * sequentially in CSR order.  A row longer than HB_NNZ_CAP gets the whole
- * workgroup.  exact forces T = 1 and sequential long rows. */
+ * workgroup.  exact forces T = 1 and sequential long rows.
+ * The arrays come in as __restrict__ parameters (kept as noalias scopes when
+ * inlined): the compiler keeps several val/col/x loads in flight and reads
+ * uniform values through scalar loads.  SEED1 gathers p_set[seed], SEED2
+ * gathers r_{k-1} (in rring) and s, PLAIN gathers x. */
 enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 
-/* The SpMV takes its arrays as __restrict__ arguments: the compiler then
- * keeps several val/col/x loads in flight and reads uniform values through
- * scalar loads.  SEED1 gathers p_set[seed], SEED2 gathers r_{k-1} (in
- * rring) and s, PLAIN gathers x. */
 template <int MODE, bool EXACT>
-__global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, const double *__restrict__ val,
-                                                   const unsigned *__restrict__ col,
-                                                   const unsigned *__restrict__ ptr,
-                                                   const unsigned *__restrict__ blk,
-                                                   const double *__restrict__ xbase,
-                                                   double *__restrict__ rring,
-                                                   const double *__restrict__ svec,
-                                                   const double *__restrict__ rhat,
-                                                   double *__restrict__ out1,
-                                                   double *__restrict__ qring,
-                                                   double *__restrict__ part, double shift)
+DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
+                   const unsigned *__restrict__ col, const unsigned *__restrict__ ptr,
+                   const unsigned *__restrict__ blk, const double *__restrict__ xbase,
+                   double *__restrict__ rring, const double *__restrict__ svec,
+                   const double *__restrict__ rhat, double *__restrict__ out1,
+                   double *__restrict__ qring, double *__restrict__ part, double shift)
 {
     __shared__ double prod[HB_NNZ_CAP];
     __shared__ unsigned rp[HB_ROWS_CAP + 1];
@@ -143,7 +142,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, const double *__res
         hb_state *st = d.st;
         /* the shift kernel runs once per scalar stage: it compares k with
          * the epoch written here, after the previous shift kernel finished */
-        if (MODE == SP_SEED1 && blockIdx.x == 0 && threadIdx.x == 0) st->k4_epoch = st->k;
+        if (MODE == SP_SEED1 && b == 0 && threadIdx.x == 0) st->k4_epoch = st->k;
         if (st->done) return;
         const int k = st->k, seed = st->seed;
         sig = d.sigma[seed];
@@ -166,7 +165,6 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, const double *__res
         return xa[c];
     };
 
-    const unsigned b = blockIdx.x;
     const unsigned r0 = blk[b], r1 = blk[b + 1];
     const int R = (int)(r1 - r0);
     const unsigned e0 = ptr[r0];
@@ -236,36 +234,44 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d, const double *__res
 }
 
 template <int MODE, bool EXACT>
-static void launch_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s)
+DEV void spmv_tile(const hb_dev &d, unsigned b)
 {
-    const dim3 g(d->nblk), b(HB_BLOCK);
     if (MODE == SP_SEED1)
-        hipLaunchKernelGGL((k_spmv<MODE, EXACT>), g, b, 0, s, *d, d->val, d->col, d->ptr, d->blk,
-                           (const double *)d->p_set, d->rring, (const double *)nullptr, d->rhat,
-                           d->s, (double *)nullptr, d->part, 0.0);
-    else if (MODE == SP_SEED2)
-        hipLaunchKernelGGL((k_spmv<MODE, EXACT>), g, b, 0, s, *d, d->val, d->col, d->ptr, d->blk,
-                           (const double *)nullptr, d->rring, (const double *)d->s, d->rhat,
-                           d->y, d->qring, d->part, 0.0);
+        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.ptr, d.blk, d.p_set, d.rring, nullptr,
+                               d.rhat, d.s, nullptr, d.part, 0.0);
     else
-        hipLaunchKernelGGL((k_spmv<MODE, EXACT>), g, b, 0, s, *d, d->val, d->col, d->ptr, d->blk,
-                           x, (double *)nullptr, (const double *)nullptr, d->rhat, y,
-                           (double *)nullptr, (double *)nullptr, shift);
+        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
+                               d.y, d.qring, d.part, 0.0);
 }
 
-/* One workgroup: combines the SpMV partials and forms alpha~[k] (after SpMV
- * 1, src/solver_dynamic.c:163-167, with the post-switch (r^,r) of :323) or
- * omega~[k] (after SpMV 2, :183-190).  A separate launch instead of a
- * last-workgroup-to-arrive tail: on gfx950 every agent-scope release is an
- * L2 write-back, far too costly to pay once per SpMV workgroup. */
 template <int MODE, bool EXACT>
-__global__ __launch_bounds__(HB_BLOCK) void k_fin_spmv(hb_dev d)
+__global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
+{
+    spmv_tile<MODE, EXACT>(d, blockIdx.x);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_spmv_plain(hb_dev d, const double *x, double *y,
+                                                         double shift)
+{
+    spmv_body<SP_PLAIN, EXACT>(d, blockIdx.x, d.val, d.col, d.ptr, d.blk, x, nullptr, nullptr,
+                               nullptr, y, nullptr, nullptr, shift);
+}
+
+/* Combines the SpMV partials and forms alpha~[k] (after SpMV 1,
+ * src/solver_dynamic.c:163-167, with the post-switch (r^,r) of :323) or
+ * omega~[k] (after SpMV 2, :183-190).  One workgroup per problem: on gfx950
+ * an agent-scope release is an L2 write-back, far too costly to pay once per
+ * SpMV workgroup for a last-to-arrive tail. */
+template <int MODE, bool EXACT>
+DEV void fin_spmv_body(const hb_dev &d)
 {
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     hb_state *st = d.st;
     if (st->done) return;
     const int k = st->k, tid = threadIdx.x;
     const int switched = (MODE == SP_SEED1) ? st->switched : 0;
+    const double rTr = st->rTr;
     double a0, a1;
     if (EXACT) {
         const double *rprev = d.rring + off((k - 1) % (d.W + 1), d.n);
@@ -282,23 +288,30 @@ __global__ __launch_bounds__(HB_BLOCK) void k_fin_spmv(hb_dev d)
     }
     if (tid == 0) {
         if (MODE == SP_SEED1) {
+            const double rtr = switched ? a1 : rTr;
+            const double alpha = rtr / a0;
             if (switched) { st->rTr = a1; st->switched = 0; }
             st->rTs = a0;
-            const double alpha = st->rTr / a0;
             d.aa[k] = alpha;
             if (a0 == 0.0 || !isfinite(alpha)) st->status |= HB_ST_BREAKDOWN_RTS;
         } else {
+            const double omega = a0 / a1;
             st->qTq = a0;
             st->qTy = a1;
-            const double omega = a0 / a1;
             d.wa[k] = omega;
-            st->rTr_old = st->rTr;
+            st->rTr_old = rTr;
             if (a1 == 0.0 || !isfinite(omega)) st->status |= HB_ST_BREAKDOWN_QTY;
         }
     }
 }
 
-/* ---- scalar stage (one workgroup, after k_seed_update) ---------------------
+template <int MODE, bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_fin_spmv(hb_dev d)
+{
+    fin_spmv_body<MODE, EXACT>(d);
+}
+
+/* ---- scalar stage (one workgroup per problem, after the seed update) -------
  * src/solver_dynamic.c:204 (beta~), :243-253 (per-shift recurrences),
  * :278-303 (convergence check), :312-341 (seed switch). */
 DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
@@ -462,7 +475,6 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         __syncthreads();
     }
 
-    __shared__ int sh_flush;
     const int len = slot + 1;
     if (tid == 0) {
         const int done_next = (st->stop_count >= M) || (k + 1 >= d.cap) ||
@@ -477,7 +489,6 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             st->switched = 1;
         }
         const int fl = do_switch || done_next || len >= d.W;
-        sh_flush = fl;
         st->flush = fl;
         if (fl) {
             st->flush_k0 = win_k0;
@@ -486,29 +497,24 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         }
         st->k = k + 1;
         st->done = done_next;
-    }
-    if (tid == 0) {
         /* seed p update: p, r, s in, p out; at a flush also the window
          * vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1} and x_j, p_j of every
          * listed shift, read and written once */
         double bytes = 32.0;
-        if (sh_flush) bytes += 8.0 * (2 * len + 1) + 32.0 * st->flush_cnt;
+        if (fl) bytes += 8.0 * (2 * len + 1) + 32.0 * st->flush_cnt;
         st->shift_bytes += bytes * (double)d.n;
     }
 }
 
 /* ---- seed update: x_s, r, (r,r), (r^,r) (src/solver_dynamic.c:194-204) --- */
-__global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d, const double *__restrict__ qring,
-                                                          const double *__restrict__ y,
-                                                          const double *__restrict__ pset,
-                                                          double *__restrict__ xset,
-                                                          const double *__restrict__ rhat,
-                                                          double *__restrict__ rring,
-                                                          double *__restrict__ part)
+DEV void seed_update_body(const hb_dev &d, int b, const double *__restrict__ qring,
+                          const double *__restrict__ y, const double *__restrict__ pset,
+                          double *__restrict__ xset, const double *__restrict__ rhat,
+                          double *__restrict__ rring, double *__restrict__ part)
 {
     const hb_state *st = d.st;
     if (st->done) return;
-    const int k = st->k, s = st->seed, n = d.n;
+    const int k = st->k, s = st->seed, n = d.n, nb = d.vec_grid;
     const double a = d.aa[k], w = d.wa[k], nw = -d.wa[k];
     const double *q = qring + off(k % d.W, n);
     double *rn = rring + off(k % (d.W + 1), n);
@@ -516,7 +522,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d, const double
     double *xs = xset + off(s, n);
     double part0 = 0.0, part1 = 0.0;
 #pragma unroll 2
-    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < n; i += gridDim.x * HB_BLOCK) {
+    for (int i = b * HB_BLOCK + threadIdx.x; i < n; i += nb * HB_BLOCK) {
         const double qi = q[i];
         double xi = xs[i];
         xi = xi + a * ps[i];
@@ -528,17 +534,26 @@ __global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d, const double
         part1 += rhat[i] * ri;
     }
     block_sum2(part0, part1);
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = part0; part[2 * blockIdx.x + 1] = part1; }
+    if (threadIdx.x == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
 }
 
-/* One workgroup: (r,r), (r^,r) from the seed-update partials, then the
- * scalar stage. */
+DEV void seed_update_tile(const hb_dev &d, int b)
+{
+    seed_update_body(d, b, d.qring, d.y, d.p_set, d.x_set, d.rhat, d.rring, d.part);
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d)
+{
+    seed_update_tile(d, blockIdx.x);
+}
+
+/* (r,r), (r^,r) from the seed-update partials, then the scalar stage. */
 template <bool EXACT>
-__global__ __launch_bounds__(HB_BLOCK) void k_fin_seed(hb_dev d)
+DEV void fin_seed_body(const hb_dev &d)
 {
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     __shared__ double sh[2];
-    hb_state *st = d.st;
+    const hb_state *st = d.st;
     if (st->done) return;
     double dr, rr;
     if (EXACT) {
@@ -553,8 +568,14 @@ __global__ __launch_bounds__(HB_BLOCK) void k_fin_seed(hb_dev d)
     scalar_stage(d, sh[0], sh[1]);
 }
 
-/* V consecutive doubles; V = 2 moves them with one 16-byte access. */
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_fin_seed(hb_dev d)
+{
+    fin_seed_body<EXACT>(d);
+}
 
+/* ---- shift window ------------------------------------------------------------ */
+/* V consecutive doubles; V = 2 moves them with one 16-byte access. */
 template <int V>
 struct Vd {
     double v[V];
@@ -609,33 +630,30 @@ DEV void shift_step(Vd<V> &x, Vd<V> &p, const double *c, const Vd<V> &q, const V
     }
 }
 
-/* ---- shift window kernel ---------------------------------------------------
- * Each thread owns V consecutive elements.  Row 0 of the grid applies the
- * seed p update of the finished iteration (src/solver_dynamic.c:208-210).
- * When a flush is due, workgroup row y applies the pending window to shifts
- * flist[y*G .. y*G+G): the window's r and q values of the thread's elements
- * stay in registers, x_j and p_j are read and written once per flush (U
- * shifts in flight per thread), and every window iteration replays the six
- * vector operations of src/solver_dynamic.c:248-255 in their order.
- * The pointer arguments are __restrict__ so that the per-shift
- * coefficients, uniform across a wavefront, come through scalar loads. */
+/* Tile (bx, by): each thread owns V consecutive elements.  Row by == 0
+ * applies the seed p update of the finished iteration
+ * (src/solver_dynamic.c:208-210).  When a flush is due, row by applies the
+ * pending window to the listed shifts by*G .. by*G+G-1: the window's r and q
+ * values of the thread's elements stay in registers, x_j and p_j are read
+ * and written once per flush (U shifts in flight per thread), and every
+ * window iteration replays the six vector operations of
+ * src/solver_dynamic.c:248-255 in their order.  The pointer parameters are
+ * __restrict__ so that the coefficients, uniform across a wavefront, come
+ * through scalar loads. */
 template <int WM, int V, int U>
-__global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d, double *__restrict__ xset,
-                                                    double *__restrict__ pset,
-                                                    const double *__restrict__ rring,
-                                                    const double *__restrict__ qring,
-                                                    const double *__restrict__ svec,
-                                                    const double *__restrict__ coef,
-                                                    const int *__restrict__ flist,
-                                                    const int *__restrict__ act_len)
+DEV void shift_body(const hb_dev &d, int bx, int by, double *__restrict__ xset,
+                    double *__restrict__ pset, const double *__restrict__ rring,
+                    const double *__restrict__ qring, const double *__restrict__ svec,
+                    const double *__restrict__ coef, const int *__restrict__ flist,
+                    const int *__restrict__ act_len)
 {
     const hb_state *st = d.st;
     if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     const int n = d.n, W = d.W, M = d.M;
-    const int i = (blockIdx.x * HB_BLOCK + threadIdx.x) * V;
+    const int i = (bx * HB_BLOCK + threadIdx.x) * V;
     if (i >= n) return;
     const int kp = st->kprev;
-    if (blockIdx.y == 0) {
+    if (by == 0) {
         double *ps = pset + off(st->prev_seed, n) + i;
         const double pb = st->pb, pbw = st->pbw;
         Vd<V> p = vload<V, false>(ps);
@@ -661,7 +679,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d, double *__restrict
         }
     }
     const int G = d.shift_group;
-    const int j0 = blockIdx.y * G;
+    const int j0 = by * G;
     const int j1 = min(cnt, j0 + G);
     int jj = j0;
     for (; jj + U <= j1; jj += U) {
@@ -713,13 +731,27 @@ __global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d, double *__restrict
     }
 }
 
-/* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
-__global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b)
+template <int WM, int V, int U>
+DEV void shift_tile(const hb_dev &d, int bx, int by)
 {
-    const int n = d.n, M = d.M;
+    shift_body<WM, V, U>(d, bx, by, d.x_set, d.p_set, d.rring, d.qring, d.s, d.coef, d.flist,
+                         d.act_len);
+}
+
+template <int WM, int V, int U>
+__global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d)
+{
+    shift_tile<WM, V, U>(d, blockIdx.x, blockIdx.y);
+}
+
+/* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
+DEV void init_body(const hb_dev &d, int blk)
+{
+    const int n = d.n, M = d.M, nb = d.vec_grid;
+    const double *b = d.b_in;
     double *rhat = (double *)d.rhat;
     double part0 = 0.0, part1 = 0.0;
-    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < n; i += gridDim.x * HB_BLOCK) {
+    for (int i = blk * HB_BLOCK + threadIdx.x; i < n; i += nb * HB_BLOCK) {
         const double bi = b[i];
         rhat[i] = bi;
         d.rring[i] = bi;
@@ -727,17 +759,22 @@ __global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d, const double *b)
         part0 += bi * bi;
     }
     block_sum2(part0, part1);
-    if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = part0; d.part[2 * blockIdx.x + 1] = 0.0; }
+    if (threadIdx.x == 0) { d.part[2 * blk] = part0; d.part[2 * blk + 1] = 0.0; }
 }
 
-/* One workgroup: (b,b) and the scalar initialisation. */
+__global__ __launch_bounds__(HB_BLOCK) void k_init(hb_dev d)
+{
+    init_body(d, blockIdx.x);
+}
+
+/* (b,b) and the scalar initialisation. */
 template <bool EXACT>
-__global__ __launch_bounds__(HB_BLOCK) void k_init_fin(hb_dev d, const double *b, int seed)
+DEV void init_fin_body(const hb_dev &d)
 {
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     const int M = d.M;
     double bb, unused;
-    if (EXACT) bb = serial_dot(b, b, d.n, lds);
+    if (EXACT) bb = serial_dot(d.b_in, d.b_in, d.n, lds);
     else reduce_partials(d.part, d.vec_grid, bb, unused);
     for (int j = threadIdx.x; j < M; j += HB_BLOCK) {
         d.eta[j] = 0.0;
@@ -759,8 +796,8 @@ __global__ __launch_bounds__(HB_BLOCK) void k_init_fin(hb_dev d, const double *b
         st->dot_zero = bb;
         st->k = 1;
         st->kprev = 0;
-        st->seed = seed;
-        st->prev_seed = seed;
+        st->seed = d.seed_in;
+        st->prev_seed = d.seed_in;
         st->stop_count = 0;
         st->max_sigma = -1;
         st->done = (d.cap <= 1);
@@ -771,7 +808,30 @@ __global__ __launch_bounds__(HB_BLOCK) void k_init_fin(hb_dev d, const double *b
         st->k4_epoch = 1;
         st->status = 0;
         st->zscale = 1.0;
+        st->shift_bytes = 0.0;
     }
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void k_init_fin(hb_dev d)
+{
+    init_fin_body<EXACT>(d);
+}
+
+/* A switch in the last iteration leaves r unscaled; apply it as the
+ * reference does before returning r (src/solver_dynamic.c:322). */
+DEV void finalize_body(const hb_dev &d, int blk)
+{
+    const hb_state *st = d.st;
+    if (!st->switched) return;
+    double *r = d.rring + off(st->kprev % (d.W + 1), d.n);
+    const double c = st->zscale;
+    for (int i = blk * HB_BLOCK + threadIdx.x; i < d.n; i += d.vec_grid * HB_BLOCK) r[i] = r[i] * c;
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void k_finalize(hb_dev d)
+{
+    finalize_body(d, blockIdx.x);
 }
 
 __global__ void k_publish(const hb_state *st, int *flag)
@@ -781,17 +841,6 @@ __global__ void k_publish(const hb_state *st, int *flag)
         flag[1] = st->k;
         __threadfence_system();
     }
-}
-
-/* A switch in the last iteration leaves r unscaled; apply it as the
- * reference does before returning r (src/solver_dynamic.c:322). */
-__global__ __launch_bounds__(HB_BLOCK) void k_finalize(hb_dev d)
-{
-    const hb_state *st = d.st;
-    if (!st->switched) return;
-    double *r = d.rring + off(st->kprev % (d.W + 1), d.n);
-    const double c = st->zscale;
-    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < d.n; i += gridDim.x * HB_BLOCK) r[i] = r[i] * c;
 }
 
 /* ---- true residual of every shift ---------------------------------------- */
@@ -823,54 +872,142 @@ __global__ __launch_bounds__(HB_BLOCK) void k_resid_final(const double *part, in
     if (threadIdx.x == 0) out[blockIdx.x] = sqrt(a) / sqrt(b);
 }
 
+/* ---- batched launches --------------------------------------------------------
+ * Launch tile t of a phase works on problem tab[2t] tile tab[2t+1]; finalize
+ * launches use workgroup p for problem p.  Problems that are done return at
+ * once inside the bodies. */
+template <int MODE, bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void kb_spmv(const hb_dev *__restrict__ devs,
+                                                    const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    spmv_tile<MODE, EXACT>(d, tab[2 * blockIdx.x + 1]);
+}
+
+template <int MODE, bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void kb_fin_spmv(const hb_dev *__restrict__ devs)
+{
+    const hb_dev &d = devs[blockIdx.x];
+    fin_spmv_body<MODE, EXACT>(d);
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void kb_seed_update(const hb_dev *__restrict__ devs,
+                                                           const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    seed_update_tile(d, tab[2 * blockIdx.x + 1]);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void kb_fin_seed(const hb_dev *__restrict__ devs)
+{
+    const hb_dev &d = devs[blockIdx.x];
+    fin_seed_body<EXACT>(d);
+}
+
+/* shift tiles: tab[2t+1] = by * gx + bx of the problem's (gx, gy) grid */
+template <int WM, int V, int U>
+__global__ __launch_bounds__(HB_BLOCK) void kb_shift(const hb_dev *__restrict__ devs,
+                                                     const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    const int t = tab[2 * blockIdx.x + 1];
+    shift_tile<WM, V, U>(d, t % d.shift_gx, t / d.shift_gx);
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void kb_init(const hb_dev *__restrict__ devs,
+                                                    const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    init_body(d, tab[2 * blockIdx.x + 1]);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(HB_BLOCK) void kb_init_fin(const hb_dev *__restrict__ devs)
+{
+    const hb_dev &d = devs[blockIdx.x];
+    init_fin_body<EXACT>(d);
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void kb_finalize(const hb_dev *__restrict__ devs,
+                                                        const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    finalize_body(d, tab[2 * blockIdx.x + 1]);
+}
+
+/* flag[0] = 1 once every problem is done; flag[1] = problems still running */
+__global__ __launch_bounds__(HB_BLOCK) void kb_publish(const hb_dev *__restrict__ devs, int nprob,
+                                                       int *flag)
+{
+    __shared__ int cnt[HB_BLOCK];
+    int c = 0;
+    for (int p = threadIdx.x; p < nprob; p += HB_BLOCK) c += devs[p].st->done ? 0 : 1;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = HB_BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) cnt[threadIdx.x] += cnt[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        flag[0] = cnt[0] == 0;
+        flag[1] = cnt[0];
+        __threadfence_system();
+    }
+}
+
 /* ---- launch layer --------------------------------------------------------- */
 template <int WM, int V, int U>
 static void launch_shift(const hb_dev *d, hipStream_t s)
 {
     const int per = HB_BLOCK * V;
     const dim3 g((d->n + per - 1) / per, d->shift_gy), b(HB_BLOCK);
-    hipLaunchKernelGGL((k_shift<WM, V, U>), g, b, 0, s, *d, d->x_set, d->p_set, d->rring,
-                       d->qring, d->s, d->coef, d->flist, d->act_len);
+    hipLaunchKernelGGL((k_shift<WM, V, U>), g, b, 0, s, *d);
+}
+
+template <int WM>
+static void launch_shift_b(const hb_batch *B, hipStream_t s)
+{
+    hipLaunchKernelGGL((kb_shift<WM, 1, 4>), dim3(B->n_shift_tiles), dim3(HB_BLOCK), 0, s,
+                       B->devs, B->t_shift);
 }
 
 extern "C" {
 
-int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s)
+int hb_k_init(const hb_dev *d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d, b);
-    if (d->exact) hipLaunchKernelGGL(k_init_fin<true>, dim3(1), dim3(HB_BLOCK), 0, s, *d, b, seed);
-    else hipLaunchKernelGGL(k_init_fin<false>, dim3(1), dim3(HB_BLOCK), 0, s, *d, b, seed);
+    hipLaunchKernelGGL(k_init, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d);
+    if (d->exact) hipLaunchKernelGGL(k_init_fin<true>, dim3(1), dim3(HB_BLOCK), 0, s, *d);
+    else hipLaunchKernelGGL(k_init_fin<false>, dim3(1), dim3(HB_BLOCK), 0, s, *d);
     return (int)hipGetLastError();
 }
 
 int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
 {
-    const dim3 gv(d->vec_grid), bl(HB_BLOCK);
+    const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK), one(1);
     switch (stage) {
     case HB_STAGE_SPMV1:
         if (d->exact) {
-            launch_spmv<SP_SEED1, true>(d, nullptr, nullptr, 0.0, s);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), dim3(1), bl, 0, s, *d);
+            hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), one, bl, 0, s, *d);
         } else {
-            launch_spmv<SP_SEED1, false>(d, nullptr, nullptr, 0.0, s);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), dim3(1), bl, 0, s, *d);
+            hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), one, bl, 0, s, *d);
         }
         break;
     case HB_STAGE_SPMV2:
         if (d->exact) {
-            launch_spmv<SP_SEED2, true>(d, nullptr, nullptr, 0.0, s);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), dim3(1), bl, 0, s, *d);
+            hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), one, bl, 0, s, *d);
         } else {
-            launch_spmv<SP_SEED2, false>(d, nullptr, nullptr, 0.0, s);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, false>), dim3(1), bl, 0, s, *d);
+            hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d);
+            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, false>), one, bl, 0, s, *d);
         }
         break;
     case HB_STAGE_SEED:
-        hipLaunchKernelGGL(k_seed_update, gv, bl, 0, s, *d, (const double *)d->qring,
-                           (const double *)d->y, (const double *)d->p_set, d->x_set, d->rhat,
-                           d->rring, d->part);
-        if (d->exact) hipLaunchKernelGGL(k_fin_seed<true>, dim3(1), bl, 0, s, *d);
-        else hipLaunchKernelGGL(k_fin_seed<false>, dim3(1), bl, 0, s, *d);
+        hipLaunchKernelGGL(k_seed_update, gv, bl, 0, s, *d);
+        if (d->exact) hipLaunchKernelGGL(k_fin_seed<true>, one, bl, 0, s, *d);
+        else hipLaunchKernelGGL(k_fin_seed<false>, one, bl, 0, s, *d);
         break;
     default:
         /* windows up to 4 move 16-byte pairs; longer windows hold one
@@ -923,8 +1060,75 @@ int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *o
 
 int hb_k_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s)
 {
-    if (d->exact) launch_spmv<SP_PLAIN, true>(d, x, y, shift, s);
-    else launch_spmv<SP_PLAIN, false>(d, x, y, shift, s);
+    if (d->exact) hipLaunchKernelGGL(k_spmv_plain<true>, dim3(d->nblk), dim3(HB_BLOCK), 0, s, *d, x, y, shift);
+    else hipLaunchKernelGGL(k_spmv_plain<false>, dim3(d->nblk), dim3(HB_BLOCK), 0, s, *d, x, y, shift);
+    return (int)hipGetLastError();
+}
+
+/* ---- batched ---- */
+int hb_kb_init(const hb_batch *B, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_init, dim3(B->n_vec_tiles), dim3(HB_BLOCK), 0, s, B->devs, B->t_vec);
+    if (B->exact) hipLaunchKernelGGL(kb_init_fin<true>, dim3(B->nprob), dim3(HB_BLOCK), 0, s, B->devs);
+    else hipLaunchKernelGGL(kb_init_fin<false>, dim3(B->nprob), dim3(HB_BLOCK), 0, s, B->devs);
+    return (int)hipGetLastError();
+}
+
+int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s)
+{
+    const dim3 bl(HB_BLOCK), gp(B->nprob);
+    switch (stage) {
+    case HB_STAGE_SPMV1:
+        if (B->exact) {
+            hipLaunchKernelGGL((kb_spmv<SP_SEED1, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, true>), gp, bl, 0, s, B->devs);
+        } else {
+            hipLaunchKernelGGL((kb_spmv<SP_SEED1, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, false>), gp, bl, 0, s, B->devs);
+        }
+        break;
+    case HB_STAGE_SPMV2:
+        if (B->exact) {
+            hipLaunchKernelGGL((kb_spmv<SP_SEED2, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, true>), gp, bl, 0, s, B->devs);
+        } else {
+            hipLaunchKernelGGL((kb_spmv<SP_SEED2, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, false>), gp, bl, 0, s, B->devs);
+        }
+        break;
+    case HB_STAGE_SEED:
+        hipLaunchKernelGGL(kb_seed_update, dim3(B->n_vec_tiles), bl, 0, s, B->devs, B->t_vec);
+        if (B->exact) hipLaunchKernelGGL(kb_fin_seed<true>, gp, bl, 0, s, B->devs);
+        else hipLaunchKernelGGL(kb_fin_seed<false>, gp, bl, 0, s, B->devs);
+        break;
+    default:
+        if (B->W <= 1) launch_shift_b<1>(B, s);
+        else if (B->W <= 4) launch_shift_b<4>(B, s);
+        else if (B->W <= 8) launch_shift_b<8>(B, s);
+        else launch_shift_b<16>(B, s);
+        break;
+    }
+    return (int)hipGetLastError();
+}
+
+int hb_kb_iteration(const hb_batch *B, hipStream_t s)
+{
+    for (int st = 0; st < HB_NSTAGE; ++st) {
+        const int e = hb_kb_stage(B, st, s);
+        if (e) return e;
+    }
+    return 0;
+}
+
+int hb_kb_publish(const hb_batch *B, int *flag_dev, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_publish, dim3(1), dim3(HB_BLOCK), 0, s, B->devs, B->nprob, flag_dev);
+    return (int)hipGetLastError();
+}
+
+int hb_kb_finalize(const hb_batch *B, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_finalize, dim3(B->n_vec_tiles), dim3(HB_BLOCK), 0, s, B->devs, B->t_vec);
     return (int)hipGetLastError();
 }
 

@@ -13,9 +13,10 @@
 extern "C" {
 #endif
 
-/* r^ = r_0 = p_j = b for every shift, the reference's scalar initialisation
- * (src/solver_dynamic.c:101-123); (b,b) reduced on the device. */
-int hb_k_init(const hb_dev *d, const double *b, int seed, hipStream_t s);
+/* r^ = r_0 = p_j = b (d->b_in) for every shift and the reference's scalar
+ * initialisation with seed d->seed_in (src/solver_dynamic.c:101-123);
+ * (b,b) reduced on the device. */
+int hb_k_init(const hb_dev *d, hipStream_t s);
 
 /* One iteration: seed SpMV 1, fused q + SpMV 2, seed update with the scalar
  * stage (shift recurrences, convergence check, seed switch), and the shift
@@ -42,6 +43,14 @@ int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *o
 
 /* y = A x + shift*x through the row-binned SpMV (exact: row-sequential sums). */
 int hb_k_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s);
+
+/* Batched counterparts: each launch covers the tiles of every problem of
+ * the batch; done problems return at once. */
+int hb_kb_init(const hb_batch *B, hipStream_t s);
+int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s);
+int hb_kb_iteration(const hb_batch *B, hipStream_t s);
+int hb_kb_publish(const hb_batch *B, int *flag_dev, hipStream_t s);
+int hb_kb_finalize(const hb_batch *B, hipStream_t s);
 
 #ifdef __cplusplus
 }

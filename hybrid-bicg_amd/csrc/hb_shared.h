@@ -85,7 +85,22 @@ typedef struct hb_dev {
     int    *switch_k, *switch_to; /* [M] switch log                        */
     double *part;              /* workgroup partial sums [max(nblk,vec_grid)*2] */
     hb_state *st;
+    const double *b_in;        /* right-hand side read by the initialisation */
+    int seed_in;               /* first seed                               */
+    int pad_;
 } hb_dev;
+
+/* A batch of independent problems: per-problem hb_dev records in device
+ * memory and, per phase, a table of (problem, tile) pairs so that one
+ * launch covers the tiles of every problem. */
+typedef struct hb_batch {
+    int nprob, exact, W;
+    int n_spmv_tiles, n_vec_tiles, n_shift_tiles;
+    const hb_dev *devs;        /* [nprob] device array                     */
+    const int *t_spmv;         /* [2*n_spmv_tiles]  (problem, SpMV tile)   */
+    const int *t_vec;          /* [2*n_vec_tiles]   (problem, vector tile) */
+    const int *t_shift;        /* [2*n_shift_tiles] (problem, by*gx + bx)  */
+} hb_batch;
 
 #ifdef __cplusplus
 }

@@ -6,4 +6,4 @@ and provides the torch.autograd binding.
 """
 from . import problems  # noqa: F401
 from ._lib import LIB_PATH, load  # noqa: F401
-from .solver import VARIANTS, Plan, SolveResult, dropin_solve, make_params  # noqa: F401
+from .solver import VARIANTS, Batch, Plan, SolveResult, dropin_solve, make_params  # noqa: F401

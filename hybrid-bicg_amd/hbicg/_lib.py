@@ -60,6 +60,11 @@ class Report(C.Structure):
                 ("t_setup_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_total_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    """include/hbicg.h hbicg_tuning."""
+    _fields_ = [("window", C.c_int), ("group_cap", C.c_int), ("ms_per_iter", C.c_double)]
+
+
 class Csr(C.Structure):
     """include/hbicg.h hbicg_csr."""
     _fields_ = [("val", C.c_void_p), ("col", C.c_void_p), ("ptr", C.c_void_p), ("n", C.c_uint),
@@ -76,7 +81,9 @@ EXPORTS = {
     "hbicg.h": ["hbicg_default_params", "hbicg_set_params", "hbicg_get_params",
                 "hbicg_last_report", "hbicg_plan_create", "hbicg_plan_destroy", "hbicg_errno",
                 "hbicg_plan_solve", "hbicg_plan_true_residuals", "hbicg_plan_spmv",
-                "hbicg_plan_report", "hbicg_plan_kernel_stats"],
+                "hbicg_plan_report", "hbicg_plan_kernel_stats", "hbicg_batch_create",
+                "hbicg_batch_destroy", "hbicg_batch_solve", "hbicg_batch_report",
+                "hbicg_batch_true_residuals", "hbicg_batch_autotune"],
 }
 
 _lib = None
@@ -135,6 +142,18 @@ def load():
     L.hbicg_plan_report.restype = C.c_int
     L.hbicg_plan_kernel_stats.argtypes = [C.c_void_p, C.POINTER(KernelStat), C.c_int]
     L.hbicg_plan_kernel_stats.restype = C.c_int
+    vpp = C.POINTER(C.c_void_p)
+    L.hbicg_batch_create.argtypes = [C.POINTER(Csr), C.c_int, C.c_int, _ip, C.POINTER(Params), C.c_void_p]
+    L.hbicg_batch_create.restype = C.c_void_p
+    L.hbicg_batch_destroy.argtypes = [C.c_void_p]
+    L.hbicg_batch_solve.argtypes = [C.c_void_p, vpp, vpp, _ip, vpp, _ip, _dp]
+    L.hbicg_batch_solve.restype = C.c_int
+    L.hbicg_batch_report.argtypes = [C.c_void_p, C.c_int, C.POINTER(Report), _ip, _ip, _ip, C.c_int]
+    L.hbicg_batch_report.restype = C.c_int
+    L.hbicg_batch_true_residuals.argtypes = [C.c_void_p, C.c_int, C.c_void_p, _dp, C.c_void_p, _dp]
+    L.hbicg_batch_true_residuals.restype = C.c_int
+    L.hbicg_batch_autotune.argtypes = [C.c_void_p, vpp, vpp, _ip, vpp, C.c_int, C.POINTER(Tuning)]
+    L.hbicg_batch_autotune.restype = C.c_int
     _lib = L
     return L
 

@@ -194,3 +194,111 @@ class Plan:
         self._leave()
         _lib.check(rc, "hbicg_plan_spmv")
         return y
+
+
+class Batch:
+    """Many independent shifted problems solved together on one GPU
+    (hbicg_batch_*).  problems: list of problems.ShiftedProblem."""
+
+    def __init__(self, problems, params=None, device="cuda", stream=None):
+        import torch
+        self._torch = torch
+        L = _lib.load()
+        self.L = L
+        dev = torch.device(device)
+        self.device = dev
+        self.problems = list(problems)
+        self.params = params if params is not None else make_params(graph=1)
+        self.stream = stream if stream is not None else torch.cuda.Stream(dev)
+        self._keep = []
+        csrs = (_lib.Csr * len(self.problems))()
+        for i, p in enumerate(self.problems):
+            val = torch.as_tensor(np.ascontiguousarray(p.A.val, dtype=np.float64)).to(dev)
+            col = torch.as_tensor(np.ascontiguousarray(p.A.col, dtype=np.uint32).view(np.int32)).to(dev)
+            ptr = torch.as_tensor(np.ascontiguousarray(p.A.ptr, dtype=np.uint32).view(np.int32)).to(dev)
+            self._keep += [val, col, ptr]
+            csrs[i] = _lib.Csr(val.data_ptr(), col.data_ptr(), ptr.data_ptr(), p.n, int(p.A.ptr[-1]))
+        self.M = np.array([p.M for p in self.problems], dtype=np.int32)
+        torch.cuda.synchronize(dev)
+        self.h = L.hbicg_batch_create(csrs, 1, len(self.problems),
+                                      self.M.ctypes.data_as(C.POINTER(C.c_int)),
+                                      C.byref(self.params), C.c_void_p(self.stream.cuda_stream))
+        if not self.h:
+            _lib.check(L.hbicg_errno() or -4, "hbicg_batch_create")
+        self.b = [torch.as_tensor(p.b).to(dev) for p in self.problems]
+        self.sig = [np.ascontiguousarray(p.sigma, dtype=np.float64) for p in self.problems]
+        self.seed = np.array([p.seed for p in self.problems], dtype=np.int32)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hbicg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ptrs(self, xs):
+        nb = len(self.problems)
+        VP = C.c_void_p * nb
+        b = VP(*[t.data_ptr() for t in self.b])
+        s = VP(*[a.ctypes.data for a in self.sig])
+        x = VP(*[t.data_ptr() for t in xs])
+        return b, s, x
+
+    def new_x(self):
+        return [self._torch.zeros((p.M, p.n), dtype=self._torch.float64, device=self.device)
+                for p in self.problems]
+
+    def solve(self, xs=None):
+        """Solves every problem (xs accumulated in place; zeros if None).
+        Returns (k per problem, xs, device ms)."""
+        if xs is None:
+            xs = self.new_x()
+        b, s, x = self._ptrs(xs)
+        k = np.zeros(len(self.problems), dtype=np.int32)
+        ms = C.c_double(0.0)
+        vpp = C.POINTER(C.c_void_p)
+        self.stream.wait_stream(self._torch.cuda.current_stream(self.device))
+        rc = self.L.hbicg_batch_solve(self.h, C.cast(b, vpp), C.cast(s, vpp),
+                                      self.seed.ctypes.data_as(C.POINTER(C.c_int)), C.cast(x, vpp),
+                                      k.ctypes.data_as(C.POINTER(C.c_int)), C.byref(ms))
+        self._torch.cuda.current_stream(self.device).wait_stream(self.stream)
+        _lib.check(rc, "hbicg_batch_solve")
+        return k, xs, ms.value
+
+    def autotune(self, iters=32):
+        xs = self.new_x()
+        b, s, x = self._ptrs(xs)
+        t = _lib.Tuning()
+        vpp = C.POINTER(C.c_void_p)
+        self.stream.wait_stream(self._torch.cuda.current_stream(self.device))
+        rc = self.L.hbicg_batch_autotune(self.h, C.cast(b, vpp), C.cast(s, vpp),
+                                         self.seed.ctypes.data_as(C.POINTER(C.c_int)), C.cast(x, vpp),
+                                         int(iters), C.byref(t))
+        self._torch.cuda.current_stream(self.device).wait_stream(self.stream)
+        _lib.check(rc, "hbicg_batch_autotune")
+        return {"window": t.window, "group_cap": t.group_cap, "ms_per_iter": t.ms_per_iter}
+
+    def report(self, i):
+        p = self.problems[i]
+        rep = _lib.Report()
+        stop = np.zeros(p.M, dtype=np.int32)
+        swk = np.zeros(p.M, dtype=np.int32)
+        swt = np.zeros(p.M, dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        ns = self.L.hbicg_batch_report(self.h, i, C.byref(rep), stop.ctypes.data_as(ip),
+                                       swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), p.M)
+        return _report_dict(rep), stop, [(int(swk[j]), int(swt[j])) for j in range(ns)]
+
+    def true_residuals(self, i, x):
+        p = self.problems[i]
+        out = np.zeros(p.M)
+        self.stream.wait_stream(self._torch.cuda.current_stream(self.device))
+        rc = self.L.hbicg_batch_true_residuals(self.h, i, C.c_void_p(self.b[i].data_ptr()),
+                                               _dp(self.sig[i]), C.c_void_p(x.data_ptr()), _dp(out))
+        self._torch.cuda.current_stream(self.device).wait_stream(self.stream)
+        _lib.check(rc, "hbicg_batch_true_residuals")
+        return out

@@ -117,6 +117,40 @@ int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax);
 int hbicg_plan_report(hbicg_plan *pl, hbicg_report *rep, int *stop_iter,
                       int *switch_k, int *switch_to, int max_switch);
 
+/* ---- batched solver ----------------------------------------------------------
+ * Many independent shifted problems of different n and shift counts solved
+ * together: every phase of the iteration is one launch over the tiles of all
+ * problems (no counterpart in the reference, which solves one family per
+ * call). */
+typedef struct hbicg_batch hbicg_batch;
+
+typedef struct hbicg_tuning {
+    int    window;       /* shift-update window chosen                          */
+    int    group_cap;    /* shifts per shift-kernel workgroup (0: all of a problem) */
+    double ms_per_iter;  /* measured time per batch iteration                   */
+} hbicg_tuning;
+
+/* A[p] (host, a_on_device = 0, or device memory), M[p] shifts each.
+ * params: tol, max_iter, window (0: 16), exact_order, graph (0 off), chunk. */
+hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, const int *M,
+                                const hbicg_params *p, void *stream);
+void hbicg_batch_destroy(hbicg_batch *bt);
+
+/* b[p] device vectors, x[p] device [M_p][n_p] arrays (accumulated in place),
+ * sigma[p] host arrays, seed[p] first seeds.  k_out[p] (optional) receives
+ * each problem's k; solve_ms (optional) the device time of the batch. */
+int hbicg_batch_solve(hbicg_batch *bt, const double *const *b, const double *const *sigma,
+                      const int *seed, double *const *x, int *k_out, double *solve_ms);
+int hbicg_batch_report(hbicg_batch *bt, int p, hbicg_report *rep, int *stop_iter,
+                       int *switch_k, int *switch_to, int max_switch);
+int hbicg_batch_true_residuals(hbicg_batch *bt, int p, const double *b, const double *sigma,
+                               const double *x_set, double *out);
+
+/* Times candidate (window, shifts-per-workgroup) layouts on the batch for
+ * `iters` iterations each and keeps the fastest (x is overwritten). */
+int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *const *sigma,
+                         const int *seed, double *const *x, int iters, hbicg_tuning *best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,85 @@
+"""GPU parity of the batched solver (hbicg_batch_*).
+
+Each problem of a batch runs the same tiles and the same partial-sum order
+as the plain plan, so the batched results must equal per-problem plain
+solves bit for bit; every shift's true residual is recomputed, and in
+exact_order mode a batch must equal the oracle itself."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as O
+from hbicg import Batch, Plan, make_params
+from hbicg import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    # uneven sizes (2-D and 3-D), 8..64 shifts, per-problem convection
+    probs = P.config3(count=10, seed=3, n_lo=300, n_hi=30000)
+    for p in probs[::3]:
+        p.seed = p.M // 2           # middle seeds: exercises seed switching
+    return probs
+
+
+def test_batch_equals_plain_bitwise(small_batch):
+    print("case batch vs plain, 10 problems", flush=True)
+    params = make_params(tol=1e-10, graph=1, window=16)
+    bt = Batch(small_batch, params)
+    k, xs, ms = bt.solve()
+    for i, p in enumerate(small_batch):
+        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1, window=16))
+        kp, xp, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, p.seed)
+        rep, stop, sw = bt.report(i)
+        prep, pstop, psw = pl.report()
+        assert k[i] == kp, (i, p.name)
+        assert np.array_equal(stop, pstop)
+        assert sw == psw
+        assert torch.equal(xs[i], xp), (i, p.name)
+        tr = bt.true_residuals(i, xs[i])
+        assert tr.max() <= 10 * p.tol, (i, p.name, tr.max())
+        pl.close()
+    bt.close()
+
+
+def test_batch_exact_equals_oracle(small_batch):
+    probs = small_batch[:4]
+    print("case batch exact vs oracle, 4 problems", flush=True)
+    bt = Batch(probs, make_params(tol=1e-10, graph=1, exact_order=True, window=8))
+    k, xs, _ = bt.solve()
+    for i, p in enumerate(probs):
+        ref = O.solve(p.A, p.b, p.sigma, p.seed, tol=1e-10)
+        rep, stop, sw = bt.report(i)
+        assert k[i] == ref.k
+        assert np.array_equal(stop, ref.stop_iter)
+        assert sw == ref.switches
+        assert np.array_equal(xs[i].cpu().numpy(), ref.x)
+    bt.close()
+
+
+def test_autotune_keeps_results(small_batch):
+    print("case batch autotune", flush=True)
+    bt = Batch(small_batch, make_params(tol=1e-10, graph=1))
+    k0, x0, _ = bt.solve()
+    best = bt.autotune(iters=16)
+    assert best["window"] in (4, 8, 16) and best["group_cap"] in (0, 4, 16)
+    assert best["ms_per_iter"] > 0
+    k1, x1, _ = bt.solve()
+    assert np.array_equal(k0, k1)
+    for a, b in zip(x0, x1):
+        assert torch.equal(a, b)
+    bt.close()
+
+
+def test_plan_device_path_matches_dropin():
+    """The device-pointer plan and the host drop-in compute the same x."""
+    from hbicg import dropin_solve
+    p = P.config0()
+    got = dropin_solve(p.A, p.b, p.sigma, 8, "static", make_params(tol=p.tol, window=16))
+    pl = Plan(p.A, p.M, make_params(tol=p.tol, window=16, graph=1))
+    k, x, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, 8)
+    assert k == got.k
+    assert np.array_equal(x.cpu().numpy(), got.x)
+    pl.close()
