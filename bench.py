@@ -43,7 +43,66 @@ def parse():
     ap.add_argument("--window", type=int, default=0, help="shift-update window (0: variant default)")
     ap.add_argument("--cpu-iters", type=int, default=40, help="oracle iterations timed (0: skip)")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-problem batched leg")
+    ap.add_argument("--batch-count", type=int, default=256)
     return ap.parse_args()
+
+
+def batched_leg(args, log):
+    """256 independent problems (n log-uniform in [1k, 200k], half 2-D
+    5-point, half 3-D 7-point, 8..64 shifts each) in one batch, against the
+    plain solver called once per problem (plans and graphs prepared, solves
+    timed back to back)."""
+    import numpy as np
+    import torch
+
+    from hbicg import Batch, Plan, make_params
+    from hbicg import problems as P
+
+    t = time.perf_counter()
+    probs = P.config3(count=args.batch_count)
+    log(f"batched: {len(probs)} problems generated in {time.perf_counter() - t:.1f} s "
+        f"(sum n = {sum(p.n for p in probs)}, sum M = {sum(p.M for p in probs)})")
+    bt = Batch(probs, make_params(tol=1e-10, graph=1))
+    tuning = bt.autotune(iters=32)
+    log(f"batched: autotuned {tuning}")
+    bt.solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k, xs, dev_ms = bt.solve()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    conv = all(bt.report(i)[0]["stop_count"] == p.M for i, p in enumerate(probs))
+    worst = max(float(bt.true_residuals(i, xs[i]).max()) for i in range(len(probs)))
+    log(f"batched: {t_batch * 1e3:.1f} ms, all converged {conv}, max true residual {worst:.2e}")
+    del xs
+    bt.close()
+    # plain solver, one call per problem
+    t_plain = 0.0
+    for p in probs:
+        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1))
+        b = torch.as_tensor(p.b).cuda()
+        x = torch.zeros((p.M, p.n), dtype=torch.float64, device="cuda")
+        pl.solve(b, p.sigma, p.seed, x)
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pl.solve(b, p.sigma, p.seed, x)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        pl.close()
+        del x, b
+    sum_m = int(sum(p.M for p in probs))
+    log(f"batched: plain per-problem total {t_plain * 1e3:.1f} ms")
+    return {"problems": len(probs), "shift_systems": sum_m,
+            "batched_ms": round(t_batch * 1e3, 3),
+            "batched_problems_per_s": round(len(probs) / t_batch, 2),
+            "batched_shift_solves_per_s": round(sum_m / t_batch, 2),
+            "plain_per_problem_ms": round(t_plain * 1e3, 3),
+            "plain_shift_solves_per_s": round(sum_m / t_plain, 2),
+            "speedup_vs_plain": round(t_plain / t_batch, 3),
+            "all_converged": bool(conv), "true_residual_max": worst,
+            "max_iterations": int(np.max(k)) - 1, "autotuned": tuning}
 
 
 def main():
@@ -149,6 +208,10 @@ def main():
                           f"seed-step and per-shift-update timers extrapolated to the GPU run's "
                           f"{iters} iterations and {updates} shift updates -> {t_full:.1f} s per solve")}
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args, log)
+
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -173,6 +236,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batched": batched,
         "gen_s": round(t_gen, 2),
     }
     print(json.dumps(line))

@@ -1,0 +1,245 @@
+/*
+ * hb_adjoint.hip -- device CSR transpose and the gradient assembly of the
+ * adjoint of a shifted solve (no counterpart in the reference, which has no
+ * differentiation).
+ *
+ * For x_k = (A + sigma_k I)^{-1} b and a loss L(x_1..x_M), the adjoint
+ * solutions lam_k = (A + sigma_k I)^{-T} dL/dx_k give
+ *   dL/db        = sum_k lam_k
+ *   dL/dsigma_k  = -lam_k . x_k
+ *   dL/dA_ij     = -sum_k lam_k[i] x_k[j]     (on the sparsity pattern)
+ * The adjoint solutions come from the shifted solver run on A^T.
+ */
+#include <hip/hip_runtime.h>
+#include "hb_adjoint.h"
+
+#define TB 256
+
+/* ---- transpose: count, scan, scatter, per-row sort -------------------------- */
+__global__ __launch_bounds__(TB) void k_zero_u(unsigned *__restrict__ a, int n)
+{
+    for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB) a[i] = 0u;
+}
+
+__global__ __launch_bounds__(TB) void k_count_cols(const unsigned *__restrict__ col, unsigned nnz,
+                                                   unsigned *__restrict__ cnt)
+{
+    for (unsigned e = blockIdx.x * TB + threadIdx.x; e < nnz; e += gridDim.x * TB)
+        atomicAdd(&cnt[col[e]], 1u);
+}
+
+/* Exclusive scan of cnt[0..n) into tptr[0..n], three passes: per-block
+ * sums, a one-workgroup scan of the block sums, then the per-element pass. */
+__global__ __launch_bounds__(TB) void k_scan_blocks(const unsigned *__restrict__ cnt, int n,
+                                                    unsigned *__restrict__ bsum)
+{
+    __shared__ unsigned sh[TB];
+    const int i = blockIdx.x * TB + threadIdx.x;
+    sh[threadIdx.x] = i < n ? cnt[i] : 0u;
+    __syncthreads();
+    for (int w = TB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(TB) void k_scan_sums(unsigned *__restrict__ bsum, int nb)
+{
+    __shared__ unsigned sh[TB];
+    unsigned carry = 0;
+    for (int base = 0; base < nb; base += TB) {
+        const int i = base + threadIdx.x;
+        const unsigned v = i < nb ? bsum[i] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < TB; o <<= 1) {
+            const unsigned t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;   /* exclusive */
+        const unsigned tot = sh[TB - 1];
+        __syncthreads();
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(TB) void k_scan_final(const unsigned *__restrict__ cnt, int n,
+                                                   const unsigned *__restrict__ bsum,
+                                                   unsigned *__restrict__ tptr,
+                                                   unsigned *__restrict__ fill)
+{
+    __shared__ unsigned sh[TB];
+    const int i = blockIdx.x * TB + threadIdx.x;
+    const unsigned v = i < n ? cnt[i] : 0u;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < TB; o <<= 1) {
+        const unsigned t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) {
+        const unsigned ex = bsum[blockIdx.x] + sh[threadIdx.x] - v;
+        tptr[i] = ex;
+        fill[i] = ex;
+        if (i == n - 1) tptr[n] = ex + v;
+    }
+}
+
+/* one thread per row of A; entries land in their column's segment of A^T
+ * in arbitrary order (sorted afterwards) */
+__global__ __launch_bounds__(TB) void k_scatter(const unsigned *__restrict__ ptr,
+                                                const unsigned *__restrict__ col,
+                                                const double *__restrict__ val, int n,
+                                                unsigned *__restrict__ fill,
+                                                unsigned *__restrict__ tcol,
+                                                double *__restrict__ tval)
+{
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    for (unsigned e = ptr[i]; e < ptr[i + 1]; ++e) {
+        const unsigned pos = atomicAdd(&fill[col[e]], 1u);
+        tcol[pos] = (unsigned)i;
+        tval[pos] = val[e];
+    }
+}
+
+/* insertion sort of each row of A^T by column: the rows come out in
+ * ascending source-row order, the order a stable transpose produces */
+__global__ __launch_bounds__(TB) void k_sort_rows(const unsigned *__restrict__ tptr, int n,
+                                                  unsigned *__restrict__ tcol,
+                                                  double *__restrict__ tval)
+{
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const unsigned lo = tptr[i], hi = tptr[i + 1];
+    for (unsigned a = lo + 1; a < hi; ++a) {
+        const unsigned c = tcol[a];
+        const double v = tval[a];
+        unsigned b = a;
+        while (b > lo && tcol[b - 1] > c) {
+            tcol[b] = tcol[b - 1];
+            tval[b] = tval[b - 1];
+            --b;
+        }
+        tcol[b] = c;
+        tval[b] = v;
+    }
+}
+
+/* ---- gradients -------------------------------------------------------------- */
+/* lam_k = w_k * mu_k (w == nullptr: w_k = 1).  grad_b[i] = sum_k lam_k[i],
+ * summed over k in ascending order. */
+__global__ __launch_bounds__(TB) void k_grad_b(const double *__restrict__ mu,
+                                               const double *__restrict__ w, int n, int M,
+                                               double *__restrict__ gb)
+{
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int k = 0; k < M; ++k) {
+        const double l = mu[(size_t)k * n + i];
+        acc += w ? w[k] * l : l;
+    }
+    gb[i] = acc;
+}
+
+/* partial dots lam_k . x_k per workgroup, then one workgroup per shift sums
+ * them in order: grad_sigma[k] = -sum */
+__global__ __launch_bounds__(TB) void k_grad_sigma_part(const double *__restrict__ mu,
+                                                        const double *__restrict__ x, int n,
+                                                        double *__restrict__ part)
+{
+    __shared__ double sh[TB];
+    const int k = blockIdx.y;
+    double acc = 0.0;
+    for (int i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB)
+        acc += mu[(size_t)k * n + i] * x[(size_t)k * n + i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = TB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)k * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(TB) void k_grad_sigma_fin(const double *__restrict__ part, int nb,
+                                                       const double *__restrict__ w,
+                                                       double *__restrict__ gs)
+{
+    __shared__ double sh[TB];
+    const int k = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += TB) acc += part[(size_t)k * nb + b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = TB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gs[k] = -(w ? w[k] * sh[0] : sh[0]);
+}
+
+/* one thread per row i: grad_val[e] = -sum_k lam_k[i] * x_k[col[e]] for the
+ * row's nonzeros, shifts in ascending order */
+__global__ __launch_bounds__(TB) void k_grad_vals(const unsigned *__restrict__ ptr,
+                                                  const unsigned *__restrict__ col, int n, int M,
+                                                  const double *__restrict__ mu,
+                                                  const double *__restrict__ w,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ gv)
+{
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
+    const unsigned lo = ptr[i], hi = ptr[i + 1];
+    for (unsigned e = lo; e < hi; ++e) {
+        const unsigned c = col[e];
+        double acc = 0.0;
+        for (int k = 0; k < M; ++k) {
+            const double l = w ? w[k] * mu[(size_t)k * n + i] : mu[(size_t)k * n + i];
+            acc = acc - l * x[(size_t)k * n + c];
+        }
+        gv[e] = acc;
+    }
+}
+
+extern "C" {
+
+int hb_transpose(int n, unsigned nnz, const unsigned *ptr, const unsigned *col, const double *val,
+                 unsigned *tptr, unsigned *tcol, double *tval, unsigned *work, hipStream_t s)
+{
+    /* work: cnt[n] | fill[n] | bsum[ceil(n/TB)] */
+    const int nb = (n + TB - 1) / TB;
+    unsigned *cnt = work, *fill = work + n, *bsum = work + 2 * (size_t)n;
+    hipLaunchKernelGGL(k_zero_u, dim3(nb < 4096 ? nb : 4096), dim3(TB), 0, s, cnt, n);
+    const int g = nnz ? (int)((nnz + TB - 1) / TB < 65536 ? (nnz + TB - 1) / TB : 65536) : 1;
+    hipLaunchKernelGGL(k_count_cols, dim3(g), dim3(TB), 0, s, col, nnz, cnt);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(TB), 0, s, cnt, n, bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TB), 0, s, bsum, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(TB), 0, s, cnt, n, bsum, tptr, fill);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TB), 0, s, ptr, col, val, n, fill, tcol, tval);
+    hipLaunchKernelGGL(k_sort_rows, dim3(nb), dim3(TB), 0, s, tptr, n, tcol, tval);
+    return (int)hipGetLastError();
+}
+
+int hb_adjoint_grads(int n, int M, const unsigned *ptr, const unsigned *col, const double *mu,
+                     const double *w, const double *x, double *grad_b, double *grad_sigma,
+                     double *grad_val, double *work, hipStream_t s)
+{
+    const int nb = (n + TB - 1) / TB;
+    if (grad_b) hipLaunchKernelGGL(k_grad_b, dim3(nb), dim3(TB), 0, s, mu, w, n, M, grad_b);
+    if (grad_sigma) {
+        const int pb = nb < 256 ? nb : 256;
+        hipLaunchKernelGGL(k_grad_sigma_part, dim3(pb, M), dim3(TB), 0, s, mu, x, n, work);
+        hipLaunchKernelGGL(k_grad_sigma_fin, dim3(M), dim3(TB), 0, s, work, pb, w, grad_sigma);
+    }
+    if (grad_val) hipLaunchKernelGGL(k_grad_vals, dim3(nb), dim3(TB), 0, s, ptr, col, n, M, mu, w, x, grad_val);
+    return (int)hipGetLastError();
+}
+
+} /* extern "C" */

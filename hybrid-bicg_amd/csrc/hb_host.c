@@ -16,6 +16,7 @@
 #include <string.h>
 #include <time.h>
 
+#include "hb_adjoint.h"
 #include "hb_launch.h"
 #include "hb_shared.h"
 #include "../../include/hbicg.h"
@@ -1014,4 +1015,47 @@ int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *
         best->ms_per_iter = best_ms / iters;
     }
     return 0;
+}
+
+/* ---- adjoint support ---------------------------------------------------------- */
+int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, double *tval, void *stream)
+{
+    int rc = 0;
+    unsigned *work = NULL;
+    if (!A || !A->ptr || !tptr || (A->nz && (!A->col || !A->val || !tcol || !tval)) || A->n == 0)
+        return HBICG_E_ARG;
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t n = A->n;
+    work = (unsigned *)dalloc((2 * n + (n + HB_BLOCK - 1) / HB_BLOCK + 1) * sizeof(unsigned), &rc);
+    if (rc) return rc;
+    if (hb_transpose((int)n, A->nz, A->ptr, A->col, A->val, tptr, tcol, tval, work, s) != 0) {
+        rc = HBICG_E_HIP;
+        goto fail;
+    }
+    HIPCK(hipStreamSynchronize(s));
+fail:
+    hipFree(work);
+    return rc;
+}
+
+int hbicg_adjoint_gradients(const hbicg_csr *A, int M, const double *mu, const double *w,
+                            const double *x, double *grad_b, double *grad_sigma, double *grad_val,
+                            void *stream)
+{
+    int rc = 0;
+    double *work = NULL;
+    if (!A || !A->ptr || M <= 0 || !mu || !x || A->n == 0) return HBICG_E_ARG;
+    const hipStream_t s = (hipStream_t)stream;
+    const int n = (int)A->n;
+    const int nb = (n + HB_BLOCK - 1) / HB_BLOCK;
+    work = (double *)dalloc((size_t)M * (nb < 256 ? nb : 256) * sizeof(double), &rc);
+    if (rc) return rc;
+    if (hb_adjoint_grads(n, M, A->ptr, A->col, mu, w, x, grad_b, grad_sigma, grad_val, work, s) != 0) {
+        rc = HBICG_E_HIP;
+        goto fail;
+    }
+    HIPCK(hipStreamSynchronize(s));
+fail:
+    hipFree(work);
+    return rc;
 }

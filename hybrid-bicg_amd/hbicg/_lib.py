@@ -83,7 +83,8 @@ EXPORTS = {
                 "hbicg_plan_solve", "hbicg_plan_true_residuals", "hbicg_plan_spmv",
                 "hbicg_plan_report", "hbicg_plan_kernel_stats", "hbicg_batch_create",
                 "hbicg_batch_destroy", "hbicg_batch_solve", "hbicg_batch_report",
-                "hbicg_batch_true_residuals", "hbicg_batch_autotune"],
+                "hbicg_batch_true_residuals", "hbicg_batch_autotune", "hbicg_csr_transpose",
+                "hbicg_adjoint_gradients"],
 }
 
 _lib = None
@@ -154,6 +155,11 @@ def load():
     L.hbicg_batch_true_residuals.restype = C.c_int
     L.hbicg_batch_autotune.argtypes = [C.c_void_p, vpp, vpp, _ip, vpp, C.c_int, C.POINTER(Tuning)]
     L.hbicg_batch_autotune.restype = C.c_int
+    L.hbicg_csr_transpose.argtypes = [C.POINTER(Csr), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.hbicg_csr_transpose.restype = C.c_int
+    L.hbicg_adjoint_gradients.argtypes = [C.POINTER(Csr), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.hbicg_adjoint_gradients.restype = C.c_int
     _lib = L
     return L
 

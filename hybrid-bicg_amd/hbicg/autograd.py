@@ -1,0 +1,136 @@
+"""torch.autograd binding of the shifted solver with a hand-written adjoint.
+
+    x = shifted_solve(val, ptr, col, b, sigma, seed)      # x: [M, n]
+
+solves (A + sigma_k I) x_k = b for every shift on the GPU (val holds A's
+nonzeros in CSR order; ptr/col are int32 tensors holding the unsigned
+32-bit CSR indices).  Backward, for g_k = dL/dx_k:
+
+    lam_k = (A + sigma_k I)^{-T} g_k
+    dL/db = sum_k lam_k,  dL/dsigma_k = -lam_k . x_k,
+    dL/dA_ij = -sum_k lam_k[i] x_k[j] on A's pattern.
+
+A^T is built on the device (hbicg_csr_transpose).  When the g_k are
+collinear, g_k = w_k c (a loss sum_k w_k c^T x_k), one shifted solve on A^T
+with right-hand side c gives every mu_k and lam_k = w_k mu_k.  Otherwise
+each shift is its own single-shift problem on A^T, all solved together by
+the batched solver.  Gradients are assembled by hbicg_adjoint_gradients.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .solver import Batch, Plan, make_params
+
+
+def _csr(val, col, ptr):
+    n = int(ptr.numel()) - 1
+    return _lib.Csr(val.data_ptr(), col.data_ptr(), ptr.data_ptr(), n, int(val.numel()))
+
+
+def transpose(val, col, ptr):
+    """A^T of a device CSR: (tval, tcol, tptr), rows in ascending column order."""
+    L = _lib.load()
+    n = int(ptr.numel()) - 1
+    tptr = torch.empty(n + 1, dtype=torch.int32, device=val.device)
+    tcol = torch.empty_like(col)
+    tval = torch.empty_like(val)
+    s = torch.cuda.current_stream(val.device)
+    rc = L.hbicg_csr_transpose(C.byref(_csr(val, col, ptr)), C.c_void_p(tptr.data_ptr()),
+                               C.c_void_p(tcol.data_ptr()), C.c_void_p(tval.data_ptr()),
+                               C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "hbicg_csr_transpose")
+    return tval, tcol, tptr
+
+
+def gradients(val, col, ptr, mu, w, x, need_b=True, need_sigma=True, need_val=True):
+    """(grad_b, grad_sigma, grad_val) from adjoint solutions lam_k = w_k mu_k."""
+    L = _lib.load()
+    n = int(ptr.numel()) - 1
+    M = int(x.shape[0])
+    dev = x.device
+    gb = torch.empty(n, dtype=torch.float64, device=dev) if need_b else None
+    gs = torch.empty(M, dtype=torch.float64, device=dev) if need_sigma else None
+    gv = torch.empty_like(val) if need_val else None
+    p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+    s = torch.cuda.current_stream(dev)
+    rc = L.hbicg_adjoint_gradients(C.byref(_csr(val, col, ptr)), M, p(mu), p(w), p(x), p(gb), p(gs),
+                                   p(gv), C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "hbicg_adjoint_gradients")
+    return gb, gs, gv
+
+
+def collinear_split(g, rtol=1e-13):
+    """(w, c) with g_k = w_k c when the rows of g are collinear, else None."""
+    norms = torch.linalg.vector_norm(g, dim=1)
+    k0 = int(torch.argmax(norms))
+    c = g[k0]
+    cc = torch.dot(c, c)
+    if float(cc) == 0.0:
+        return torch.zeros(g.shape[0], dtype=g.dtype, device=g.device), c
+    w = (g @ c) / cc
+    resid = torch.linalg.vector_norm(g - w[:, None] * c)
+    if float(resid) <= rtol * float(torch.linalg.vector_norm(g)):
+        return w.contiguous(), c.contiguous()
+    return None
+
+
+class ShiftedSolve(torch.autograd.Function):
+    """x = (A + sigma_k I)^{-1} b for every shift, differentiable in val, b
+    and sigma (see the module docstring)."""
+
+    @staticmethod
+    def forward(ctx, val, b, sigma, ptr, col, seed, tol, max_iter, adj_tol):
+        M = int(sigma.numel())
+        sig = sigma.detach().double().cpu().numpy()
+        params = make_params(tol=tol, max_iter=max_iter, graph=1)
+        vd = val.detach().contiguous()
+        plan = Plan(None, M, params, device=val.device, tensors=(vd, col, ptr))
+        k, x, rep = plan.solve(b.detach().contiguous(), sig, int(seed))
+        plan.close()
+        ctx.save_for_backward(vd, b, x)
+        ctx.ptr, ctx.col, ctx.sig, ctx.seed = ptr, col, sig, int(seed)
+        ctx.adj_tol, ctx.max_iter = adj_tol, max_iter
+        ctx.sigma_device, ctx.sigma_dtype = sigma.device, sigma.dtype
+        ctx.k = k
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        vd, b, x = ctx.saved_tensors
+        ptr, col, sig = ctx.ptr, ctx.col, ctx.sig
+        M = x.shape[0]
+        gx = gx.contiguous()
+        tval, tcol, tptr = transpose(vd, col, ptr)
+        params = make_params(tol=ctx.adj_tol, max_iter=ctx.max_iter, graph=1)
+        split = collinear_split(gx)
+        if split is not None:
+            w, c = split
+            plan = Plan(None, M, params, device=x.device, tensors=(tval, tcol, tptr))
+            _, mu, _ = plan.solve(c, sig, ctx.seed)
+            plan.close()
+        else:
+            w = None
+            spec = [{"val": tval, "col": tcol, "ptr": tptr, "b": gx[k], "sigma": sig[k:k + 1],
+                     "seed": 0} for k in range(M)]
+            bt = Batch(None, params, device=x.device, device_spec=spec)
+            _, lams, _ = bt.solve()
+            bt.close()
+            mu = torch.cat([l_ for l_ in lams], dim=0).contiguous()
+        gb, gs, gv = gradients(vd, col, ptr, mu, w, x, ctx.needs_input_grad[1],
+                               ctx.needs_input_grad[2], ctx.needs_input_grad[0])
+        if gs is not None:
+            gs = gs.to(device=ctx.sigma_device, dtype=ctx.sigma_dtype)
+        return gv, gb, gs, None, None, None, None, None, None
+
+
+def shifted_solve(val, ptr, col, b, sigma, seed=0, tol=1e-12, max_iter=2000, adj_tol=None):
+    """Differentiable shifted solve; returns x [M, n] (float64, on val's device).
+    sigma may live on the host or the device; its gradient has sigma's device."""
+    adj = tol if adj_tol is None else adj_tol
+    x = ShiftedSolve.apply(val, b, sigma, ptr, col, seed, tol, max_iter, adj)
+    return x

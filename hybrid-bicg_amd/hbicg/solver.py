@@ -91,19 +91,27 @@ def dropin_solve(A, b, sigma, seed, variant="dynamic", params=None, x0=None) -> 
 class Plan:
     """Device-resident solver plan over a CSR matrix held in torch tensors."""
 
-    def __init__(self, A, M, params=None, device="cuda", stream=None):
+    def __init__(self, A, M, params=None, device="cuda", stream=None, tensors=None):
+        """A: problems.CSR on the host, or None with tensors=(val, col, ptr)
+        already on the device (float64, int32, int32; col/ptr hold the
+        unsigned 32-bit indices)."""
         import torch
         self._torch = torch
         L = _lib.load()
         self.L = L
-        self.n = A.n
-        self.M = int(M)
         dev = torch.device(device)
         self.device = dev
-        self.val = torch.as_tensor(np.ascontiguousarray(A.val, dtype=np.float64)).to(dev)
-        self.col = torch.as_tensor(np.ascontiguousarray(A.col, dtype=np.uint32).view(np.int32)).to(dev)
-        self.ptr = torch.as_tensor(np.ascontiguousarray(A.ptr, dtype=np.uint32).view(np.int32)).to(dev)
-        self.nnz = int(A.ptr[-1])
+        if tensors is not None:
+            self.val, self.col, self.ptr = tensors
+            self.n = int(self.ptr.numel()) - 1
+            self.nnz = int(self.val.numel())
+        else:
+            self.n = A.n
+            self.val = torch.as_tensor(np.ascontiguousarray(A.val, dtype=np.float64)).to(dev)
+            self.col = torch.as_tensor(np.ascontiguousarray(A.col, dtype=np.uint32).view(np.int32)).to(dev)
+            self.ptr = torch.as_tensor(np.ascontiguousarray(A.ptr, dtype=np.uint32).view(np.int32)).to(dev)
+            self.nnz = int(A.ptr[-1])
+        self.M = int(M)
         self.params = params if params is not None else make_params()
         # A stream of the plan's own (default priority); every call orders it
         # after the caller's current stream and the caller's stream after it.
@@ -200,34 +208,48 @@ class Batch:
     """Many independent shifted problems solved together on one GPU
     (hbicg_batch_*).  problems: list of problems.ShiftedProblem."""
 
-    def __init__(self, problems, params=None, device="cuda", stream=None):
+    def __init__(self, problems, params=None, device="cuda", stream=None, device_spec=None):
+        """problems: list of problems.ShiftedProblem (host data), or None with
+        device_spec = list of dicts {val, col, ptr, b (device tensors),
+        sigma (host array), seed} describing problems already on the GPU."""
         import torch
         self._torch = torch
         L = _lib.load()
         self.L = L
         dev = torch.device(device)
         self.device = dev
-        self.problems = list(problems)
         self.params = params if params is not None else make_params(graph=1)
         self.stream = stream if stream is not None else torch.cuda.Stream(dev)
         self._keep = []
-        csrs = (_lib.Csr * len(self.problems))()
-        for i, p in enumerate(self.problems):
-            val = torch.as_tensor(np.ascontiguousarray(p.A.val, dtype=np.float64)).to(dev)
-            col = torch.as_tensor(np.ascontiguousarray(p.A.col, dtype=np.uint32).view(np.int32)).to(dev)
-            ptr = torch.as_tensor(np.ascontiguousarray(p.A.ptr, dtype=np.uint32).view(np.int32)).to(dev)
-            self._keep += [val, col, ptr]
-            csrs[i] = _lib.Csr(val.data_ptr(), col.data_ptr(), ptr.data_ptr(), p.n, int(p.A.ptr[-1]))
-        self.M = np.array([p.M for p in self.problems], dtype=np.int32)
+        if device_spec is None:
+            spec = []
+            for p in problems:
+                spec.append({
+                    "val": torch.as_tensor(np.ascontiguousarray(p.A.val, dtype=np.float64)).to(dev),
+                    "col": torch.as_tensor(np.ascontiguousarray(p.A.col, dtype=np.uint32).view(np.int32)).to(dev),
+                    "ptr": torch.as_tensor(np.ascontiguousarray(p.A.ptr, dtype=np.uint32).view(np.int32)).to(dev),
+                    "b": torch.as_tensor(p.b).to(dev), "sigma": p.sigma, "seed": p.seed})
+            self.problems = list(problems)
+        else:
+            spec = list(device_spec)
+            self.problems = None
+        self._spec = spec
+        nprob = len(spec)
+        self.n = [int(d["ptr"].numel()) - 1 for d in spec]
+        csrs = (_lib.Csr * nprob)()
+        for i, d in enumerate(spec):
+            self._keep += [d["val"], d["col"], d["ptr"]]
+            csrs[i] = _lib.Csr(d["val"].data_ptr(), d["col"].data_ptr(), d["ptr"].data_ptr(),
+                               self.n[i], int(d["val"].numel()))
+        self.sig = [np.ascontiguousarray(d["sigma"], dtype=np.float64) for d in spec]
+        self.M = np.array([s_.shape[0] for s_ in self.sig], dtype=np.int32)
         torch.cuda.synchronize(dev)
-        self.h = L.hbicg_batch_create(csrs, 1, len(self.problems),
-                                      self.M.ctypes.data_as(C.POINTER(C.c_int)),
+        self.h = L.hbicg_batch_create(csrs, 1, nprob, self.M.ctypes.data_as(C.POINTER(C.c_int)),
                                       C.byref(self.params), C.c_void_p(self.stream.cuda_stream))
         if not self.h:
             _lib.check(L.hbicg_errno() or -4, "hbicg_batch_create")
-        self.b = [torch.as_tensor(p.b).to(dev) for p in self.problems]
-        self.sig = [np.ascontiguousarray(p.sigma, dtype=np.float64) for p in self.problems]
-        self.seed = np.array([p.seed for p in self.problems], dtype=np.int32)
+        self.b = [d["b"] for d in spec]
+        self.seed = np.array([d["seed"] for d in spec], dtype=np.int32)
 
     def close(self):
         if getattr(self, "h", None):
@@ -241,7 +263,7 @@ class Batch:
             pass
 
     def _ptrs(self, xs):
-        nb = len(self.problems)
+        nb = len(self.b)
         VP = C.c_void_p * nb
         b = VP(*[t.data_ptr() for t in self.b])
         s = VP(*[a.ctypes.data for a in self.sig])
@@ -249,8 +271,8 @@ class Batch:
         return b, s, x
 
     def new_x(self):
-        return [self._torch.zeros((p.M, p.n), dtype=self._torch.float64, device=self.device)
-                for p in self.problems]
+        return [self._torch.zeros((int(m), n), dtype=self._torch.float64, device=self.device)
+                for m, n in zip(self.M, self.n)]
 
     def solve(self, xs=None):
         """Solves every problem (xs accumulated in place; zeros if None).
@@ -258,7 +280,7 @@ class Batch:
         if xs is None:
             xs = self.new_x()
         b, s, x = self._ptrs(xs)
-        k = np.zeros(len(self.problems), dtype=np.int32)
+        k = np.zeros(len(self.b), dtype=np.int32)
         ms = C.c_double(0.0)
         vpp = C.POINTER(C.c_void_p)
         self.stream.wait_stream(self._torch.cuda.current_stream(self.device))
@@ -283,19 +305,18 @@ class Batch:
         return {"window": t.window, "group_cap": t.group_cap, "ms_per_iter": t.ms_per_iter}
 
     def report(self, i):
-        p = self.problems[i]
+        m = int(self.M[i])
         rep = _lib.Report()
-        stop = np.zeros(p.M, dtype=np.int32)
-        swk = np.zeros(p.M, dtype=np.int32)
-        swt = np.zeros(p.M, dtype=np.int32)
+        stop = np.zeros(m, dtype=np.int32)
+        swk = np.zeros(m, dtype=np.int32)
+        swt = np.zeros(m, dtype=np.int32)
         ip = C.POINTER(C.c_int)
         ns = self.L.hbicg_batch_report(self.h, i, C.byref(rep), stop.ctypes.data_as(ip),
-                                       swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), p.M)
+                                       swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), m)
         return _report_dict(rep), stop, [(int(swk[j]), int(swt[j])) for j in range(ns)]
 
     def true_residuals(self, i, x):
-        p = self.problems[i]
-        out = np.zeros(p.M)
+        out = np.zeros(int(self.M[i]))
         self.stream.wait_stream(self._torch.cuda.current_stream(self.device))
         rc = self.L.hbicg_batch_true_residuals(self.h, i, C.c_void_p(self.b[i].data_ptr()),
                                                _dp(self.sig[i]), C.c_void_p(x.data_ptr()), _dp(out))

@@ -146,6 +146,21 @@ int hbicg_batch_report(hbicg_batch *bt, int p, hbicg_report *rep, int *stop_iter
 int hbicg_batch_true_residuals(hbicg_batch *bt, int p, const double *b, const double *sigma,
                                const double *x_set, double *out);
 
+/* ---- adjoint support (device pointers) ------------------------------------- */
+/* A^T of an n x n device CSR into tptr[n+1], tcol[nz], tval[nz] (device),
+ * rows listing their entries in ascending column order.  stream: hipStream_t
+ * or NULL.  Synchronises the stream. */
+int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, double *tval, void *stream);
+
+/* Gradients of a loss through x_k = (A + sigma_k I)^{-1} b from the adjoint
+ * solutions lam_k = w_k mu_k (w NULL: w_k = 1), all [M][n] shift-major:
+ *   grad_b = sum_k lam_k,  grad_sigma_k = -lam_k . x_k,
+ *   grad_val[e] = -sum_k lam_k[row(e)] x_k[col(e)] on A's pattern.
+ * Any output may be NULL.  Synchronises the stream. */
+int hbicg_adjoint_gradients(const hbicg_csr *A, int M, const double *mu, const double *w,
+                            const double *x, double *grad_b, double *grad_sigma, double *grad_val,
+                            void *stream);
+
 /* Times candidate (window, shifts-per-workgroup) layouts on the batch for
  * `iters` iterations each and keeps the fastest (x is overwritten). */
 int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *const *sigma,
