@@ -79,6 +79,59 @@ def collinear_split(g, rtol=1e-13):
     return None
 
 
+class _PlanCache:
+    """Plans kept across calls, keyed by the device arrays they read (the
+    plan reads val in place, so updated values are seen) and the solve
+    parameters; a few entries, least recently used dropped first."""
+
+    def __init__(self, size=4):
+        self.size = size
+        self.items = {}
+
+    def get(self, key, make):
+        if key in self.items:
+            plan = self.items.pop(key)
+        else:
+            plan = make()
+            while len(self.items) >= self.size:
+                old = self.items.pop(next(iter(self.items)))
+                old.close()
+        self.items[key] = plan
+        return plan
+
+
+_plans = _PlanCache()
+_transposes = {}
+
+
+def _plan_for(val, col, ptr, M, tol, max_iter):
+    key = (val.data_ptr(), col.data_ptr(), ptr.data_ptr(), int(val.numel()), M, tol, max_iter)
+    params = make_params(tol=tol, max_iter=max_iter, graph=1)
+    return _plans.get(key, lambda: Plan(None, M, params, device=val.device, tensors=(val, col, ptr)))
+
+
+def _transpose_into(val, col, ptr):
+    """A^T in buffers kept per (val, col, ptr) so that the adjoint plan on
+    them can be reused; the values are rebuilt on every call."""
+    key = (val.data_ptr(), col.data_ptr(), ptr.data_ptr(), int(val.numel()))
+    bufs = _transposes.get(key)
+    if bufs is None:
+        if len(_transposes) >= 4:
+            _transposes.pop(next(iter(_transposes)))
+        n = int(ptr.numel()) - 1
+        bufs = (torch.empty_like(val), torch.empty_like(col),
+                torch.empty(n + 1, dtype=torch.int32, device=val.device))
+        _transposes[key] = bufs
+    tval, tcol, tptr = bufs
+    L = _lib.load()
+    s = torch.cuda.current_stream(val.device)
+    rc = L.hbicg_csr_transpose(C.byref(_csr(val, col, ptr)), C.c_void_p(tptr.data_ptr()),
+                               C.c_void_p(tcol.data_ptr()), C.c_void_p(tval.data_ptr()),
+                               C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "hbicg_csr_transpose")
+    return tval, tcol, tptr
+
+
 class ShiftedSolve(torch.autograd.Function):
     """x = (A + sigma_k I)^{-1} b for every shift, differentiable in val, b
     and sigma (see the module docstring)."""
@@ -87,11 +140,11 @@ class ShiftedSolve(torch.autograd.Function):
     def forward(ctx, val, b, sigma, ptr, col, seed, tol, max_iter, adj_tol):
         M = int(sigma.numel())
         sig = sigma.detach().double().cpu().numpy()
-        params = make_params(tol=tol, max_iter=max_iter, graph=1)
-        vd = val.detach().contiguous()
-        plan = Plan(None, M, params, device=val.device, tensors=(vd, col, ptr))
+        vd = val.detach()
+        if not vd.is_contiguous():
+            vd = vd.contiguous()
+        plan = _plan_for(vd, col, ptr, M, tol, max_iter)
         k, x, rep = plan.solve(b.detach().contiguous(), sig, int(seed))
-        plan.close()
         ctx.save_for_backward(vd, b, x)
         ctx.ptr, ctx.col, ctx.sig, ctx.seed = ptr, col, sig, int(seed)
         ctx.adj_tol, ctx.max_iter = adj_tol, max_iter
@@ -105,14 +158,13 @@ class ShiftedSolve(torch.autograd.Function):
         ptr, col, sig = ctx.ptr, ctx.col, ctx.sig
         M = x.shape[0]
         gx = gx.contiguous()
-        tval, tcol, tptr = transpose(vd, col, ptr)
+        tval, tcol, tptr = _transpose_into(vd, col, ptr)
         params = make_params(tol=ctx.adj_tol, max_iter=ctx.max_iter, graph=1)
         split = collinear_split(gx)
         if split is not None:
             w, c = split
-            plan = Plan(None, M, params, device=x.device, tensors=(tval, tcol, tptr))
+            plan = _plan_for(tval, tcol, tptr, M, ctx.adj_tol, ctx.max_iter)
             _, mu, _ = plan.solve(c, sig, ctx.seed)
-            plan.close()
         else:
             w = None
             spec = [{"val": tval, "col": tcol, "ptr": tptr, "b": gx[k], "sigma": sig[k:k + 1],
