@@ -40,11 +40,18 @@ def stream_key(seed: int, stream: int) -> int:
 def splitmix64(key: int, count: int, offset: int = 0) -> np.ndarray:
     """count outputs of SplitMix64 started at state key, skipping offset."""
     with np.errstate(over="ignore"):
-        i = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
-        z = np.uint64(key) + i * np.uint64(_GAMMA)
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        return z ^ (z >> np.uint64(31))
+        z = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        z *= np.uint64(_GAMMA)
+        z += np.uint64(key)
+        t = z >> np.uint64(30)
+        z ^= t
+        z *= np.uint64(0xBF58476D1CE4E5B9)
+        np.right_shift(z, np.uint64(27), out=t)
+        z ^= t
+        z *= np.uint64(0x94D049BB133111EB)
+        np.right_shift(z, np.uint64(31), out=t)
+        z ^= t
+        return z
 
 
 def uniform(seed: int, stream: int, count: int, lo: float = 0.0, hi: float = 1.0,
@@ -190,31 +197,41 @@ def zipf_matrix(n: int, mean: float = 30.0, cap: int = 2000, seed: int = 2) -> C
     lens = np.minimum(lens, n - 1)
     tot = int(lens.sum())
     rows = np.repeat(np.arange(n, dtype=np.int64), lens)
-    cols = (splitmix64(stream_key(seed, 12), tot) % np.uint64(n)).astype(np.int64)
-    redraw_round = 0
-    while True:
-        key = rows * n + cols
-        order = np.argsort(key, kind="stable")
-        ks = key[order]
-        dup = np.zeros(tot, dtype=bool)
-        dup[order[1:]] = ks[1:] == ks[:-1]
-        bad = dup | (cols == rows)
-        nb = int(bad.sum())
-        if nb == 0:
-            break
-        redraw_round += 1
-        fresh = splitmix64(stream_key(seed, 100 + redraw_round), nb)
-        cols[bad] = (fresh % np.uint64(n)).astype(np.int64)
+    rown = rows * n
+    key = rown + (splitmix64(stream_key(seed, 12), tot) % np.uint64(n)).astype(np.int64)
+    key.sort()                           # row-major keys: rows stay grouped, columns ascend
+    cols = key - rown
+    del key
+    bad = cols == rows
+    bad[1:] |= (cols[1:] == cols[:-1]) & (rows[1:] == rows[:-1])   # a repeat in the row
+    starts = np.zeros(n, dtype=np.int64)
+    starts[1:] = np.cumsum(lens)[:-1]
+    for r in np.unique(rows[bad]):
+        # rows with a repeated or diagonal column draw replacements from a
+        # stream of their own until they hold lens[r] distinct columns
+        a, m = int(starts[r]), int(lens[r])
+        have = sorted(set(int(c) for c in cols[a:a + m]) - {int(r)})
+        extra, off = [], 0
+        while len(have) + len(extra) < m:
+            cand = splitmix64(stream_key(seed, 1000 + int(r)), 64, off) % np.uint64(n)
+            off += 64
+            taken = set(have) | set(extra) | {int(r)}
+            for c in cand:
+                c = int(c)
+                if c not in taken and len(have) + len(extra) < m:
+                    extra.append(c)
+                    taken.add(c)
+        cols[a:a + m] = np.sort(np.array(have + extra, dtype=np.int64))
     off_val = uniform(seed, 13, tot, -1.0, 1.0)
     absum = np.bincount(rows, weights=np.abs(off_val), minlength=n)
-    all_rows = np.concatenate([rows, np.arange(n, dtype=np.int64)])
-    all_cols = np.concatenate([cols, np.arange(n, dtype=np.int64)])
-    all_vals = np.concatenate([off_val, 1.1 * absum])
-    order = np.lexsort((all_cols, all_rows))
-    counts = lens + 1
+    # the diagonal goes after the row's columns smaller than the row index
+    below = np.bincount(rows, weights=(cols < rows), minlength=n).astype(np.int64)
+    at = starts + below
+    all_cols = np.insert(cols, at, np.arange(n, dtype=np.int64))
+    all_vals = np.insert(off_val, at, 1.1 * absum)
     ptr = np.zeros(n + 1, dtype=np.uint32)
-    ptr[1:] = np.cumsum(counts)
-    return CSR(ptr, all_cols[order].astype(np.uint32), all_vals[order], n)
+    ptr[1:] = np.cumsum(lens + 1)
+    return CSR(ptr, all_cols.astype(np.uint32), all_vals, n)
 
 
 def logspace_shifts(m: int, lo: float, hi: float) -> np.ndarray:

@@ -77,3 +77,26 @@ def test_window_bitwise_independent(c0, window):
     got = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", make_params(tol=c0.tol, window=window))
     assert got.k == base.k
     assert np.array_equal(got.x, base.x)
+
+
+@pytest.mark.parametrize("seed", [0, 8, 15])
+def test_against_golden_fixture(c0, seed):
+    """GPU results against the committed oracle fixture
+    (tests/golden/c0_oracle.npz): exact order bit for bit, default order
+    within the parity bar."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c0_oracle.npz"))
+    ex = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", make_params(tol=c0.tol, exact_order=True))
+    assert ex.k == int(g[f"k_{seed}"])
+    assert np.array_equal(ex.stop_iter, g[f"stop_{seed}"])
+    assert np.array_equal(ex.x[:, ::41], g[f"xs_{seed}"])
+    assert np.array_equal(ex.x.sum(axis=1), g[f"xsum_{seed}"])
+    tr = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", make_params(tol=c0.tol))
+    assert abs(tr.k - int(g[f"k_{seed}"])) <= 2
+    assert np.all(np.abs(tr.stop_iter - g[f"stop_{seed}"]) <= 2)
+    gs = g[f"switch_{seed}"]
+    assert len(tr.switches) == len(gs)
+    for (kg, sg), (kr, sr) in zip(tr.switches, gs):
+        assert abs(kg - kr) <= 2 and sg == sr
+    rel = np.abs(tr.x[:, ::41] - g[f"xs_{seed}"]).max() / np.abs(g[f"xs_{seed}"]).max()
+    assert rel <= 1e-9
