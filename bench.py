@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-problem batched leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device copy (counter calibration under rocprofv3)")
     return ap.parse_args()
 
 
@@ -169,6 +171,15 @@ def main():
     from hbicg import Plan, make_params
     from hbicg import problems as P
 
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy probe: 3 copies of 1 GiB done")
+
     t_gen = time.perf_counter()
     prob = P.config1(grid=args.grid, m=args.shifts)
     t_gen = time.perf_counter() - t_gen
@@ -176,7 +187,7 @@ def main():
     dev = torch.device("cuda:0")
     M, n = prob.M, prob.n
     graph = 1 if args.variant == "dynamic" else 0
-    window = args.window if args.window > 0 else (1 if args.variant == "normal" else 16)
+    window = args.window if args.window > 0 else (1 if args.variant == "normal" else 64)
     params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph)
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)

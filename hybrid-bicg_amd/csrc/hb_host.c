@@ -39,6 +39,7 @@ void hbicg_default_params(hbicg_params *p)
     p->exit_on_error = 1;
     p->upload_x = 1;
     p->profile = 0;
+    p->growth_limit = 0.0;
 }
 
 static const hbicg_params *params(void)
@@ -90,7 +91,7 @@ static double now_ms(void)
 /* ---- plan ------------------------------------------------------------------ */
 struct hbicg_plan {
     int n, M, W, cap, exact, graph, chunk, upload_x, device, profile, vec2;
-    double tol;
+    double tol, growth_limit;
     unsigned nz;
     hipStream_t stream;
     int own_stream;
@@ -100,7 +101,7 @@ struct hbicg_plan {
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
-    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *part;
+    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *ccoef, *grow, *part;
     int *stop, *stop_iter, *act_len, *flist, *fpos, *switch_k, *switch_to;
     hb_state *st;
     int *flag_host;      /* pinned: {done, k} */
@@ -166,7 +167,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -185,10 +186,15 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     free(pl);
 }
 
+/* Window of a plan: the caller's, else 1 for the normal variant (variant 0,
+ * per-iteration updates) and the largest the arithmetic mode supports for
+ * the others.  Exact-order results do not depend on the window, so an exact
+ * plan clamps it to the exact replay's limit. */
 static int window_for(const hbicg_params *p, int variant)
 {
-    if (p->window > 0) return p->window;
-    return variant == 0 ? 1 : 16;   /* variant 0: normal */
+    const int wmax = p->exact_order ? HB_WMAX_EXACT : HB_WMAX;
+    if (p->window > 0) return p->window < wmax ? p->window : wmax;
+    return variant == 0 ? 1 : wmax;
 }
 
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
@@ -198,7 +204,10 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     unsigned *hptr = NULL, *hblk = NULL;
     hbicg_plan *pl = NULL;
     g_errno = 0;
-    if (!A || !A->ptr || !p || M <= 0 || A->n == 0) { g_errno = HBICG_E_ARG; return NULL; }
+    if (!A || !A->ptr || !p || M <= 0 || A->n == 0 || p->window < 0 || p->window > HB_WMAX) {
+        g_errno = HBICG_E_ARG;
+        return NULL;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_errno = HBICG_E_NODEVICE; return NULL; }
     pl = (hbicg_plan *)calloc(1, sizeof(*pl));
@@ -212,6 +221,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->W = window_for(p, variant);
     pl->cap = p->max_iter + 1;
     pl->tol = p->tol;
+    pl->growth_limit = p->growth_limit > 0.0 ? p->growth_limit : 0.0;
     pl->exact = p->exact_order ? 1 : 0;
     pl->graph = p->graph < 0 ? (variant == 2) : (p->graph ? 1 : 0);
     pl->chunk = p->chunk > 0 ? p->chunk : 16;
@@ -283,6 +293,9 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->eta = (double *)dalloc((size_t)M * 8, &rc);
     pl->zeta = (double *)dalloc((size_t)M * 8, &rc);
     pl->coef = (double *)dalloc((size_t)W * M * 6 * 8, &rc);
+    const size_t ccn = (size_t)((M + HB_UG - 1) / HB_UG) * HB_CC_ROWS(W) * (2 * HB_UG);
+    pl->ccoef = (double *)dalloc(ccn * 8, &rc);
+    pl->grow = (double *)dalloc((size_t)2 * M * 8, &rc);
     pl->part = (double *)dalloc((size_t)maxparts * 2 * 8, &rc);
     pl->stop = (int *)dalloc((size_t)M * 4, &rc);
     pl->stop_iter = (int *)dalloc((size_t)M * 4, &rc);
@@ -293,6 +306,8 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->switch_to = (int *)dalloc((size_t)M * 4, &rc);
     pl->st = (hb_state *)dalloc(sizeof(hb_state), &rc);
     if (rc) goto fail;
+    /* unused combined-coefficient columns stay finite */
+    HIPCK(hipMemset(pl->ccoef, 0, ccn * 8));
     HIPCK(hipHostMalloc((void **)&pl->flag_host, 2 * sizeof(int), hipHostMallocDefault));
     HIPCK(hipEventCreateWithFlags(&pl->ev[0], hipEventDisableTiming));
     HIPCK(hipEventCreateWithFlags(&pl->ev[1], hipEventDisableTiming));
@@ -311,11 +326,14 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->shift_gx = gx; d->shift_gy = gy; d->shift_group = G; d->vec2 = vec2;
     pl->vec2 = vec2;
     d->tol = pl->tol;
+    d->growth_limit = pl->growth_limit;
     d->val = pl->val; d->col = pl->col; d->ptr = pl->ptr; d->blk = pl->blk;
     d->sigma = pl->sigma; d->rhat = pl->rhat; d->rring = pl->rring; d->qring = pl->qring;
     d->s = pl->s; d->y = pl->y; d->p_set = pl->p_set; d->x_set = NULL;
     d->aa = pl->aa; d->ba = pl->ba; d->wa = pl->wa; d->pi = pl->pi; d->eta = pl->eta;
-    d->zeta = pl->zeta; d->coef = pl->coef; d->stop = pl->stop; d->stop_iter = pl->stop_iter;
+    d->zeta = pl->zeta; d->coef = pl->coef; d->ccoef = pl->ccoef; d->grow = pl->grow;
+    d->stop = pl->stop;
+    d->stop_iter = pl->stop_iter;
     d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
     d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st;
 
@@ -696,7 +714,7 @@ struct hbicg_batch {
     hbicg_plan **plans;
     hb_dev *hdevs;             /* host copies of the per-problem records */
     hb_dev *ddevs;             /* device array                           */
-    int *t_spmv, *t_vec, *t_shift;
+    int *t_spmv, *t_vec, *t_shift, *t_coef;
     hb_batch B;
     int *flag_host;
     hipEvent_t ev[2], t0, t1;
@@ -721,6 +739,7 @@ void hbicg_batch_destroy(hbicg_batch *bt)
     if (bt->t_spmv) hipFree(bt->t_spmv);
     if (bt->t_vec) hipFree(bt->t_vec);
     if (bt->t_shift) hipFree(bt->t_shift);
+    if (bt->t_coef) hipFree(bt->t_coef);
     if (bt->flag_host) hipHostFree(bt->flag_host);
     for (int i = 0; i < 2; ++i) if (bt->ev[i]) hipEventDestroy(bt->ev[i]);
     if (bt->t0) hipEventDestroy(bt->t0);
@@ -737,11 +756,14 @@ static void shift_grid(const hbicg_plan *pl, int G, int *gx, int *gy)
 }
 
 /* (Re)builds the shift tile table for group cap G (0: all shifts of a
- * problem in one workgroup row) and uploads the per-problem records. */
+ * problem in one workgroup row) and uploads the per-problem records.  The
+ * default mode's combined-coefficient flush takes every shift in one row. */
 static int batch_layout(hbicg_batch *bt, int W, int group_cap)
 {
     int rc = 0, *tab = NULL;
     size_t tiles = 0;
+    if (!bt->exact) group_cap = 0;
+    if (W > (bt->exact ? HB_WMAX_EXACT : HB_WMAX)) return HBICG_E_ARG;
     for (int p = 0; p < bt->nprob; ++p) {
         const hbicg_plan *pl = bt->plans[p];
         const int G = group_cap > 0 && group_cap < pl->M ? group_cap : pl->M;
@@ -802,18 +824,30 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     bt->plans = (hbicg_plan **)calloc((size_t)nprob, sizeof(hbicg_plan *));
     bt->hdevs = (hb_dev *)calloc((size_t)nprob, sizeof(hb_dev));
     if (!bt->plans || !bt->hdevs) { rc = HBICG_E_NOMEM; goto fail; }
-    size_t nsp = 0, nvec = 0;
+    size_t nsp = 0, nvec = 0, ncoef = 0;
+    const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WMAX;
     for (int q = 0; q < nprob; ++q) {
-        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, HB_WMAX);
+        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax);
         if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
         bt->hdevs[q] = bt->plans[q]->dev;
         nsp += (size_t)bt->plans[q]->nblk;
         nvec += (size_t)bt->plans[q]->dev.vec_grid;
+        ncoef += (size_t)M[q];
     }
-    tab = (int *)malloc((nsp > nvec ? nsp : nvec) * 2 * sizeof(int));
+    {
+        size_t most = nsp > nvec ? nsp : nvec;
+        if (ncoef > most) most = ncoef;
+        tab = (int *)malloc(most * 2 * sizeof(int));
+    }
     if (!tab) { rc = HBICG_E_NOMEM; goto fail; }
     {
         size_t t = 0;
+        for (int q = 0; q < nprob; ++q)
+            for (int j = 0; j < M[q]; ++j) { tab[2 * t] = q; tab[2 * t + 1] = j; ++t; }
+        bt->t_coef = (int *)dalloc(ncoef * 2 * sizeof(int), &rc);
+        if (rc) goto fail;
+        HIPCK(hipMemcpy(bt->t_coef, tab, ncoef * 2 * sizeof(int), hipMemcpyHostToDevice));
+        t = 0;
         for (int q = 0; q < nprob; ++q)
             for (int b = 0; b < bt->plans[q]->nblk; ++b) { tab[2 * t] = q; tab[2 * t + 1] = b; ++t; }
         bt->t_spmv = (int *)dalloc(nsp * 2 * sizeof(int), &rc);
@@ -840,7 +874,10 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     bt->B.t_vec = bt->t_vec;
     bt->B.n_spmv_tiles = (int)nsp;
     bt->B.n_vec_tiles = (int)nvec;
-    if ((rc = batch_layout(bt, p->window > 0 ? p->window : HB_WMAX, 0)) != 0) goto fail;
+    bt->B.t_coef = bt->t_coef;
+    bt->B.n_coef_tiles = (int)ncoef;
+    if ((rc = batch_layout(bt, p->window > 0 && p->window < wmax ? p->window : wmax, 0)) != 0)
+        goto fail;
     free(tab);
     return bt;
 fail:
@@ -992,14 +1029,19 @@ int hbicg_batch_true_residuals(hbicg_batch *bt, int p, const double *b, const do
 int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *const *sigma,
                          const int *seed, double *const *x, int iters, hbicg_tuning *best)
 {
-    static const int windows[] = {4, 8, 16};
-    static const int groups[] = {0, 16, 4};
+    /* exact order: window replay with (window, shifts per workgroup row);
+     * default mode: the combined-coefficient flush, window only */
+    static const int windows_exact[] = {4, 8, 16}, groups_exact[] = {0, 16, 4};
+    static const int windows_cc[] = {16, 32, 64}, groups_cc[] = {0};
+    const int *windows = bt->exact ? windows_exact : windows_cc;
+    const int *groups = bt->exact ? groups_exact : groups_cc;
+    const size_t nw = 3, ng = bt->exact ? 3 : 1;
     int rc = 0;
     if (!bt || iters < 1) return HBICG_E_ARG;
     double best_ms = 1e300;
     int bw = bt->W, bg = bt->group_cap;
-    for (size_t wi = 0; wi < sizeof(windows) / sizeof(windows[0]); ++wi)
-        for (size_t gi = 0; gi < sizeof(groups) / sizeof(groups[0]); ++gi) {
+    for (size_t wi = 0; wi < nw; ++wi)
+        for (size_t gi = 0; gi < ng; ++gi) {
             if ((rc = batch_layout(bt, windows[wi], groups[gi])) != 0) return rc;
             double ms = 0.0;
             for (int rep = 0; rep < 2; ++rep) {   /* first run warms the graph */

@@ -66,14 +66,23 @@ DEV void block_sum2(double &a, double &b)
     __syncthreads();
 }
 
-/* Combines per-workgroup partial pairs in a fixed order (thread 0 result).
- * Eight 16-byte pairs are in flight per thread before any is added, so the
- * combine costs about one memory latency, not one per partial. */
+/* Combines per-workgroup partial pairs in a fixed order (thread 0 result):
+ * thread t sums partials t, t + 256, t + 512, ... in ascending order, then
+ * the workgroup sum.  32 16-byte pairs are in flight per thread before any
+ * is added (the combining workgroup is alone on its CU, so registers are
+ * free), so 8192 partials cost one memory latency, not four. */
 DEV void reduce_partials(const double *part, int nb, double &a, double &b)
 {
     const dbl2 *p2 = (const dbl2 *)part;
     double sa = 0.0, sb = 0.0;
     int i = threadIdx.x;
+    for (; i + 31 * HB_BLOCK < nb; i += 32 * HB_BLOCK) {
+        dbl2 v[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) v[u] = p2[i + u * HB_BLOCK];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) { sa += v[u].x; sb += v[u].y; }
+    }
     for (; i + 7 * HB_BLOCK < nb; i += 8 * HB_BLOCK) {
         dbl2 v[8];
 #pragma unroll
@@ -320,7 +329,8 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     __shared__ double sh_zp[HB_BLOCK];
     __shared__ int sh_j[HB_BLOCK];
     __shared__ int sh_nc[HB_BLOCK];
-    __shared__ int sh_switch;
+    __shared__ int sh_switch, sh_spike, sh_flush;
+    __shared__ double sh_rprev;
     hb_state *st = d.st;
     const int tid = threadIdx.x;
     /* every value this stage reads, loaded before its first store */
@@ -332,6 +342,8 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
 
     if (tid == 0) {
         const double beta = (ak / wk) * (rTr / rTr_old);
+        sh_rprev = st->dot_r;
+        sh_spike = 0;
         st->dot_r = dot_r;
         st->rTr = rTr;
         d.ba[k] = beta;
@@ -375,6 +387,18 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         if (tid == 0) st->flush_cnt = base;
     }
 
+    /* Opt-in guard of the default mode (growth_limit > 0): how large the
+     * window's combined representation of p_j grows.  grow[j] bounds
+     * sum_k |KP_k| |g_k| (plus the window-start p_j term); an iteration adds
+     * |c4| (grow + |c2| |q| + |c3| |r_old|) + |c5| |r_new|, while its own
+     * terms weigh |c4| (|c2| |q| + |c3| |r_old|) + |c5| |r_new|.  Past
+     * growth_limit times the iteration's own terms the window closes, with
+     * this iteration applied on its own (the per-iteration order).  Off by
+     * default: with the coefficients carried in double-double the regrouped
+     * flush already matches the per-iteration accuracy, and closing a window
+     * where the bound peaks lands on the iterations of heaviest
+     * cancellation. */
+    const double nq = sqrt(st->qTq), nro = sqrt(sh_rprev), nrn = sqrt(dot_r);
     double best_zp = 1.0;
     int best_j = -1, nconv = 0;
     for (int j = tid; j < M; j += HB_BLOCK) {
@@ -404,6 +428,14 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             double *c = coef + (size_t)pos * 6;
             c[0] = c1; c[1] = aj; c[2] = c2; c[3] = c3; c[4] = bj; c[5] = c4;
             d.act_len[pos] = slot + 1;
+            if (d.growth_limit > 0.0) {
+                const double own = fabs(bj) * (fabs(c2) * nq + fabs(c3) * nro) + fabs(c4) * nrn;
+                const double prev = k == 1 ? sqrt(dot_zero) : d.grow[j];   /* p_j = b at k = 1 */
+                const double bound = fabs(bj) * prev + own;
+                if (!(bound <= d.growth_limit * own)) sh_spike = 1;
+                d.grow[j] = bound;
+                d.grow[M + j] = own;
+            }
             zp = fabs(1.0 / (znew * pik));
         } else {
             zp = 1.0;
@@ -488,13 +520,16 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             st->seed = st->max_sigma;
             st->switched = 1;
         }
-        const int fl = do_switch || done_next || len >= d.W;
+        const int spike = sh_spike;
+        const int fl = do_switch || done_next || len >= d.W || spike;
         st->flush = fl;
         if (fl) {
             st->flush_k0 = win_k0;
             st->flush_len = len;
+            st->flush_tail = spike;
             st->win_k0 = k + 1;
         }
+        sh_flush = fl;
         st->k = k + 1;
         st->done = done_next;
         /* seed p update: p, r, s in, p out; at a flush also the window
@@ -503,6 +538,13 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         double bytes = 32.0;
         if (fl) bytes += 8.0 * (2 * len + 1) + 32.0 * st->flush_cnt;
         st->shift_bytes += bytes * (double)d.n;
+    }
+    if (d.growth_limit > 0.0) {
+        /* a closing window leaves p_j materialised: the next window's bound
+         * starts from this iteration's own terms */
+        __syncthreads();
+        if (sh_flush)
+            for (int j = tid; j < M; j += HB_BLOCK) d.grow[j] = d.grow[M + j];
     }
 }
 
@@ -611,7 +653,7 @@ DEV void vstore(double *p, const Vd<V> &r)
 }
 
 /* The six vector operations of one shift for window iteration b, in the
- * reference's order (src/solver_dynamic.c:248-255). */
+ * reference's order and roundings (src/solver_dynamic.c:248-255). */
 template <int V>
 DEV void shift_step(Vd<V> &x, Vd<V> &p, const double *c, const Vd<V> &q, const Vd<V> &ro,
                     const Vd<V> &rn)
@@ -630,7 +672,8 @@ DEV void shift_step(Vd<V> &x, Vd<V> &p, const double *c, const Vd<V> &q, const V
     }
 }
 
-/* Tile (bx, by): each thread owns V consecutive elements.  Row by == 0
+/* Exact-order window replay.  Tile (bx, by): each thread owns V consecutive
+ * elements.  Row by == 0
  * applies the seed p update of the finished iteration
  * (src/solver_dynamic.c:208-210).  When a flush is due, row by applies the
  * pending window to the listed shifts by*G .. by*G+G-1: the window's r and q
@@ -742,6 +785,254 @@ template <int WM, int V, int U>
 __global__ __launch_bounds__(HB_BLOCK) void k_shift(hb_dev d)
 {
     shift_tile<WM, V, U>(d, blockIdx.x, blockIdx.y);
+}
+
+/* ---- combined-coefficient window flush (default mode) ----------------------
+ * Applying the window's L iterations to shift j is linear in the window's
+ * vectors g_0 = r_{k0-1}, g_{2b+1} = q_{k0+b}, g_{2b+2} = r_{k0+b}
+ * (b = 0 .. L-1):
+ *     x_j <- x_j + Xp p_j + sum_k KX_k g_k,     p_j <- Pp p_j + sum_k KP_k g_k
+ * with x_j, p_j taken at the window start.  flush_coef_body forms
+ * (Xp, Pp, KX, KP) of one listed shift by running the six operations of
+ * src/solver_dynamic.c:248-255 on coefficient vectors (lane k of the
+ * workgroup carries component k, the last lane the p_j component) for the
+ * act_len iterations the shift was updated in; components of later
+ * iterations stay 0.  shift_cc_body then streams every g_k once per tile
+ * and accumulates HB_UG shifts per wavefront with fused multiply-adds, so
+ * a flush costs 2L+1 vector reads and one read and write of x_j, p_j, and
+ * its arithmetic does not grow with the window's per-step bookkeeping.
+ *
+ * Regrouping moves where cancellation happens.  When one iteration's
+ * coefficients are huge (a pi_k or 1 - omega~ d of the shift near zero) and
+ * the next cancel them, the per-iteration order cancels whole vectors, but
+ * the regrouped form cancels in the coefficients: KX_k collects
+ * c1 * KP_k over iterations, and two nearly opposite large terms leave it
+ * with few correct digits in double precision.  The coefficients are
+ * therefore carried in double-double arithmetic (an unevaluated sum
+ * hi + lo, with Dekker's and Knuth's error-free product and sum -- the
+ * scheme of Hida, Li and Bailey's QD library) and rounded to double once;
+ * the flush sums then lose no more than the per-iteration order does.  As a
+ * second guard the scalar stage watches a bound on the growth of the
+ * regrouped p_j and, past growth_limit, closes the window with a tail: the
+ * first L-1 iterations regrouped, the last applied on its own with its six
+ * operations.
+ *
+ * Combined coefficients of list position pos live in chunk pos / HB_UG,
+ * column pos % HB_UG of a [2W+5][2*HB_UG] block: row 0 holds (Xp | Pp),
+ * row 1+k holds (KX_k | KP_k), x columns first; rows 2W+2 .. 2W+4 hold the
+ * tail iteration's (c0 | c1), (c2 | c3), (c4 | c5), the identity
+ * (0, 0, 0, 0, 1, 0) for a shift that was not updated in it. */
+struct dd {
+    double hi, lo;
+};
+
+/* s + e = a + b exactly (Knuth's two-sum) */
+DEV dd two_sum(double a, double b)
+{
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+
+/* s + e = a + b exactly, given |a| >= |b| (Dekker's fast two-sum) */
+DEV dd quick_two_sum(double a, double b)
+{
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+
+/* a + c */
+DEV dd dd_add(dd a, double c)
+{
+    const dd s = two_sum(a.hi, c);
+    return quick_two_sum(s.hi, s.lo + a.lo);
+}
+
+/* a + b */
+DEV dd dd_add(dd a, dd b)
+{
+    dd s = two_sum(a.hi, b.hi);
+    const dd t = two_sum(a.lo, b.lo);
+    s = quick_two_sum(s.hi, s.lo + t.hi);
+    return quick_two_sum(s.hi, s.lo + t.lo);
+}
+
+/* a * c: the product's rounding error from one fused multiply-add */
+DEV dd dd_mul(dd a, double c)
+{
+    const double p = a.hi * c;
+    const double e = __builtin_fma(a.hi, c, -p);
+    return quick_two_sum(p, __builtin_fma(a.lo, c, e));
+}
+DEV void flush_coef_body(const hb_dev &d, int pos, const double *__restrict__ coef,
+                         double *__restrict__ ccoef)
+{
+    const hb_state *st = d.st;
+    if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
+    if (!st->flush || pos >= st->flush_cnt) return;
+    const int L = st->flush_len, M = d.M, W = d.W;
+    const int Lc = L - st->flush_tail;   /* regrouped iterations */
+    const int lv = d.act_len[pos];
+    const int t = threadIdx.x;
+    const bool pch = t == HB_BLOCK - 1;   /* the p_j component */
+    double *blk = ccoef + (size_t)(pos / HB_UG) * (size_t)HB_CC_ROWS(W) * (2 * HB_UG);
+    if (pch && Lc < L) {
+        const double *c = coef + ((size_t)(L - 1) * M + pos) * 6;
+        const bool on = lv == L;
+        const int base = (2 * W + 2) * (2 * HB_UG) + pos % HB_UG;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            const double id = e == 4 ? 1.0 : 0.0;
+            blk[base + (e / 2) * (2 * HB_UG) + (e % 2) * HB_UG] = on ? c[e] : id;
+        }
+    }
+    if (!pch && t > 2 * Lc) return;
+    dd xk = {0.0, 0.0}, pk = {pch ? 1.0 : 0.0, 0.0};
+    const int nb = min(lv, Lc);
+    for (int b = 0; b < nb; ++b) {
+        const double *c = coef + ((size_t)b * M + pos) * 6;
+        const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
+        /* x += c0 g_{2b+1} + c1 p ;  p = c4 (p + c2 g_{2b+1} + c3 g_{2b}) + c5 g_{2b+2} */
+        dd xn = dd_add(xk, dd_mul(pk, c1));
+        dd tp = pk;
+        if (t == 2 * b + 1) { xn = dd_add(xn, c0); tp = dd_add(tp, c2); }
+        if (t == 2 * b) tp = dd_add(tp, c3);
+        dd pn = dd_mul(tp, c4);
+        if (t == 2 * b + 2) pn = dd_add(pn, c5);
+        xk = xn;
+        pk = pn;
+    }
+    const int row = pch ? 0 : 1 + t;
+    blk[(size_t)row * (2 * HB_UG) + pos % HB_UG] = xk.hi + xk.lo;
+    blk[(size_t)row * (2 * HB_UG) + HB_UG + pos % HB_UG] = pk.hi + pk.lo;
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void k_flush_coef(hb_dev d)
+{
+    flush_coef_body(d, blockIdx.x, d.coef, d.ccoef);
+}
+
+/* Tile bx: 256 elements.  Every thread first applies the seed p update of
+ * the finished iteration to its element (src/solver_dynamic.c:208-210).  At
+ * a flush the four wavefronts then walk the tile's four 64-element
+ * sub-tiles together, wavefront w taking shift chunks w, w+4, ...: the
+ * wavefronts read the same g_k slices at about the same time, so each g_k
+ * comes from HBM once and from cache for the other chunks.  The chunk's
+ * coefficients are uniform across the wavefront and arrive through scalar
+ * loads; the wavefront index goes through readfirstlane so the compiler
+ * knows it is uniform. */
+template <int UG>
+DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
+                       double *__restrict__ pset, const double *__restrict__ rring,
+                       const double *__restrict__ qring, const double *__restrict__ svec,
+                       const double *__restrict__ ccoef, const int *__restrict__ flist)
+{
+    const hb_state *st = d.st;
+    if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
+    const int n = d.n, W = d.W;
+    const int i0 = bx * HB_BLOCK + threadIdx.x;
+    if (i0 < n) {
+        double *ps = pset + off(st->prev_seed, n) + i0;
+        const double pb = st->pb, pbw = st->pbw;
+        const double r = rring[off(st->kprev % (W + 1), n) + i0];
+        double pe = pb * *ps;
+        pe = pe + r;
+        *ps = pe + pbw * svec[i0];
+    }
+    if (!st->flush) return;
+    const int k0 = st->flush_k0, cnt = st->flush_cnt;
+    const int Lc = st->flush_len - st->flush_tail, tail = st->flush_tail;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nch = (cnt + UG - 1) / UG;
+    const size_t cstride = (size_t)HB_CC_ROWS(W) * (2 * UG);
+    for (int sub = 0; sub < HB_BLOCK / 64; ++sub) {
+        const int i = bx * HB_BLOCK + sub * 64 + lane;
+        if (i >= n) break;
+        for (int c = wave; c < nch; c += HB_BLOCK / 64) {
+            const double *__restrict__ cb = ccoef + (size_t)c * cstride;
+            const int nu = min(UG, cnt - c * UG);
+            double x[UG], p[UG];
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                if (u < nu) {
+                    const int j = flist[c * UG + u];
+                    x[u] = __builtin_nontemporal_load(xset + off(j, n) + i);
+                    p[u] = __builtin_nontemporal_load(pset + off(j, n) + i);
+                } else {
+                    x[u] = 0.0;
+                    p[u] = 0.0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                x[u] = __builtin_fma(cb[u], p[u], x[u]);
+                p[u] = cb[UG + u] * p[u];
+            }
+            int rs = (k0 - 1) % (W + 1), qs = k0 % W;
+            double gr = rring[off(rs, n) + i];
+            {
+                const double *__restrict__ ck = cb + 2 * UG;
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    x[u] = __builtin_fma(ck[u], gr, x[u]);
+                    p[u] = __builtin_fma(ck[UG + u], gr, p[u]);
+                }
+            }
+            for (int b = 0; b < Lc; ++b) {
+                rs = rs + 1 == W + 1 ? 0 : rs + 1;
+                const double gq = qring[off(qs, n) + i];
+                gr = rring[off(rs, n) + i];
+                qs = qs + 1 == W ? 0 : qs + 1;
+                const double *__restrict__ ck = cb + (size_t)(2 * b + 2) * (2 * UG);
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    x[u] = __builtin_fma(ck[u], gq, x[u]);
+                    p[u] = __builtin_fma(ck[UG + u], gq, p[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    x[u] = __builtin_fma(ck[2 * UG + u], gr, x[u]);
+                    p[u] = __builtin_fma(ck[3 * UG + u], gr, p[u]);
+                }
+            }
+            if (tail) {
+                /* the last iteration on its own: gr is r_{k0+Lc-1} */
+                rs = rs + 1 == W + 1 ? 0 : rs + 1;
+                const double gq = qring[off(qs, n) + i];
+                const double gn = rring[off(rs, n) + i];
+                const double *__restrict__ ct = cb + (size_t)(2 * W + 2) * (2 * UG);
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    double xe = __builtin_fma(ct[u], gq, x[u]);
+                    xe = __builtin_fma(ct[UG + u], p[u], xe);
+                    double pe = __builtin_fma(ct[2 * UG + u], gq, p[u]);
+                    pe = __builtin_fma(ct[3 * UG + u], gr, pe);
+                    pe = ct[4 * UG + u] * pe;
+                    p[u] = __builtin_fma(ct[5 * UG + u], gn, pe);
+                    x[u] = xe;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                if (u < nu) {
+                    const int j = flist[c * UG + u];
+                    __builtin_nontemporal_store(x[u], xset + off(j, n) + i);
+                    __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
+                }
+            }
+        }
+    }
+}
+
+DEV void shift_cc_tile(const hb_dev &d, int bx)
+{
+    shift_cc_body<HB_UG>(d, bx, d.x_set, d.p_set, d.rring, d.qring, d.s, d.ccoef, d.flist);
+}
+
+__global__ __launch_bounds__(HB_BLOCK) void k_shift_cc(hb_dev d)
+{
+    shift_cc_tile(d, blockIdx.x);
 }
 
 /* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
@@ -915,6 +1206,22 @@ __global__ __launch_bounds__(HB_BLOCK) void kb_shift(const hb_dev *__restrict__ 
     shift_tile<WM, V, U>(d, t % d.shift_gx, t / d.shift_gx);
 }
 
+/* combined-coefficient flush: tab[2t+1] = list position */
+__global__ __launch_bounds__(HB_BLOCK) void kb_flush_coef(const hb_dev *__restrict__ devs,
+                                                          const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    flush_coef_body(d, tab[2 * blockIdx.x + 1], d.coef, d.ccoef);
+}
+
+/* tab[2t+1] = 256-element tile (the problem's shift grid has one row) */
+__global__ __launch_bounds__(HB_BLOCK) void kb_shift_cc(const hb_dev *__restrict__ devs,
+                                                        const int *__restrict__ tab)
+{
+    const hb_dev &d = devs[tab[2 * blockIdx.x]];
+    shift_cc_tile(d, tab[2 * blockIdx.x + 1]);
+}
+
 __global__ __launch_bounds__(HB_BLOCK) void kb_init(const hb_dev *__restrict__ devs,
                                                     const int *__restrict__ tab)
 {
@@ -968,7 +1275,8 @@ static void launch_shift(const hb_dev *d, hipStream_t s)
 template <int WM>
 static void launch_shift_b(const hb_batch *B, hipStream_t s)
 {
-    hipLaunchKernelGGL((kb_shift<WM, 1, 4>), dim3(B->n_shift_tiles), dim3(HB_BLOCK), 0, s,
+    constexpr int U = WM >= 8 ? 2 : 4;
+    hipLaunchKernelGGL((kb_shift<WM, 1, U>), dim3(B->n_shift_tiles), dim3(HB_BLOCK), 0, s,
                        B->devs, B->t_shift);
 }
 
@@ -1010,9 +1318,18 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
         else hipLaunchKernelGGL(k_fin_seed<false>, one, bl, 0, s, *d);
         break;
     default:
-        /* windows up to 4 move 16-byte pairs; longer windows hold one
-         * element per thread (the window's r and q stay in registers) with
-         * four shifts in flight */
+        if (!d->exact) {
+            /* combined coefficients: one workgroup per list position, then
+             * the streaming flush over 256-element tiles */
+            hipLaunchKernelGGL(k_flush_coef, dim3(d->M), bl, 0, s, *d);
+            hipLaunchKernelGGL(k_shift_cc, dim3(d->shift_gx), bl, 0, s, *d);
+            break;
+        }
+        /* exact order: windows up to 4 move 16-byte pairs; longer windows
+         * hold one element per thread (the window's r and q stay in
+         * registers) with two shifts in flight: four cost registers
+         * (occupancy 4 instead of 5 waves per SIMD) and measured 5% slower
+         * at 128^3, 64 shifts */
         if (d->W <= 1) {
             if (d->vec2) launch_shift<1, 2, 2>(d, s);
             else launch_shift<1, 1, 4>(d, s);
@@ -1020,9 +1337,9 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
             if (d->vec2) launch_shift<4, 2, 2>(d, s);
             else launch_shift<4, 1, 4>(d, s);
         } else if (d->W <= 8) {
-            launch_shift<8, 1, 4>(d, s);
+            launch_shift<8, 1, 2>(d, s);
         } else {
-            launch_shift<16, 1, 4>(d, s);
+            launch_shift<16, 1, 2>(d, s);
         }
         break;
     }
@@ -1102,6 +1419,11 @@ int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s)
         else hipLaunchKernelGGL(kb_fin_seed<false>, gp, bl, 0, s, B->devs);
         break;
     default:
+        if (!B->exact) {
+            hipLaunchKernelGGL(kb_flush_coef, dim3(B->n_coef_tiles), bl, 0, s, B->devs, B->t_coef);
+            hipLaunchKernelGGL(kb_shift_cc, dim3(B->n_shift_tiles), bl, 0, s, B->devs, B->t_shift);
+            break;
+        }
         if (B->W <= 1) launch_shift_b<1>(B, s);
         else if (B->W <= 4) launch_shift_b<4>(B, s);
         else if (B->W <= 8) launch_shift_b<8>(B, s);

@@ -13,7 +13,10 @@ extern "C" {
 #define HB_BLOCK        256   /* threads per workgroup of every kernel        */
 #define HB_NNZ_CAP      2048  /* nonzeros staged in LDS by one SpMV workgroup */
 #define HB_ROWS_CAP     256   /* rows per SpMV workgroup                       */
-#define HB_WMAX         16    /* largest shift-update window                   */
+#define HB_WMAX         64    /* largest shift-update window                   */
+#define HB_WMAX_EXACT   16    /* largest window of the exact-order replay      */
+#define HB_UG           16    /* shifts per chunk of the combined-coefficient flush */
+#define HB_CC_ROWS(W)   (2 * (W) + 5) /* rows of one chunk's coefficient block  */
 
 /* Error bits in hb_state.status */
 #define HB_ST_BREAKDOWN_RTS   1   /* (r^,s) == 0 or non-finite alpha          */
@@ -51,8 +54,19 @@ typedef struct hb_state {
     int flush_cnt;     /* shifts in the flush list                        */
     int k4_epoch;      /* k when the shift kernel last ran (runs while < k) */
     int status;        /* HB_ST_* bits                                    */
-    int pad;
+    int flush_tail;    /* 1: the flush applies its last iteration on its own */
 } hb_state;
+
+/* Every pointer below addresses global device memory.  Device code sees
+ * that in the pointee type (address space 1), so kernels that read a record
+ * from device memory (the batched ones) still get global and scalar memory
+ * instructions rather than flat ones; host code sees plain pointers of the
+ * same size. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HB_G __attribute__((address_space(1)))
+#else
+#define HB_G
+#endif
 
 /* Device pointers and sizes of one solve (passed by value to kernels). */
 typedef struct hb_dev {
@@ -63,29 +77,33 @@ typedef struct hb_dev {
     int shift_group;           /* shifts per shift-kernel workgroup row    */
     int vec2;                  /* 1: shift kernel moves 2 doubles per access (n even) */
     double tol;
-    const double   *val;
-    const unsigned *col;
-    const unsigned *ptr;
-    const unsigned *blk;       /* SpMV workgroup row starts [nblk+1]       */
-    const double   *sigma;     /* [M]                                      */
-    const double   *rhat;      /* [n]   shadow residual r^ = b             */
-    double *rring;             /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */
-    double *qring;             /* [W][n]     q of iteration k at slot k mod W            */
-    double *s, *y;             /* [n] seed SpMV results                    */
-    double *x_set, *p_set;     /* [M][n] shift-major                       */
-    double *aa, *ba, *wa;      /* seed archives alpha~, beta~, omega~ [cap] */
-    double *pi;                /* [M][cap]                                 */
-    double *eta, *zeta;        /* [M]                                      */
-    double *coef;              /* [W][M][6] vector coefficients by window iteration and list position */
-    int    *stop;              /* [M]                                      */
-    int    *stop_iter;         /* [M]                                      */
-    int    *act_len;           /* [M] by list position: window iterations that shift was updated in */
-    int    *flist;             /* [M] the window's shift list, ascending   */
-    int    *fpos;              /* [M] position of shift j in the list      */
-    int    *switch_k, *switch_to; /* [M] switch log                        */
-    double *part;              /* workgroup partial sums [max(nblk,vec_grid)*2] */
-    hb_state *st;
-    const double *b_in;        /* right-hand side read by the initialisation */
+    double growth_limit;       /* > 0: combined-window growth that closes a window (tail) */
+    HB_G const double   *val;
+    HB_G const unsigned *col;
+    HB_G const unsigned *ptr;
+    HB_G const unsigned *blk;  /* SpMV workgroup row starts [nblk+1]       */
+    HB_G const double   *sigma;/* [M]                                      */
+    HB_G const double   *rhat; /* [n]   shadow residual r^ = b             */
+    HB_G double *rring;        /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */
+    HB_G double *qring;        /* [W][n]     q of iteration k at slot k mod W            */
+    HB_G double *s, *y;        /* [n] seed SpMV results                    */
+    HB_G double *x_set, *p_set;/* [M][n] shift-major                       */
+    HB_G double *aa, *ba, *wa; /* seed archives alpha~, beta~, omega~ [cap] */
+    HB_G double *pi;           /* [M][cap]                                 */
+    HB_G double *eta, *zeta;   /* [M]                                      */
+    HB_G double *coef;         /* [W][M][6] vector coefficients by window iteration and list position */
+    HB_G double *ccoef;        /* [ceil(M/HB_UG)][2W+5][2*HB_UG] combined window coefficients */
+    HB_G double *grow;         /* [2][M] size bound of each shift's combined p, and its
+                                  last iteration's own terms */
+    HB_G int    *stop;         /* [M]                                      */
+    HB_G int    *stop_iter;    /* [M]                                      */
+    HB_G int    *act_len;      /* [M] by list position: window iterations that shift was updated in */
+    HB_G int    *flist;        /* [M] the window's shift list, ascending   */
+    HB_G int    *fpos;         /* [M] position of shift j in the list      */
+    HB_G int    *switch_k, *switch_to; /* [M] switch log                   */
+    HB_G double *part;         /* workgroup partial sums [max(nblk,vec_grid)*2] */
+    HB_G hb_state *st;
+    HB_G const double *b_in;   /* right-hand side read by the initialisation */
     int seed_in;               /* first seed                               */
     int pad_;
 } hb_dev;
@@ -95,11 +113,12 @@ typedef struct hb_dev {
  * launch covers the tiles of every problem. */
 typedef struct hb_batch {
     int nprob, exact, W;
-    int n_spmv_tiles, n_vec_tiles, n_shift_tiles;
+    int n_spmv_tiles, n_vec_tiles, n_shift_tiles, n_coef_tiles;
     const hb_dev *devs;        /* [nprob] device array                     */
     const int *t_spmv;         /* [2*n_spmv_tiles]  (problem, SpMV tile)   */
     const int *t_vec;          /* [2*n_vec_tiles]   (problem, vector tile) */
     const int *t_shift;        /* [2*n_shift_tiles] (problem, by*gx + bx)  */
+    const int *t_coef;         /* [2*n_coef_tiles]  (problem, list position) */
 } hb_batch;
 
 #ifdef __cplusplus

@@ -44,7 +44,7 @@ class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("window", C.c_int),
                 ("exact_order", C.c_int), ("graph", C.c_int), ("chunk", C.c_int),
                 ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int),
-                ("profile", C.c_int)]
+                ("profile", C.c_int), ("growth_limit", C.c_double)]
 
 
 class KernelStat(C.Structure):

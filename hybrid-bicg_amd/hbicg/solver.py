@@ -18,7 +18,8 @@ VARIANTS = ("normal", "static", "dynamic")
 
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
-                device=0, exit_on_error=False, upload_x=True, profile=False) -> _lib.Params:
+                device=0, exit_on_error=False, upload_x=True, profile=False,
+                growth_limit=0.0) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -30,6 +31,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.exit_on_error = int(bool(exit_on_error))
     p.upload_x = int(bool(upload_x))
     p.profile = int(bool(profile))
+    p.growth_limit = float(growth_limit)
     return p
 
 

@@ -31,14 +31,22 @@ extern "C" {
 typedef struct hbicg_params {
     double tol;          /* stopping rule, default EPS (1e-12)                           */
     int    max_iter;     /* iteration cap, default MAX_ITER (2000)                       */
-    int    window;       /* shift-update window 1..16; 0 = variant default (normal 1, else 8) */
-    int    exact_order;  /* 1: serial dots and row-sequential SpMV (reference order)     */
+    int    window;       /* shift-update window 1..64 (exact_order: capped at 16); 0 = variant
+                            default: normal 1, static/dynamic 64 (exact_order 16)           */
+    int    exact_order;  /* 1: serial dots, row-sequential SpMV and per-iteration replay of
+                            the shift updates (reference order and roundings, any window);
+                            0: tree reductions and the combined-coefficient window flush
+                            (fused multiply-adds; x_j depend on the window by roundings) */
     int    graph;        /* -1 variant default (dynamic on), 0 off, 1 on                  */
     int    chunk;        /* iterations between host checks of the done flag (default 16) */
     int    device;       /* HIP device ordinal                                            */
     int    exit_on_error;/* 1 (default): non-square input prints and exit(1) like the reference */
     int    upload_x;     /* 1 (default): x_loc_set copied to the GPU (honours x0 != 0)   */
     int    profile;      /* 1: time every kernel with HIP events (direct launches)       */
+    double growth_limit; /* default mode, > 0: a window closes early, its last iteration
+                            applied on its own, when the size bound of a shift's regrouped
+                            p exceeds this multiple of one iteration's own terms (0: off;
+                            the regrouped coefficients are exact to double precision)    */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -126,12 +134,15 @@ typedef struct hbicg_batch hbicg_batch;
 
 typedef struct hbicg_tuning {
     int    window;       /* shift-update window chosen                          */
-    int    group_cap;    /* shifts per shift-kernel workgroup (0: all of a problem) */
+    int    group_cap;    /* exact order: shifts per shift-kernel workgroup (0: all
+                            of a problem); always 0 in the default mode         */
     double ms_per_iter;  /* measured time per batch iteration                   */
 } hbicg_tuning;
 
 /* A[p] (host, a_on_device = 0, or device memory), M[p] shifts each.
- * params: tol, max_iter, window (0: 16), exact_order, graph (0 off), chunk. */
+ * params: tol, max_iter, window (0: 64, exact_order 16), exact_order, graph
+ * (0 off), chunk.  Ring storage is sized for the largest window the mode
+ * allows, so autotuning can change it. */
 hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, const int *M,
                                 const hbicg_params *p, void *stream);
 void hbicg_batch_destroy(hbicg_batch *bt);
@@ -161,8 +172,9 @@ int hbicg_adjoint_gradients(const hbicg_csr *A, int M, const double *mu, const d
                             const double *x, double *grad_b, double *grad_sigma, double *grad_val,
                             void *stream);
 
-/* Times candidate (window, shifts-per-workgroup) layouts on the batch for
- * `iters` iterations each and keeps the fastest (x is overwritten). */
+/* Times candidate layouts on the batch for `iters` iterations each and keeps
+ * the fastest (x is overwritten): windows 16/32/64 in the default mode,
+ * (window 4/8/16) x (shifts per workgroup all/16/4) in exact order. */
 int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *const *sigma,
                          const int *seed, double *const *x, int iters, hbicg_tuning *best);
 

@@ -36,7 +36,7 @@ def c2():
     return P.config2()
 
 
-def _capped(prob, cap, exact, window=16):
+def _capped(prob, cap, exact, window=0):
     pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=cap, exact_order=exact,
                                           window=window, graph=1))
     b = torch.as_tensor(prob.b).cuda()
@@ -71,7 +71,7 @@ def test_full_solve_true_residuals(name, request):
     b = torch.as_tensor(prob.b).cuda()
     xs = []
     for graph in (1, 0):
-        pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, window=16, graph=graph))
+        pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, graph=graph))
         k, x, rep = pl.solve(b, prob.sigma, prob.seed)
         _, stop, sw = pl.report()
         tr = pl.true_residuals(b, prob.sigma, x)

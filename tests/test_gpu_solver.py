@@ -58,25 +58,74 @@ def test_tree_parity_c0(c0, oracle_c0, variant, seed):
     assert tr.max() <= 10 * c0.tol, tr
 
 
-def test_variants_bitwise_equal(c0):
+def _same_up_to_window(got, base, seed, switches):
+    """Default mode, two windows: without a seed switch the seed's own
+    iteration never reads a shift's vectors, so k, every stop iteration, r
+    and the seed's x agree exactly and the other x_j only by the roundings of
+    the regrouped window sums.  After a switch the new seed continues from
+    its windowed p_j, so the rest of the run moves by roundings too."""
+    rel = np.linalg.norm(got.x - base.x, axis=1) / np.linalg.norm(base.x, axis=1)
+    if not switches:
+        assert got.k == base.k
+        np.testing.assert_array_equal(got.stop_iter, base.stop_iter)
+        assert np.array_equal(got.r, base.r)
+        assert np.array_equal(got.x[seed], base.x[seed])
+        assert rel.max() <= 1e-10, rel.max()
+    else:
+        assert abs(got.k - base.k) <= 2
+        assert np.all(np.abs(got.stop_iter - base.stop_iter) <= 2)
+        assert [s for _, s in got.switches] == [s for _, s in base.switches]
+        assert rel.max() <= 1e-9, rel.max()
+
+
+@pytest.mark.parametrize("seed", [0, 8])
+def test_variants_equal(c0, oracle_c0, seed):
     """The three entry points compute the same result (src/solver_{normal,
-    static,dynamic}.c share their arithmetic)."""
-    outs = [dropin_solve(c0.A, c0.b, c0.sigma, 8, v, make_params(tol=c0.tol)) for v in
+    static,dynamic}.c share their arithmetic).  static and dynamic run the
+    same window and agree bit for bit; normal applies every iteration
+    (window 1)."""
+    outs = [dropin_solve(c0.A, c0.b, c0.sigma, seed, v, make_params(tol=c0.tol)) for v in
             ("normal", "static", "dynamic")]
     for o in outs[1:]:
-        assert o.k == outs[0].k
-        assert np.array_equal(o.x, outs[0].x)
-        assert np.array_equal(o.r, outs[0].r)
+        _same_up_to_window(o, outs[0], seed, oracle_c0[seed].switches)
+    assert outs[1].k == outs[2].k
+    assert np.array_equal(outs[1].x, outs[2].x)
 
 
 @pytest.mark.parametrize("window", [1, 3, 8, 16])
-def test_window_bitwise_independent(c0, window):
-    """Deferring the shift updates over a window replays the same per-element
-    operations: x_set must not depend on the window length."""
-    base = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", make_params(tol=c0.tol, window=1))
-    got = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", make_params(tol=c0.tol, window=window))
+def test_window_bitwise_independent_exact(c0, window):
+    """Exact order: deferring the shift updates over a window replays the
+    same per-element operations, so x_set must not depend on the window."""
+    mk = lambda w: make_params(tol=c0.tol, window=w, exact_order=True)
+    base = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", mk(1))
+    got = dropin_solve(c0.A, c0.b, c0.sigma, 8, "static", mk(window))
     assert got.k == base.k
     assert np.array_equal(got.x, base.x)
+
+
+@pytest.mark.parametrize("seed", [0, 8])
+@pytest.mark.parametrize("window", [3, 8, 16, 33, 64])
+def test_window_independent_default(c0, oracle_c0, window, seed):
+    """Default mode: the combined-coefficient flush regroups each window's
+    sums; results move only by roundings as the window changes."""
+    mk = lambda w: make_params(tol=c0.tol, window=w)
+    base = dropin_solve(c0.A, c0.b, c0.sigma, seed, "static", mk(1))
+    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, "static", mk(window))
+    _same_up_to_window(got, base, seed, oracle_c0[seed].switches)
+
+
+@pytest.mark.parametrize("seed", [0, 8])
+def test_window_tail_path(c0, oracle_c0, seed):
+    """growth_limit below 1 closes every window after its first iteration
+    with that iteration applied on its own (the tail path of the flush):
+    the result is the per-iteration order, close to window 1 and to the
+    oracle, and every shift meets the stopping rule."""
+    base = dropin_solve(c0.A, c0.b, c0.sigma, seed, "static", make_params(tol=c0.tol, window=1))
+    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic",
+                       make_params(tol=c0.tol, window=64, growth_limit=0.5))
+    _same_up_to_window(got, base, seed, oracle_c0[seed].switches)
+    tr = O.true_residuals(c0.A, got.x, c0.b, c0.sigma)
+    assert tr.max() <= 10 * c0.tol, tr.max()
 
 
 @pytest.mark.parametrize("seed", [0, 8, 15])
