@@ -8,8 +8,12 @@ complete shifted solve (x zeroed, all 64 systems to the stopping rule) with
 A, b and x already resident in HBM.
 
 Printed: one JSON line with the contract fields plus
-  roofline      the dominant kernel's algorithmic bytes / HIP-event time,
-                measured in a profiled solve of the same workload;
+  roofline      the dominant kernel's algorithmic bytes per launch over its
+                average duration, timed with HIP events in a profiled solve
+                of the same workload (back-to-back replays of the kernel
+                between two events, so no per-launch event gap is counted);
+                traffic: HBM bytes per launch from the committed rocprofv3
+                FETCH_SIZE / WRITE_SIZE summary of this workload, when there;
   cpu_baseline  the oracle (CPU restatement of the reference) timed on a
                 bounded sample of the same solve on this host, extrapolated
                 to the full solve from its seed-step and per-shift timers.
@@ -30,6 +34,19 @@ METRIC = "shift-solves/s @ relres 1e-10; SpMV %HBM peak; batched & adjoint vs pl
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (profiles/pmc_traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = table.get(workload, {}).get(kernel)
+    return None if entry is None else entry["bytes_per_launch"]
 
 
 def parse():
@@ -230,12 +247,15 @@ def main():
         del xp
         dom = max(kernels, key=lambda name: kernels[name]["total_ms"])
         kd = kernels[dom]
-        avg_ms = kd["total_ms"] / max(kd["launches"], 1)
+        single_ms = kd["total_ms"] / max(kd["launches"], 1)
+        avg_ms = kd["replay_ms"] / kd["replays"] if kd.get("replays") else single_ms
         per_launch = kd["bytes"] / max(kd["launches"], 1)
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(dom, f"{args.grid}^3/{args.shifts}")
         roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                    "traffic": None, "avg_launch_ms": round(avg_ms, 5),
+                    "traffic": traffic, "avg_launch_ms": round(avg_ms, 5),
+                    "avg_single_launch_ms": round(single_ms, 5),
                     "algorithmic_bytes_per_launch": per_launch}
         for name, kd in kernels.items():
             kd["frac"] = round(kd["gbps"] / HBM_PEAK_GBPS, 4)

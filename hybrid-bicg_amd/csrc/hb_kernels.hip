@@ -1295,33 +1295,36 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
     const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK), one(1);
     switch (stage) {
     case HB_STAGE_SPMV1:
-        if (d->exact) {
-            hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), one, bl, 0, s, *d);
-        } else {
-            hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), one, bl, 0, s, *d);
-        }
+        if (d->exact) hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d);
+        else hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d);
+        break;
+    case HB_STAGE_FIN1:
+        if (d->exact) hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), one, bl, 0, s, *d);
+        else hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), one, bl, 0, s, *d);
         break;
     case HB_STAGE_SPMV2:
-        if (d->exact) {
-            hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), one, bl, 0, s, *d);
-        } else {
-            hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d);
-            hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, false>), one, bl, 0, s, *d);
-        }
+        if (d->exact) hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d);
+        else hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d);
+        break;
+    case HB_STAGE_FIN2:
+        if (d->exact) hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), one, bl, 0, s, *d);
+        else hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, false>), one, bl, 0, s, *d);
         break;
     case HB_STAGE_SEED:
         hipLaunchKernelGGL(k_seed_update, gv, bl, 0, s, *d);
+        break;
+    case HB_STAGE_SCALAR:
         if (d->exact) hipLaunchKernelGGL(k_fin_seed<true>, one, bl, 0, s, *d);
         else hipLaunchKernelGGL(k_fin_seed<false>, one, bl, 0, s, *d);
         break;
+    case HB_STAGE_COEF:
+        /* combined coefficients: one workgroup per list position (the exact
+         * replay needs none) */
+        if (!d->exact) hipLaunchKernelGGL(k_flush_coef, dim3(d->M), bl, 0, s, *d);
+        break;
     default:
         if (!d->exact) {
-            /* combined coefficients: one workgroup per list position, then
-             * the streaming flush over 256-element tiles */
-            hipLaunchKernelGGL(k_flush_coef, dim3(d->M), bl, 0, s, *d);
+            /* the streaming flush over 256-element tiles */
             hipLaunchKernelGGL(k_shift_cc, dim3(d->shift_gx), bl, 0, s, *d);
             break;
         }
@@ -1396,31 +1399,38 @@ int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s)
     const dim3 bl(HB_BLOCK), gp(B->nprob);
     switch (stage) {
     case HB_STAGE_SPMV1:
-        if (B->exact) {
+        if (B->exact)
             hipLaunchKernelGGL((kb_spmv<SP_SEED1, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, true>), gp, bl, 0, s, B->devs);
-        } else {
+        else
             hipLaunchKernelGGL((kb_spmv<SP_SEED1, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, false>), gp, bl, 0, s, B->devs);
-        }
+        break;
+    case HB_STAGE_FIN1:
+        if (B->exact) hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, true>), gp, bl, 0, s, B->devs);
+        else hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, false>), gp, bl, 0, s, B->devs);
         break;
     case HB_STAGE_SPMV2:
-        if (B->exact) {
+        if (B->exact)
             hipLaunchKernelGGL((kb_spmv<SP_SEED2, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, true>), gp, bl, 0, s, B->devs);
-        } else {
+        else
             hipLaunchKernelGGL((kb_spmv<SP_SEED2, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-            hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, false>), gp, bl, 0, s, B->devs);
-        }
+        break;
+    case HB_STAGE_FIN2:
+        if (B->exact) hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, true>), gp, bl, 0, s, B->devs);
+        else hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, false>), gp, bl, 0, s, B->devs);
         break;
     case HB_STAGE_SEED:
         hipLaunchKernelGGL(kb_seed_update, dim3(B->n_vec_tiles), bl, 0, s, B->devs, B->t_vec);
+        break;
+    case HB_STAGE_SCALAR:
         if (B->exact) hipLaunchKernelGGL(kb_fin_seed<true>, gp, bl, 0, s, B->devs);
         else hipLaunchKernelGGL(kb_fin_seed<false>, gp, bl, 0, s, B->devs);
         break;
+    case HB_STAGE_COEF:
+        if (!B->exact)
+            hipLaunchKernelGGL(kb_flush_coef, dim3(B->n_coef_tiles), bl, 0, s, B->devs, B->t_coef);
+        break;
     default:
         if (!B->exact) {
-            hipLaunchKernelGGL(kb_flush_coef, dim3(B->n_coef_tiles), bl, 0, s, B->devs, B->t_coef);
             hipLaunchKernelGGL(kb_shift_cc, dim3(B->n_shift_tiles), bl, 0, s, B->devs, B->t_shift);
             break;
         }

@@ -18,17 +18,23 @@ extern "C" {
  * (b,b) reduced on the device. */
 int hb_k_init(const hb_dev *d, hipStream_t s);
 
-/* One iteration: seed SpMV 1, fused q + SpMV 2, seed update with the scalar
- * stage (shift recurrences, convergence check, seed switch), and the shift
- * window kernel.  Each kernel returns at once when the solve is done. */
+/* One iteration: seed SpMV 1, fused q + SpMV 2, seed update, each with its
+ * one-workgroup finalize (the last one is the scalar stage: shift
+ * recurrences, convergence check, seed switch), then the window flush
+ * (combined coefficients and the shift kernel).  Each kernel returns at once
+ * when the solve is done. */
 int hb_k_iteration(const hb_dev *d, hipStream_t s);
 
-/* The four kernels of an iteration, one at a time (for per-kernel timing). */
-#define HB_STAGE_SPMV1 0
-#define HB_STAGE_SPMV2 1
-#define HB_STAGE_SEED  2
-#define HB_STAGE_SHIFT 3
-#define HB_NSTAGE      4
+/* The kernels of an iteration, one launch at a time (per-kernel timing). */
+#define HB_STAGE_SPMV1  0   /* SpMV s = (A + sigma_s I) p_s, (r^, s) partials   */
+#define HB_STAGE_FIN1   1   /* alpha~                                           */
+#define HB_STAGE_SPMV2  2   /* q = r - alpha~ s fused into y = (A + sigma_s I) q */
+#define HB_STAGE_FIN2   3   /* omega~                                           */
+#define HB_STAGE_SEED   4   /* x_s, r updates, (r,r), (r^,r) partials           */
+#define HB_STAGE_SCALAR 5   /* beta~ and the scalar stage                       */
+#define HB_STAGE_COEF   6   /* combined window coefficients (default mode)      */
+#define HB_STAGE_SHIFT  7   /* seed p update and the window flush               */
+#define HB_NSTAGE       8
 int hb_k_stage(const hb_dev *d, int stage, hipStream_t s);
 
 /* Copies {done, k} of the device state into host-visible memory. */

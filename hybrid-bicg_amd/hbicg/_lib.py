@@ -49,7 +49,8 @@ class Params(C.Structure):
 
 class KernelStat(C.Structure):
     """include/hbicg.h hbicg_kernel_stat."""
-    _fields_ = [("launches", C.c_int), ("total_ms", C.c_double), ("bytes", C.c_double)]
+    _fields_ = [("launches", C.c_int), ("total_ms", C.c_double), ("bytes", C.c_double),
+                ("replays", C.c_int), ("replay_ms", C.c_double)]
 
 
 class Report(C.Structure):

@@ -172,18 +172,23 @@ class Plan:
                                       swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), self.M)
         return _report_dict(rep), stop, [(int(swk[i]), int(swt[i])) for i in range(ns)]
 
-    KERNELS = ("spmv_seed", "spmv_q_fused", "seed_update", "shift_window")
+    KERNELS = ("spmv_seed", "fin_spmv_seed", "spmv_q_fused", "fin_spmv_q", "seed_update",
+               "scalar_stage", "flush_coef", "shift_window")
 
     def kernel_stats(self):
-        """Per-kernel {launches, total_ms, bytes, gbps} of the last profiled solve."""
-        arr = (_lib.KernelStat * 4)()
-        m = self.L.hbicg_plan_kernel_stats(self.h, arr, 4)
+        """Per-kernel {launches, total_ms, bytes, gbps} of the last profiled
+        solve, one entry per launch of an iteration (include/hbicg.h)."""
+        n = len(self.KERNELS)
+        arr = (_lib.KernelStat * n)()
+        m = self.L.hbicg_plan_kernel_stats(self.h, arr, n)
         out = {}
         for i in range(m):
             st = arr[i]
             gbps = st.bytes / (st.total_ms * 1e-3) / 1e9 if st.total_ms > 0 else 0.0
             out[self.KERNELS[i]] = {"launches": st.launches, "total_ms": st.total_ms,
                                     "bytes": st.bytes, "gbps": gbps}
+            if st.replays:
+                out[self.KERNELS[i]].update(replays=st.replays, replay_ms=st.replay_ms)
         return out
 
     def true_residuals(self, b, sigma, x):

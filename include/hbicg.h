@@ -110,14 +110,18 @@ int hbicg_plan_true_residuals(hbicg_plan *pl, const double *b, const double *sig
 int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
 
 /* Per-kernel statistics of the plan's last solve run with params.profile:
- * stats[0] seed SpMV 1, [1] fused q + SpMV 2, [2] seed update + scalar
- * stage, [3] shift window kernel.  launches counts executed iterations,
+ * one entry per launch of an iteration: [0] seed SpMV 1, [1] its finalize
+ * (alpha~), [2] fused q + SpMV 2, [3] its finalize (omega~), [4] seed
+ * update, [5] scalar stage, [6] combined window coefficients, [7] seed p
+ * update + window flush.  launches counts executed iterations,
  * total_ms the summed HIP-event durations, bytes the algorithmic HBM bytes
  * those launches had to move.  Returns the number of entries filled. */
 typedef struct hbicg_kernel_stat {
     int    launches;
     double total_ms;
     double bytes;
+    int    replays;      /* [2] only: extra back-to-back launches timed on their own */
+    double replay_ms;    /*   (same inputs and outputs, no event gaps between them)  */
 } hbicg_kernel_stat;
 int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax);
 
