@@ -98,6 +98,9 @@ struct hbicg_plan {
     /* matrix */
     double *val;
     unsigned *col, *ptr, *blk;
+    short *col16;
+    unsigned char *cflag;
+    double col_bytes;    /* column-index bytes per SpMV (16-bit tiles count 2 per entry) */
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
@@ -165,7 +168,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->gexec) hipGraphExecDestroy(pl->gexec);
     if (pl->graph_obj) hipGraphDestroy(pl->graph_obj);
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
-    void *bufs[] = {pl->blk, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
+    void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
                     pl->coef, pl->ccoef, pl->grow, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
@@ -336,6 +339,31 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->stop_iter = pl->stop_iter;
     d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
     d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st;
+
+    /* 16-bit column offsets for the tiles whose entries stay within 32767
+     * columns of the tile's first row */
+    pl->col16 = (short *)dalloc((size_t)pl->nz * 2, &rc);
+    pl->cflag = (unsigned char *)dalloc((size_t)pl->nblk, &rc);
+    if (rc) goto fail;
+    if (hb_k_compress_cols(d, pl->col16, pl->cflag, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    HIPCK(hipStreamSynchronize(pl->stream));
+    d->col16 = pl->col16;
+    d->cflag = pl->cflag;
+    {
+        /* column-index bytes one SpMV reads, for the kernel statistics */
+        unsigned char *hf = (unsigned char *)malloc((size_t)pl->nblk);
+        if (!hf) { rc = HBICG_E_NOMEM; goto fail; }
+        if (hipMemcpy(hf, pl->cflag, (size_t)pl->nblk, hipMemcpyDeviceToHost) != hipSuccess) {
+            free(hf);
+            rc = HBICG_E_HIP;
+            goto fail;
+        }
+        double cb = (double)pl->nblk;
+        for (int t = 0; t < pl->nblk; ++t)
+            cb += (hf[t] ? 2.0 : 4.0) * (double)(hptr[hblk[t + 1]] - hptr[hblk[t]]);
+        pl->col_bytes = cb;
+        free(hf);
+    }
 
     if (a_on_device) free(hptr);
     free(hblk);
@@ -567,7 +595,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->t_total_ms = now_ms() - t_start;
     if (pl->profile) {
         const double n = (double)pl->n, nz = (double)pl->nz;
-        const double spmv = 12.0 * nz + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
+        const double spmv = 8.0 * nz + pl->col_bytes + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
         /* algorithmic bytes per launch: SpMV 1 gathers p_s and reads r^
          * (writes s), SpMV 2 reads r, s and r^ (writes q and y), the seed
          * update reads q, y, p_s, x_s, r^ (writes x_s, r); the finalizes

@@ -134,7 +134,8 @@ enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 
 template <int MODE, bool EXACT>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
-                   const unsigned *__restrict__ col, const unsigned *__restrict__ ptr,
+                   const unsigned *__restrict__ col, const short *__restrict__ col16,
+                   const unsigned char *__restrict__ cflag, const unsigned *__restrict__ ptr,
                    const unsigned *__restrict__ blk, const double *__restrict__ xbase,
                    double *__restrict__ rring, const double *__restrict__ svec,
                    const double *__restrict__ rhat, double *__restrict__ out1,
@@ -205,8 +206,16 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
 
     if (N <= HB_NNZ_CAP) {
         for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
+        if (cflag[b]) {
+            /* 16-bit column offsets from the tile's first row: 2 bytes per
+             * entry instead of 4 */
 #pragma unroll 4
-        for (int e = tid; e < N; e += HB_BLOCK) prod[e] = val[e0 + e] * xval(col[e0 + e]);
+            for (int e = tid; e < N; e += HB_BLOCK)
+                prod[e] = val[e0 + e] * xval(r0 + (unsigned)(int)col16[e0 + e]);
+        } else {
+#pragma unroll 4
+            for (int e = tid; e < N; e += HB_BLOCK) prod[e] = val[e0 + e] * xval(col[e0 + e]);
+        }
         __syncthreads();
         int T = 1;
         if (!EXACT) {
@@ -246,10 +255,10 @@ template <int MODE, bool EXACT>
 DEV void spmv_tile(const hb_dev &d, unsigned b)
 {
     if (MODE == SP_SEED1)
-        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.ptr, d.blk, d.p_set, d.rring, nullptr,
+        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, d.p_set, d.rring, nullptr,
                                d.rhat, d.s, nullptr, d.part, 0.0);
     else
-        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
+        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
                                d.y, d.qring, d.part, 0.0);
 }
 
@@ -259,11 +268,33 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
     spmv_tile<MODE, EXACT>(d, blockIdx.x);
 }
 
+/* Tile b of the SpMV: col16[e] = col[e] - (tile's first row) for its
+ * entries, and cflag[b] = 1 when every such offset fits in 16 bits (banded
+ * and stencil matrices; the SpMV then reads 2 bytes per column index).
+ * Built once per plan. */
+__global__ __launch_bounds__(HB_BLOCK) void k_compress_cols(const unsigned *__restrict__ ptr,
+                                                            const unsigned *__restrict__ col,
+                                                            const unsigned *__restrict__ blk,
+                                                            short *__restrict__ col16,
+                                                            unsigned char *__restrict__ cflag)
+{
+    const unsigned b = blockIdx.x;
+    const unsigned r0 = blk[b], e0 = ptr[r0], e1 = ptr[blk[b + 1]];
+    int ok = 1;
+    for (unsigned e = e0 + threadIdx.x; e < e1; e += HB_BLOCK) {
+        const long long o = (long long)col[e] - (long long)r0;
+        if (o < -32768 || o > 32767) ok = 0;
+        else col16[e] = (short)o;
+    }
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0) cflag[b] = (unsigned char)ok;
+}
+
 template <bool EXACT>
 __global__ __launch_bounds__(HB_BLOCK) void k_spmv_plain(hb_dev d, const double *x, double *y,
                                                          double shift)
 {
-    spmv_body<SP_PLAIN, EXACT>(d, blockIdx.x, d.val, d.col, d.ptr, d.blk, x, nullptr, nullptr,
+    spmv_body<SP_PLAIN, EXACT>(d, blockIdx.x, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, x, nullptr, nullptr,
                                nullptr, y, nullptr, nullptr, shift);
 }
 
@@ -1375,6 +1406,13 @@ int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *o
     const int nb = (d->n + HB_BLOCK - 1) / HB_BLOCK;
     hipLaunchKernelGGL(k_resid_part, dim3(nb, d->M), dim3(HB_BLOCK), 0, s, *d, b, part);
     hipLaunchKernelGGL(k_resid_final, dim3(d->M), dim3(HB_BLOCK), 0, s, part, nb, out);
+    return (int)hipGetLastError();
+}
+
+int hb_k_compress_cols(const hb_dev *d, short *col16, unsigned char *cflag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_compress_cols, dim3(d->nblk), dim3(HB_BLOCK), 0, s, d->ptr, d->col, d->blk,
+                       col16, cflag);
     return (int)hipGetLastError();
 }
 
