@@ -40,6 +40,7 @@ void hbicg_default_params(hbicg_params *p)
     p->upload_x = 1;
     p->profile = 0;
     p->growth_limit = 0.0;
+    p->trace = 0;
 }
 
 static const hbicg_params *params(void)
@@ -90,7 +91,7 @@ static double now_ms(void)
 
 /* ---- plan ------------------------------------------------------------------ */
 struct hbicg_plan {
-    int n, M, W, cap, exact, graph, chunk, upload_x, device, profile, vec2;
+    int n, M, W, cap, exact, graph, chunk, upload_x, device, profile, vec2, want_trace;
     double tol, growth_limit;
     unsigned nz;
     hipStream_t stream;
@@ -104,7 +105,7 @@ struct hbicg_plan {
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
-    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *ccoef, *grow, *part;
+    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *ccoef, *grow, *part, *trace;
     int *stop, *stop_iter, *act_len, *flist, *fpos, *switch_k, *switch_to;
     hb_state *st;
     int *flag_host;      /* pinned: {done, k} */
@@ -170,7 +171,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -225,6 +226,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->cap = p->max_iter + 1;
     pl->tol = p->tol;
     pl->growth_limit = p->growth_limit > 0.0 ? p->growth_limit : 0.0;
+    pl->want_trace = p->trace ? 1 : 0;
     pl->exact = p->exact_order ? 1 : 0;
     pl->graph = p->graph < 0 ? (variant == 2) : (p->graph ? 1 : 0);
     pl->chunk = p->chunk > 0 ? p->chunk : 16;
@@ -299,6 +301,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     const size_t ccn = (size_t)((M + HB_UG - 1) / HB_UG) * HB_CC_ROWS(W) * (2 * HB_UG);
     pl->ccoef = (double *)dalloc(ccn * 8, &rc);
     pl->grow = (double *)dalloc((size_t)2 * M * 8, &rc);
+    if (pl->want_trace) pl->trace = (double *)dalloc(cap * (size_t)M * 8, &rc);
     pl->part = (double *)dalloc((size_t)maxparts * 2 * 8, &rc);
     pl->stop = (int *)dalloc((size_t)M * 4, &rc);
     pl->stop_iter = (int *)dalloc((size_t)M * 4, &rc);
@@ -334,7 +337,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->sigma = pl->sigma; d->rhat = pl->rhat; d->rring = pl->rring; d->qring = pl->qring;
     d->s = pl->s; d->y = pl->y; d->p_set = pl->p_set; d->x_set = NULL;
     d->aa = pl->aa; d->ba = pl->ba; d->wa = pl->wa; d->pi = pl->pi; d->eta = pl->eta;
-    d->zeta = pl->zeta; d->coef = pl->coef; d->ccoef = pl->ccoef; d->grow = pl->grow;
+    d->zeta = pl->zeta; d->coef = pl->coef; d->ccoef = pl->ccoef; d->grow = pl->grow; d->trace = pl->trace;
     d->stop = pl->stop;
     d->stop_iter = pl->stop_iter;
     d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
@@ -558,6 +561,10 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     HIPCK(hipEventRecord(pl->t0, pl->stream));
     pl->dev.b_in = db;
     pl->dev.seed_in = seed;
+    if (pl->trace && hb_k_fill(pl->trace, (size_t)pl->cap * pl->M, -1.0, pl->stream) != 0) {
+        rc = HBICG_E_HIP;
+        goto fail;
+    }
     if (hb_k_init(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
     if ((rc = run_loop(pl)) != 0) goto fail;
     if (hb_k_finalize(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
@@ -629,6 +636,20 @@ fail:
     return rc;
 }
 
+int hbicg_plan_residual_trace(hbicg_plan *pl, double *out, int rows)
+{
+    int rc = 0;
+    if (!pl || !out || rows < 0 || !pl->trace) return HBICG_E_ARG;
+    const int m = rows < pl->rep.k ? rows : pl->rep.k;
+    if (m > 0) {
+        HIPCK(hipSetDevice(pl->device));
+        HIPCK(hipMemcpy(out, pl->trace, (size_t)m * pl->M * 8, hipMemcpyDeviceToHost));
+    }
+    return m;
+fail:
+    return rc;
+}
+
 int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax)
 {
     if (!pl || !stats) return HBICG_E_ARG;
@@ -696,6 +717,17 @@ int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift)
 /* ---- reference entry points ------------------------------------------------ */
 static hbicg_report g_last;
 static int *g_stop_iter = NULL, *g_switch_k = NULL, *g_switch_to = NULL, g_last_M = 0;
+static double *g_trace = NULL;
+static int g_trace_rows = 0;
+
+int hbicg_last_residual_trace(double *out, int M, int rows)
+{
+    if (!out || rows < 0 || M != g_last_M) return HBICG_E_ARG;
+    if (!g_trace) return 0;
+    const int m = rows < g_trace_rows ? rows : g_trace_rows;
+    memcpy(out, g_trace, (size_t)m * M * sizeof(double));
+    return m;
+}
 
 int hbicg_last_report(hbicg_report *rep, int *stop_iter, int M,
                       int *switch_k, int *switch_to, int max_switch)
@@ -746,6 +778,13 @@ static int dropin(CSR_Matrix *A_diag, CSR_Matrix *A_offd, INFO_Matrix *A_info, d
         if (g_stop_iter) memcpy(g_stop_iter, pl->h_stop_iter, (size_t)M * sizeof(int));
         if (g_switch_k) memcpy(g_switch_k, pl->h_switch_k, (size_t)M * sizeof(int));
         if (g_switch_to) memcpy(g_switch_to, pl->h_switch_to, (size_t)M * sizeof(int));
+        free(g_trace);
+        g_trace = NULL;
+        g_trace_rows = 0;
+        if (pl->trace) {
+            g_trace = (double *)malloc((size_t)k * M * sizeof(double));
+            if (g_trace && hbicg_plan_residual_trace(pl, g_trace, k) == k) g_trace_rows = k;
+        }
     }
     hbicg_plan_destroy(pl);
     return k;

@@ -268,6 +268,14 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
     spmv_tile<MODE, EXACT>(d, blockIdx.x);
 }
 
+/* p[0 .. count) = v (a fill kernel: usable where a memset cannot express v) */
+__global__ __launch_bounds__(HB_BLOCK) void k_fill(double *p, size_t count, double v)
+{
+    for (size_t i = (size_t)blockIdx.x * HB_BLOCK + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * HB_BLOCK)
+        p[i] = v;
+}
+
 /* Tile b of the SpMV: col16[e] = col[e] - (tile's first row) for its
  * entries, and cflag[b] = 1 when every such offset fits in 16 bits (banded
  * and stencil matrices; the SpMV then reads 2 bytes per column index).
@@ -471,6 +479,9 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         } else {
             zp = 1.0;
         }
+        /* the residual estimate the reference prints under
+         * DISPLAY_SIGMA_RESIDUAL (src/solver_dynamic.c:286-292) */
+        if (d.trace) d.trace[(size_t)k * M + j] = zp * sqrt(dot_r / dot_zero);
         if (zp * zp * dot_r <= thr) {
             d.stop[j] = 1;
             d.stop_iter[j] = k;
@@ -1406,6 +1417,12 @@ int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *o
     const int nb = (d->n + HB_BLOCK - 1) / HB_BLOCK;
     hipLaunchKernelGGL(k_resid_part, dim3(nb, d->M), dim3(HB_BLOCK), 0, s, *d, b, part);
     hipLaunchKernelGGL(k_resid_final, dim3(d->M), dim3(HB_BLOCK), 0, s, part, nb, out);
+    return (int)hipGetLastError();
+}
+
+int hb_k_fill(double *p, size_t count, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(256), dim3(HB_BLOCK), 0, s, p, count, v);
     return (int)hipGetLastError();
 }
 

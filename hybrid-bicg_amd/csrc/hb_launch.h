@@ -47,6 +47,9 @@ int hb_k_finalize(const hb_dev *d, hipStream_t s);
  * part must hold 2*M*ceil(n/HB_BLOCK) doubles. */
 int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *out, hipStream_t s);
 
+/* p[0 .. count) = v. */
+int hb_k_fill(double *p, size_t count, double v, hipStream_t s);
+
 /* col16 / cflag of the plan's SpMV tiles (see hb_dev); once per plan. */
 int hb_k_compress_cols(const hb_dev *d, short *col16, unsigned char *cflag, hipStream_t s);
 

@@ -104,6 +104,8 @@ typedef struct hb_dev {
     HB_G int    *fpos;         /* [M] position of shift j in the list      */
     HB_G int    *switch_k, *switch_to; /* [M] switch log                   */
     HB_G double *part;         /* workgroup partial sums [max(nblk,vec_grid)*2] */
+    HB_G double *trace;        /* [cap][M] estimated residual per iteration and shift,
+                                  -1 once stopped (NULL: not recorded)     */
     HB_G hb_state *st;
     HB_G const double *b_in;   /* right-hand side read by the initialisation */
     int seed_in;               /* first seed                               */

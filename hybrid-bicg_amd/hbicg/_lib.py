@@ -44,7 +44,7 @@ class Params(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int), ("window", C.c_int),
                 ("exact_order", C.c_int), ("graph", C.c_int), ("chunk", C.c_int),
                 ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int),
-                ("profile", C.c_int), ("growth_limit", C.c_double)]
+                ("profile", C.c_int), ("growth_limit", C.c_double), ("trace", C.c_int)]
 
 
 class KernelStat(C.Structure):
@@ -80,9 +80,10 @@ EXPORTS = {
                  "coo_load_matrix", "coo2csr", "csr_load_matrix", "MPI_csr_load_matrix_block",
                  "mult"],
     "hbicg.h": ["hbicg_default_params", "hbicg_set_params", "hbicg_get_params",
-                "hbicg_last_report", "hbicg_plan_create", "hbicg_plan_destroy", "hbicg_errno",
+                "hbicg_last_report", "hbicg_last_residual_trace", "hbicg_plan_create", "hbicg_plan_destroy", "hbicg_errno",
                 "hbicg_plan_solve", "hbicg_plan_true_residuals", "hbicg_plan_spmv",
-                "hbicg_plan_report", "hbicg_plan_kernel_stats", "hbicg_batch_create",
+                "hbicg_plan_report", "hbicg_plan_residual_trace", "hbicg_plan_kernel_stats",
+                "hbicg_batch_create",
                 "hbicg_batch_destroy", "hbicg_batch_solve", "hbicg_batch_report",
                 "hbicg_batch_true_residuals", "hbicg_batch_autotune", "hbicg_csr_transpose",
                 "hbicg_adjoint_gradients"],
@@ -129,6 +130,10 @@ def load():
     L.hbicg_get_params.restype = C.c_int
     L.hbicg_last_report.argtypes = [C.POINTER(Report), _ip, C.c_int, _ip, _ip, C.c_int]
     L.hbicg_last_report.restype = C.c_int
+    L.hbicg_last_residual_trace.argtypes = [_dp, C.c_int, C.c_int]
+    L.hbicg_last_residual_trace.restype = C.c_int
+    L.hbicg_plan_residual_trace.argtypes = [C.c_void_p, _dp, C.c_int]
+    L.hbicg_plan_residual_trace.restype = C.c_int
     L.hbicg_plan_create.argtypes = [C.POINTER(Csr), C.c_int, C.c_int, C.POINTER(Params), C.c_void_p]
     L.hbicg_plan_create.restype = C.c_void_p
     L.hbicg_plan_destroy.argtypes = [C.c_void_p]

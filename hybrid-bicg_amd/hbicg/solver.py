@@ -19,7 +19,7 @@ VARIANTS = ("normal", "static", "dynamic")
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
                 device=0, exit_on_error=False, upload_x=True, profile=False,
-                growth_limit=0.0) -> _lib.Params:
+                growth_limit=0.0, trace=False) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -32,6 +32,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.upload_x = int(bool(upload_x))
     p.profile = int(bool(profile))
     p.growth_limit = float(growth_limit)
+    p.trace = int(bool(trace))
     return p
 
 
@@ -43,6 +44,7 @@ class SolveResult:
     stop_iter: np.ndarray
     switches: list            # [(k, new_seed)]
     report: dict
+    trace: object = None      # [k, M] residual estimates (params.trace), -1 = stopped
 
 
 def _report_dict(rep: _lib.Report) -> dict:
@@ -86,8 +88,13 @@ def dropin_solve(A, b, sigma, seed, variant="dynamic", params=None, x0=None) -> 
     ip = C.POINTER(C.c_int)
     ns = L.hbicg_last_report(C.byref(rep), stop.ctypes.data_as(ip), M, swk.ctypes.data_as(ip),
                              swt.ctypes.data_as(ip), M)
+    trace = None
+    if p.trace:
+        trace = np.empty((k, M))
+        rows = _lib.check(L.hbicg_last_residual_trace(_dp(trace), M, k), "hbicg_last_residual_trace")
+        trace = trace[:rows]
     return SolveResult(k, x, r, stop, [(int(swk[i]), int(swt[i])) for i in range(ns)],
-                       _report_dict(rep))
+                       _report_dict(rep), trace)
 
 
 class Plan:
@@ -171,6 +178,15 @@ class Plan:
         ns = self.L.hbicg_plan_report(self.h, C.byref(rep), stop.ctypes.data_as(ip),
                                       swk.ctypes.data_as(ip), swt.ctypes.data_as(ip), self.M)
         return _report_dict(rep), stop, [(int(swk[i]), int(swt[i])) for i in range(ns)]
+
+    def residual_trace(self):
+        """[k, M] residual estimates of the last solve (plan made with
+        trace=True): row k is iteration k, -1 where a shift had stopped."""
+        k = self.report()[0]["k"]
+        out = np.empty((k, self.M))
+        rows = _lib.check(self.L.hbicg_plan_residual_trace(self.h, _dp(out), k),
+                          "hbicg_plan_residual_trace")
+        return out[:rows]
 
     KERNELS = ("spmv_seed", "fin_spmv_seed", "spmv_q_fused", "fin_spmv_q", "seed_update",
                "scalar_stage", "flush_coef", "shift_window")

@@ -47,6 +47,7 @@ typedef struct hbicg_params {
                             applied on its own, when the size bound of a shift's regrouped
                             p exceeds this multiple of one iteration's own terms (0: off;
                             the regrouped coefficients are exact to double precision)    */
+    int    trace;        /* 1: record the per-iteration residual estimate of every shift  */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -66,6 +67,14 @@ void hbicg_default_params(hbicg_params *p);
 /* Process-wide parameters used by the shifted_lopbicg_* entry points. */
 int  hbicg_set_params(const hbicg_params *p);
 int  hbicg_get_params(hbicg_params *p);
+
+/* Residual trace of the last drop-in solve (params.trace = 1): row k, shift
+ * j = |1/(zeta_j pi_j[k])| sqrt((r,r)/(b,b)), the estimate the reference
+ * prints under DISPLAY_SIGMA_RESIDUAL (src/solver_dynamic.c:274-292), for the
+ * shifts still running at iteration k (the one that stops included); -1 for
+ * shifts stopped earlier and for row 0.  out is [rows][M] row-major.
+ * Returns the rows written (the solve's k) or a negative code. */
+int  hbicg_last_residual_trace(double *out, int M, int rows);
 
 /* Report of the last solve on this process.  stop_iter[M] gets the
  * iteration at which each shift stopped (0 = never); switch_k / switch_to
@@ -124,6 +133,10 @@ typedef struct hbicg_kernel_stat {
     double replay_ms;    /*   (same inputs and outputs, no event gaps between them)  */
 } hbicg_kernel_stat;
 int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax);
+
+/* The plan's residual trace (plan created with params.trace = 1), laid out
+ * as for hbicg_last_residual_trace: min(rows, k) rows of M doubles. */
+int hbicg_plan_residual_trace(hbicg_plan *pl, double *out, int rows);
 
 /* Stop iterations / switch log of the plan's last solve (host arrays). */
 int hbicg_plan_report(hbicg_plan *pl, hbicg_report *rep, int *stop_iter,

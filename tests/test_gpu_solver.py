@@ -149,3 +149,33 @@ def test_against_golden_fixture(c0, seed):
         assert abs(kg - kr) <= 2 and sg == sr
     rel = np.abs(tr.x[:, ::41] - g[f"xs_{seed}"]).max() / np.abs(g[f"xs_{seed}"]).max()
     assert rel <= 1e-9
+
+
+@pytest.mark.parametrize("seed", [0, 8])
+def test_residual_trace(c0, oracle_c0, seed):
+    """The per-iteration residual estimates recorded on the device
+    (src/solver_dynamic.c:286-292) equal the oracle's history: bit for bit in
+    exact order, and in the default order within the drift of the tree
+    reductions while the estimates are far above the stopping threshold."""
+    ref = oracle_c0[seed]
+    ex = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic",
+                      make_params(tol=c0.tol, exact_order=True, trace=True))
+    assert ex.trace.shape == (ref.k, c0.sigma.shape[0])
+    assert np.array_equal(ex.trace, ref.res_hist[:ref.k])
+    # every shift's last recorded estimate is the one that met the rule
+    for j in range(c0.sigma.shape[0]):
+        kj = ex.stop_iter[j]
+        assert kj > 0 and ex.trace[kj, j] <= c0.tol * (1 + 1e-12) and np.all(ex.trace[kj + 1:, j] == -1.0)
+    tr = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", make_params(tol=c0.tol, trace=True))
+    m = min(tr.k, ref.k) - 3
+    a, b = tr.trace[1:m], ref.res_hist[1:m]
+    live = (a > 0) & (b > 1e-6)
+    # the tree reductions perturb alpha and beta by roundings; BiCG's
+    # irregular convergence and the 1/(zeta pi) factor of the shifts amplify
+    # that in the estimates (3.1e-5 relative at worst on this problem), while
+    # a wrong term would move them by O(1)
+    dev = np.abs(a[live] - b[live]) / b[live]
+    print("worst relative trace drift", dev.max(), flush=True)
+    assert dev.max() <= 1e-4
+    # a plan without trace=True has none
+    assert dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", make_params(tol=c0.tol)).trace is None
