@@ -100,7 +100,7 @@ struct hbicg_plan {
     double *val;
     unsigned *col, *ptr, *blk;
     short *col16;
-    unsigned char *cflag;
+    unsigned *cflag;
     double col_bytes;    /* column-index bytes per SpMV (16-bit tiles count 2 per entry) */
     int own_matrix, nblk;
     /* workspace */
@@ -346,7 +346,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     /* 16-bit column offsets for the tiles whose entries stay within 32767
      * columns of the tile's first row */
     pl->col16 = (short *)dalloc((size_t)pl->nz * 2, &rc);
-    pl->cflag = (unsigned char *)dalloc((size_t)pl->nblk, &rc);
+    pl->cflag = (unsigned *)dalloc((size_t)pl->nblk * 4, &rc);
     if (rc) goto fail;
     if (hb_k_compress_cols(d, pl->col16, pl->cflag, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
     HIPCK(hipStreamSynchronize(pl->stream));
@@ -354,14 +354,14 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->cflag = pl->cflag;
     {
         /* column-index bytes one SpMV reads, for the kernel statistics */
-        unsigned char *hf = (unsigned char *)malloc((size_t)pl->nblk);
+        unsigned *hf = (unsigned *)malloc((size_t)pl->nblk * 4);
         if (!hf) { rc = HBICG_E_NOMEM; goto fail; }
-        if (hipMemcpy(hf, pl->cflag, (size_t)pl->nblk, hipMemcpyDeviceToHost) != hipSuccess) {
+        if (hipMemcpy(hf, pl->cflag, (size_t)pl->nblk * 4, hipMemcpyDeviceToHost) != hipSuccess) {
             free(hf);
             rc = HBICG_E_HIP;
             goto fail;
         }
-        double cb = (double)pl->nblk;
+        double cb = 4.0 * (double)pl->nblk;
         for (int t = 0; t < pl->nblk; ++t)
             cb += (hf[t] ? 2.0 : 4.0) * (double)(hptr[hblk[t + 1]] - hptr[hblk[t]]);
         pl->col_bytes = cb;

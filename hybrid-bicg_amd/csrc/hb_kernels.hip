@@ -135,13 +135,13 @@ enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 template <int MODE, bool EXACT>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ col, const short *__restrict__ col16,
-                   const unsigned char *__restrict__ cflag, const unsigned *__restrict__ ptr,
+                   const unsigned *__restrict__ cflag, const unsigned *__restrict__ ptr,
                    const unsigned *__restrict__ blk, const double *__restrict__ xbase,
                    double *__restrict__ rring, const double *__restrict__ svec,
                    const double *__restrict__ rhat, double *__restrict__ out1,
                    double *__restrict__ qring, double *__restrict__ part, double shift)
 {
-    __shared__ double prod[HB_NNZ_CAP];
+    __shared__ double prod[HB_NNZ_CAP + 1];
     __shared__ unsigned rp[HB_ROWS_CAP + 1];
     const int n = d.n;
     int switched = 0;
@@ -175,61 +175,119 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         return xa[c];
     };
 
-    const unsigned r0 = blk[b], r1 = blk[b + 1];
+    const unsigned r0 = blk[b], r1 = blk[b + 1], narrow = __builtin_amdgcn_readfirstlane(cflag[b]);
     const int R = (int)(r1 - r0);
     const unsigned e0 = ptr[r0];
     const int N = (int)(ptr[r1] - e0);
     const int tid = threadIdx.x;
     double part0 = 0.0, part1 = 0.0;
 
-    auto epilogue = [&](unsigned i, double t) {
+    /* Row i's result t: the operands a, bb, c of the row's own entries
+     * (SEED1: x_i, r^_i and, after a switch, r_i; SEED2: r_i, s_i; PLAIN:
+     * x_i) are loaded before the tile's reduction, so the epilogue waits on
+     * no memory. */
+    auto epilogue = [&](unsigned i, double t, double a, double bb, double c) {
         if (MODE == SP_SEED1) {
-            const double si = t + sig * xa[i];
+            const double si = t + sig * a;
             out1[i] = si;
-            part0 += rhat[i] * si;
+            part0 += bb * si;
             if (switched) {
-                const double rr = rprev[i] * zscale;
+                const double rr = c * zscale;
                 rprev[i] = rr;
-                part1 += rhat[i] * rr;
+                part1 += bb * rr;
             }
         } else if (MODE == SP_SEED2) {
-            const double qi = rprev[i] + nalpha * svec[i];
+            const double qi = a + nalpha * bb;
             qout[i] = qi;
             const double yi = t + sig * qi;
             out1[i] = yi;
             part0 += qi * qi;
             part1 += qi * yi;
         } else {
-            out1[i] = t + sig * xa[i];
+            out1[i] = t + sig * a;
+        }
+    };
+    auto operands = [&](unsigned i, double &a, double &bb, double &c) {
+        if (MODE == SP_SEED1) {
+            a = xa[i];
+            bb = rhat[i];
+            c = switched ? rprev[i] : 0.0;
+        } else if (MODE == SP_SEED2) {
+            a = rprev[i];
+            bb = svec[i];
+        } else {
+            a = xa[i];
         }
     };
 
     if (N <= HB_NNZ_CAP) {
-        for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
-        if (cflag[b]) {
-            /* 16-bit column offsets from the tile's first row: 2 bytes per
-             * entry instead of 4 */
-#pragma unroll 4
-            for (int e = tid; e < N; e += HB_BLOCK)
-                prod[e] = val[e0 + e] * xval(r0 + (unsigned)(int)col16[e0 + e]);
-        } else {
-#pragma unroll 4
-            for (int e = tid; e < N; e += HB_BLOCK) prod[e] = val[e0 + e] * xval(col[e0 + e]);
-        }
-        __syncthreads();
+        /* rows reduced by T lanes each, T the largest power of two with
+         * T*R <= HB_BLOCK (at most 64; 1 in exact order): every row has one
+         * group of lanes, and lane 0 of the group owns its epilogue */
         int T = 1;
         if (!EXACT) {
             const int q = HB_BLOCK / R;
             while ((T << 1) <= q && (T << 1) <= 64) T <<= 1;
         }
-        const int per_pass = HB_BLOCK / T;
-        const int lane = tid & (T - 1);
-        for (int row = tid / T; row < R; row += per_pass) {
+        const int row = tid / T, lane = tid & (T - 1);
+        const unsigned i_me = r0 + (unsigned)min(row, R - 1);
+        double oa = 0.0, ob = 0.0, oc = 0.0;
+        operands(i_me, oa, ob, oc);
+
+        constexpr int U = HB_NNZ_CAP / HB_BLOCK;
+        static_assert(U * HB_BLOCK == HB_NNZ_CAP, "tile capacity");
+        if (MODE == SP_SEED2) {
+            /* two gathers per entry (r and s): entries in rounds of four,
+             * which keeps fewer gathers in flight and measured faster than
+             * all eight at once */
+            if (narrow) {
+#pragma unroll 4
+                for (int e = tid; e < N; e += HB_BLOCK)
+                    prod[e] = val[e0 + e] * xval(r0 + (unsigned)(int)col16[e0 + e]);
+            } else {
+#pragma unroll 4
+                for (int e = tid; e < N; e += HB_BLOCK) prod[e] = val[e0 + e] * xval(col[e0 + e]);
+            }
+        } else if (N > 0) {
+            /* A thread owns entries tid + u*HB_BLOCK, u < U.  All of its
+             * column and value loads are issued together (indices clamped
+             * into the tile, so no load is predicated), then all of its
+             * gathers: two memory latencies per tile, not two per entry. */
+            double v[U];
+            unsigned c[U];
+            if (narrow) {
+                /* 16-bit column offsets from the tile's first row: 2 bytes
+                 * per entry instead of 4 */
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned e = e0 + (unsigned)min(tid + u * HB_BLOCK, N - 1);
+                    c[u] = r0 + (unsigned)(int)col16[e];
+                    v[u] = val[e];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned e = e0 + (unsigned)min(tid + u * HB_BLOCK, N - 1);
+                    c[u] = col[e];
+                    v[u] = val[e];
+                }
+            }
+            /* entries past the tile end go to the spare slot prod[HB_NNZ_CAP]
+             * (no branch, so the gathers are not serialised behind one) */
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = tid + u * HB_BLOCK;
+                prod[e < N ? e : HB_NNZ_CAP] = v[u] * xval(c[u]);
+            }
+        }
+        for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
+        __syncthreads();
+        if (row < R) {
             double acc = 0.0;
             const int hi = (int)rp[row + 1];
             for (int e = (int)rp[row] + lane; e < hi; e += T) acc += prod[e];
             for (int o = T >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (lane == 0) epilogue(r0 + row, acc);
+            if (lane == 0) epilogue(i_me, acc, oa, ob, oc);
         }
     } else {
         /* one row longer than the LDS stage */
@@ -242,7 +300,11 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             double dummy = 0.0;
             block_sum2(acc, dummy);
         }
-        if (tid == 0) epilogue(r0, acc);
+        if (tid == 0) {
+            double oa = 0.0, ob = 0.0, oc = 0.0;
+            operands(r0, oa, ob, oc);
+            epilogue(r0, acc, oa, ob, oc);
+        }
     }
 
     if (MODE == SP_PLAIN) return;
@@ -284,7 +346,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_compress_cols(const unsigned *__re
                                                             const unsigned *__restrict__ col,
                                                             const unsigned *__restrict__ blk,
                                                             short *__restrict__ col16,
-                                                            unsigned char *__restrict__ cflag)
+                                                            unsigned *__restrict__ cflag)
 {
     const unsigned b = blockIdx.x;
     const unsigned r0 = blk[b], e0 = ptr[r0], e1 = ptr[blk[b + 1]];
@@ -295,7 +357,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_compress_cols(const unsigned *__re
         else col16[e] = (short)o;
     }
     ok = __syncthreads_and(ok);
-    if (threadIdx.x == 0) cflag[b] = (unsigned char)ok;
+    if (threadIdx.x == 0) cflag[b] = (unsigned)ok;
 }
 
 template <bool EXACT>
@@ -1426,7 +1488,7 @@ int hb_k_fill(double *p, size_t count, double v, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-int hb_k_compress_cols(const hb_dev *d, short *col16, unsigned char *cflag, hipStream_t s)
+int hb_k_compress_cols(const hb_dev *d, short *col16, unsigned *cflag, hipStream_t s)
 {
     hipLaunchKernelGGL(k_compress_cols, dim3(d->nblk), dim3(HB_BLOCK), 0, s, d->ptr, d->col, d->blk,
                        col16, cflag);

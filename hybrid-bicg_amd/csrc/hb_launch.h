@@ -51,7 +51,7 @@ int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *o
 int hb_k_fill(double *p, size_t count, double v, hipStream_t s);
 
 /* col16 / cflag of the plan's SpMV tiles (see hb_dev); once per plan. */
-int hb_k_compress_cols(const hb_dev *d, short *col16, unsigned char *cflag, hipStream_t s);
+int hb_k_compress_cols(const hb_dev *d, short *col16, unsigned *cflag, hipStream_t s);
 
 /* y = A x + shift*x through the row-binned SpMV (exact: row-sequential sums). */
 int hb_k_spmv(const hb_dev *d, const double *x, double *y, double shift, hipStream_t s);

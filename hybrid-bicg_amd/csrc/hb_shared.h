@@ -83,7 +83,7 @@ typedef struct hb_dev {
     HB_G const unsigned *ptr;
     HB_G const unsigned *blk;  /* SpMV workgroup row starts [nblk+1]       */
     HB_G const short    *col16;/* [nz] col - (first row of the entry's tile), where it fits */
-    HB_G const unsigned char *cflag; /* [nblk] 1: the tile's col16 entries are exact */
+    HB_G const unsigned *cflag;      /* [nblk] 1: the tile's col16 entries are exact */
     HB_G const double   *sigma;/* [M]                                      */
     HB_G const double   *rhat; /* [n]   shadow residual r^ = b             */
     HB_G double *rring;        /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */
