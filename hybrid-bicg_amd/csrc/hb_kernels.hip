@@ -142,6 +142,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    double *__restrict__ qring, double *__restrict__ part, double shift)
 {
     __shared__ double prod[HB_NNZ_CAP + 1];
+    __shared__ short scol[HB_NNZ_CAP + 1];
     __shared__ unsigned rp[HB_ROWS_CAP + 1];
     const int n = d.n;
     int switched = 0;
@@ -236,10 +237,48 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
 
         constexpr int U = HB_NNZ_CAP / HB_BLOCK;
         static_assert(U * HB_BLOCK == HB_NNZ_CAP, "tile capacity");
+        if (MODE == SP_SEED2 && T == 1 && narrow && N > 0) {
+            /* One row per thread.  Values and 16-bit column offsets are
+             * staged in LDS, then each thread walks its own row: the lanes of
+             * a wave gather the same stencil slot of consecutive rows, i.e.
+             * consecutive addresses, where the entry-per-lane order touches
+             * several cache lines per instruction.  Two gathers per entry
+             * (r and s), four entries in flight.  The row sum runs in CSR
+             * order with the products rounded on their own, as in the LDS
+             * products path below. */
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = tid + u * HB_BLOCK;
+                const unsigned ec = e0 + (unsigned)min(e, N - 1);
+                const double vv = val[ec];
+                const short cc = col16[ec];
+                prod[e < N ? e : HB_NNZ_CAP] = vv;
+                scol[e < N ? e : HB_NNZ_CAP] = cc;
+            }
+            for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
+            __syncthreads();
+            if (row < R) {
+                constexpr int CH = 4;
+                const int lo = (int)rp[row], hi = (int)rp[row + 1];
+                double acc = 0.0;
+                for (int base = lo; base < hi; base += CH) {
+                    double vv[CH], xv[CH];
+#pragma unroll
+                    for (int j = 0; j < CH; ++j) {
+                        const int e = min(base + j, hi - 1);
+                        vv[j] = prod[e];
+                        xv[j] = xval(r0 + (unsigned)(int)scol[e]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < CH; ++j)
+                        if (base + j < hi) acc += vv[j] * xv[j];
+                }
+                epilogue(i_me, acc, oa, ob, oc);
+            }
+            goto tile_done;
+        }
         if (MODE == SP_SEED2) {
-            /* two gathers per entry (r and s): entries in rounds of four,
-             * which keeps fewer gathers in flight and measured faster than
-             * all eight at once */
+            /* two gathers per entry (r and s): entries in rounds of four */
             if (narrow) {
 #pragma unroll 4
                 for (int e = tid; e < N; e += HB_BLOCK)
@@ -307,6 +346,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         }
     }
 
+tile_done:
     if (MODE == SP_PLAIN) return;
 
     block_sum2(part0, part1);
