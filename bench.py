@@ -232,6 +232,17 @@ def main():
     rep, stop_iter, switches = plan.report()
     true_res = plan.true_residuals(b, prob.sigma, x)
     iters = ks[-1] - 1
+    # shift updates the per-iteration formulation performs (one per active
+    # non-seed shift and iteration)
+    updates = sum((int(stop_iter[j]) if stop_iter[j] > 0 else iters) for j in range(M) if j != prob.seed)
+    # HBM bytes of the reference's formulation of the same solve: per
+    # iteration two SpMVs of 12 B per nonzero, 4 B per row pointer and an
+    # n-vector in and out each (src/solver_dynamic.c:156-190), the shared
+    # q, r_old, r reads and the seed p update (48 B per row), and 32 B per row
+    # for every active shift's x_j, p_j read and write (:248-255)
+    nnz = prob.A.nnz
+    ref_bytes = iters * (2 * (12.0 * nnz + 4.0 * (n + 1) + 16.0 * n) + 48.0 * n) + 32.0 * n * updates
+    ref_equiv = ref_bytes / (ms_per_step * 1e-3) / 1e9
 
     # profiled solve of the same workload: HIP events around every kernel
     kernels, roofline = {}, None
@@ -278,11 +289,6 @@ def main():
         seed_per_iter = ores.t_seed / done
         shift_per_update = ores.t_shift / (done * (M - 1))
         check_per_iter = ores.t_check / done
-        updates = 0
-        for j in range(M):
-            if j == prob.seed:
-                continue
-            updates += int(stop_iter[j]) if stop_iter[j] > 0 else iters
         t_full = (seed_per_iter + check_per_iter) * iters + shift_per_update * updates
         cpu = {"value": round(M / t_full, 4), "unit": "shift-solves/s", "cores": O.num_threads(),
                "kind": "port",
@@ -319,6 +325,12 @@ def main():
         "true_residual_max": float(true_res.max()),
         "all_converged": bool(rep["stop_count"] == M),
         "spmv_plus_shift_hbm_frac": None if spmv_shift_frac is None else round(spmv_shift_frac, 4),
+        "reference_formulation": {
+            "bytes_per_solve": ref_bytes, "shift_updates": updates,
+            "equiv_GBps": round(ref_equiv, 1), "equiv_frac_of_peak": round(ref_equiv / HBM_PEAK_GBPS, 3),
+            "note": "HBM bytes the per-iteration formulation (two SpMVs, seed vectors, x_j and p_j "
+                    "of every active shift read and written each iteration) must move for this "
+                    "solve, divided by the whole solve time measured here"},
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -3,7 +3,7 @@
  * switching, and the C-callable launch layer declared in hb_launch.h.
  *
  * One iteration of the reference loop (src/solver_dynamic.c:147-379) is
- * seven launches that keep every scalar on the device:
+ * eight launches that keep every scalar on the device:
  *   spmv<SEED1> + fin   s = (A + sigma_s I) p_s; (r^,s) -> alpha~[k]   (:156-167)
  *   spmv<SEED2> + fin   q = r - alpha~ s fused into y = (A + sigma_s I) q;
  *                       (q,q), (q,y) -> omega~[k]                      (:171-190)
@@ -11,13 +11,18 @@
  *                       then the scalar stage: per-shift recurrences,
  *                       convergence check, seed switch with replay     (:194-204,
  *                                                                       :239-359)
- *   shift               seed p update, and every W iterations the
+ *   flush_coef          at a window's end, each shift's combined
+ *                       coefficients over the window (double-double)
+ *   shift               seed p update, and at a window's end the
  *                       deferred x_j / p_j updates of all shifts       (:208-210,
  *                                                                       :248-255)
  * Per-shift vector updates are deferred over a window of up to W iterations
- * and applied in one pass that reads and writes x_j and p_j once, replaying
- * the reference's six per-iteration vector operations in their original
- * order, so x_j and p_j are bit-identical to per-iteration updates.
+ * (the window's r and q stay in ring buffers) and applied in one pass that
+ * reads and writes x_j and p_j once.  Default mode: x_j and p_j are formed
+ * from combined coefficients with fused multiply-adds (they move with the
+ * window by roundings only).  Exact order: the reference's six vector
+ * operations of every iteration are replayed in their original order with
+ * separate roundings, bit-identical to per-iteration updates.
  *
  * Every kernel body is a device function of (problem, tile).  The plain
  * solver launches one problem's tiles; the batched solver launches the
