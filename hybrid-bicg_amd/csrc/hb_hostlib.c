@@ -140,6 +140,13 @@ static int load_coo(const char *filename, COO_Matrix *coo, MM_typecode code)
     return 0;
 }
 
+/* The same reader for the device loader (hbicg_csr_load_device); code gets
+ * the banner's MM_typecode. */
+int hb_mm_read_coo(const char *filename, COO_Matrix *coo, char *code)
+{
+    return load_coo(filename, coo, code);
+}
+
 /* Replaces src/matrix.c:26-94. */
 int coo_load_matrix(char *filename, COO_Matrix *coo)
 {

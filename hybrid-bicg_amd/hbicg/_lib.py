@@ -86,7 +86,8 @@ EXPORTS = {
                 "hbicg_batch_create",
                 "hbicg_batch_destroy", "hbicg_batch_solve", "hbicg_batch_report",
                 "hbicg_batch_true_residuals", "hbicg_batch_autotune", "hbicg_csr_transpose",
-                "hbicg_adjoint_gradients"],
+                "hbicg_adjoint_gradients", "hbicg_coo2csr_device", "hbicg_csr_load_device",
+                "hbicg_csr_free_device"],
 }
 
 _lib = None
@@ -166,6 +167,13 @@ def load():
     L.hbicg_adjoint_gradients.argtypes = [C.POINTER(Csr), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.hbicg_adjoint_gradients.restype = C.c_int
+    L.hbicg_coo2csr_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
+                                       C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.hbicg_coo2csr_device.restype = C.c_int
+    L.hbicg_csr_load_device.argtypes = [C.c_char_p, C.POINTER(Csr), C.POINTER(INFO_Matrix), C.c_void_p]
+    L.hbicg_csr_load_device.restype = C.c_int
+    L.hbicg_csr_free_device.argtypes = [C.POINTER(Csr)]
+    L.hbicg_csr_free_device.restype = None
     _lib = L
     return L
 

@@ -189,6 +189,30 @@ int hbicg_adjoint_gradients(const hbicg_csr *A, int M, const double *mu, const d
                             const double *x, double *grad_b, double *grad_sigma, double *grad_val,
                             void *stream);
 
+/* ---- Matrix Market ingest on the GPU ------------------------------------------ */
+/* COO -> CSR on the device: the arrays the reference's coo2csr builds
+ * (Replaces src/matrix.c:206-232 with its stable merge sort by row,
+ * src/matrix.c:125-187): rows ascending, the entries of a row in input
+ * order, duplicates kept.  row, col, val: nz entries, 0-based, device
+ * memory (may be NULL when nz = 0); ptr[rows+1], ocol[nz], oval[nz]: device
+ * outputs.  Every row index
+ * must be < rows and every column < cols (HBICG_E_ARG otherwise, outputs
+ * untouched).  stream: hipStream_t or NULL; synchronises it.  Returns 0 or a
+ * negative code. */
+int hbicg_coo2csr_device(const unsigned *row, const unsigned *col, const double *val, unsigned nz,
+                         unsigned rows, unsigned cols, unsigned *ptr, unsigned *ocol, double *oval,
+                         void *stream);
+
+/* Reads a Matrix Market coordinate file on the host (real, integer or
+ * pattern; symmetric storage is not expanded, as in src/matrix.c:26-94) and
+ * builds its CSR on the device with hbicg_coo2csr_device.  A receives device
+ * arrays owned by the caller (release with hbicg_csr_free_device); info
+ * (optional) gets nz, rows, cols, the banner code and the single-rank
+ * recvcounts/displs, like src/matrix.c:402-419 on one rank.  A non-square
+ * matrix returns HBICG_E_NOTSQUARE. */
+int  hbicg_csr_load_device(const char *filename, hbicg_csr *A, INFO_Matrix *info, void *stream);
+void hbicg_csr_free_device(hbicg_csr *A);
+
 /* Times candidate layouts on the batch for `iters` iterations each and keeps
  * the fastest (x is overwritten): windows 16/32/64 in the default mode,
  * (window 4/8/16) x (shifts per workgroup all/16/4) in exact order. */

@@ -366,3 +366,46 @@ void orc_true_residuals(const orc_csr *A, const double *x_set, const double *b,
     }
     free(t);
 }
+
+/* ---- COO -> CSR (src/matrix.c:125-187, 206-232) ------------------------------
+ * Bottom-up merge sort of the entry indices by row.  The merge takes the left
+ * run's entry when the rows are equal (src/matrix.c:139, A[i] <= A[j]), so the
+ * sort is stable, and every stable sort yields this one order: rows ascending,
+ * input order inside a row.  The reference sorts row/col/val together; sorting
+ * the indices and gathering once yields the same arrays. */
+int orc_coo2csr(unsigned rows, unsigned nz, const unsigned *row, const unsigned *col,
+                const double *val, unsigned *ptr, unsigned *ocol, double *oval)
+{
+    for (unsigned e = 0; e < nz; ++e)
+        if (row[e] >= rows) return -1;
+    unsigned *a = malloc(((size_t)nz + 1) * sizeof(unsigned));
+    unsigned *t = malloc(((size_t)nz + 1) * sizeof(unsigned));
+    if (!a || !t) { free(a); free(t); return -1; }
+    for (unsigned e = 0; e < nz; ++e) a[e] = e;
+    for (size_t w = 1; w < nz; w *= 2) {
+        for (size_t lo = 0; lo < nz; lo += 2 * w) {
+            const size_t mid = lo + w < nz ? lo + w : nz;
+            const size_t hi = lo + 2 * w < nz ? lo + 2 * w : nz;
+            size_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) t[k++] = (row[a[i]] <= row[a[j]]) ? a[i++] : a[j++];
+            while (i < mid) t[k++] = a[i++];
+            while (j < hi) t[k++] = a[j++];
+        }
+        unsigned *s = a; a = t; t = s;
+    }
+    /* ptr as src/matrix.c:221-229 fills it: row i ends where the sorted rows
+     * pass i */
+    size_t e = 0;
+    ptr[0] = 0;
+    for (unsigned i = 0; i < rows; ++i) {
+        while (e < nz && row[a[e]] == i) {
+            ocol[e] = col[a[e]];
+            oval[e] = val[a[e]];
+            ++e;
+        }
+        ptr[i + 1] = (unsigned)e;
+    }
+    free(a);
+    free(t);
+    return 0;
+}

@@ -80,6 +80,16 @@ double orc_ddot_v3(int n, const double *x, const double *y, double *tmp);
 /* Threads the OpenMP build uses (omp_get_max_threads), 1 without OpenMP. */
 int orc_num_threads(void);
 
+/* COO -> CSR as src/matrix.c:206-232 does it: the entries are ordered by row
+ * with a stable merge sort (src/matrix.c:125-187: on equal rows the left run
+ * goes first, so entries of one row keep their input order), then ptr counts
+ * them.  row, col, val: nz 0-based entries; ptr[rows+1], ocol[nz], oval[nz]
+ * are written.  Returns 0, or -1 when an allocation fails or a row index is
+ * >= rows (the reference does not check; its loop would drop such entries
+ * from ptr). */
+int orc_coo2csr(unsigned rows, unsigned nz, const unsigned *row, const unsigned *col,
+                const double *val, unsigned *ptr, unsigned *ocol, double *oval);
+
 #ifdef __cplusplus
 }
 #endif

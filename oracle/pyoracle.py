@@ -59,6 +59,8 @@ def lib():
         L.orc_ddot_v3.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_ddot_v3.restype = C.c_double
         L.orc_num_threads.restype = C.c_int
+        L.orc_coo2csr.restype = C.c_int
+        L.orc_coo2csr.argtypes = [C.c_uint, C.c_uint, _up, _up, _dp, _up, _up, _dp]
         _lib = L
     return _lib
 
@@ -149,3 +151,19 @@ def true_residuals(A, x, b, sigma) -> np.ndarray:
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+def coo2csr(rows, row, col, val):
+    """(ptr, col, val) of the reference's coo2csr (stable merge sort by row)."""
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    nz = row.shape[0]
+    ptr = np.zeros(rows + 1, dtype=np.uint32)
+    ocol = np.zeros(max(nz, 1), dtype=np.uint32)
+    oval = np.zeros(max(nz, 1))
+    rc = lib().orc_coo2csr(rows, nz, _p(row, _up), _p(col, _up), _p(val), _p(ptr, _up),
+                           _p(ocol, _up), _p(oval))
+    if rc != 0:
+        raise ValueError("oracle coo2csr rejected its arguments")
+    return ptr, ocol[:nz], oval[:nz]

@@ -244,3 +244,48 @@ def test_thread_count_independent(c0):
         subprocess.check_call([sys.executable, "-c", code, f], cwd=root, env=env)
         outs.append(np.load(f))
     assert np.array_equal(outs[0], outs[1])
+
+
+# ---- COO -> CSR (src/matrix.c:125-187, 206-232) ------------------------------
+def _coo_bruteforce(rows, row, col, val):
+    """Row by row, each row's entries in input order: what a stable sort by
+    row gives, spelled out with no sort at all."""
+    order = [e for r in range(rows) for e in range(len(row)) if row[e] == r]
+    ptr = np.zeros(rows + 1, dtype=np.uint32)
+    for r in row:
+        ptr[r + 1] += 1
+    return np.cumsum(ptr).astype(np.uint32), np.asarray(col)[order], np.asarray(val)[order]
+
+
+@pytest.mark.parametrize("rows,nz,seed", [(1, 5, 0), (7, 60, 1), (33, 500, 2), (5, 0, 3)])
+def test_coo2csr_bruteforce(rows, nz, seed):
+    """Duplicated (row, col) pairs and many entries per row: a sort that is
+    not stable, or a merge that took the right run on ties, reorders them."""
+    rng = np.random.default_rng(seed)
+    row = rng.integers(0, rows, nz).astype(np.uint32)
+    col = rng.integers(0, 4, nz).astype(np.uint32)
+    val = rng.standard_normal(nz)
+    ptr, oc, ov = O.coo2csr(rows, row, col, val)
+    eptr, ec, ev = _coo_bruteforce(rows, row, col, val)
+    assert np.array_equal(ptr, eptr)
+    assert np.array_equal(oc, ec) and np.array_equal(ov, ev)
+
+
+def test_coo2csr_numpy_stable_argsort():
+    """Larger case against numpy's stable sort (a textbook stable merge
+    sort), with empty rows."""
+    rng = np.random.default_rng(7)
+    rows, nz = 5000, 300_000
+    row = (rng.integers(0, rows // 2, nz) * 2).astype(np.uint32)   # odd rows empty
+    col = rng.integers(0, rows, nz).astype(np.uint32)
+    val = rng.standard_normal(nz)
+    ptr, oc, ov = O.coo2csr(rows, row, col, val)
+    perm = np.argsort(row, kind="stable")
+    assert np.array_equal(oc, col[perm]) and np.array_equal(ov, val[perm])
+    assert np.array_equal(ptr, np.concatenate([[0], np.cumsum(np.bincount(row, minlength=rows))]))
+    assert np.all(ptr[2::2] == ptr[1:-1:2])
+
+
+def test_coo2csr_rejects_row_out_of_range():
+    with pytest.raises(ValueError):
+        O.coo2csr(3, [0, 3], [0, 0], [1.0, 2.0])
