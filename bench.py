@@ -16,7 +16,16 @@ Printed: one JSON line with the contract fields plus
                 FETCH_SIZE / WRITE_SIZE summary of this workload, when there;
   cpu_baseline  the oracle (CPU restatement of the reference) timed on a
                 bounded sample of the same solve on this host, extrapolated
-                to the full solve from its seed-step and per-shift timers.
+                to the full solve from its seed-step and per-shift timers;
+  batched, adjoint, ingest
+                the 256-problem batch against per-problem plain solves, the
+                forward + adjoint solve of the 64^3 / 32-shift loss, and GPU
+                COO -> CSR of the 128^3 operator's shuffled entries.
+
+--mode sweep   shift counts 1, 2, .. 1024 x {normal, dynamic} on a 64^3
+               operator with the reference sweep driver's conventions
+               (src/main_seed_diff.c:113-155); --mode repeat: 1024 shifts on
+               the 128^3 operator timed 5 times (src/main_repeat.c:130-177).
 """
 import argparse
 import json
@@ -65,6 +74,16 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration under rocprofv3)")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the GPU COO->CSR ingest leg")
+    ap.add_argument("--mode", default="headline", choices=["headline", "sweep", "repeat"],
+                    help="headline: the 128^3 / 64-shift solve and its legs; sweep: shift counts "
+                         "1..--sweep-max x {normal, dynamic} (src/main_seed_diff.c); repeat: one "
+                         "solve of --repeat-shifts shifts timed --repeats times (src/main_repeat.c)")
+    ap.add_argument("--sweep-grid", type=int, default=64, help="3-D grid side of the sweep's operator")
+    ap.add_argument("--sweep-max", type=int, default=1024)
+    ap.add_argument("--repeat-shifts", type=int, default=1024)
+    ap.add_argument("--repeats", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-10, help="stopping rule of the sweep and repeat modes")
     return ap.parse_args()
 
 
@@ -117,6 +136,159 @@ def adjoint_leg(args, log):
             "forward_ms": round(f * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
             "fwd_plus_adjoint_over_fwd": round((f + bwd) / f, 3),
             "grad_norms": {"b": float(gb.norm()), "sigma": float(gs.norm()), "A": float(gv.norm())}}
+
+
+def ingest_leg(args, log):
+    """Matrix Market ingest: the 128^3 operator's entries as a COO in a seeded
+    random order (inputs resident in HBM), converted to CSR on the GPU by
+    hbicg_coo2csr_device (stable radix sort by row, the order of the
+    reference's coo2csr).  cpu_baseline: the oracle's restatement of the
+    reference's merge-sort coo2csr on the same entries."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from hbicg import _lib
+    from hbicg import problems as P
+
+    A = P.conv_diff_3d(args.grid, args.grid, args.grid, diag=6.0, conv=0.2)
+    n, nz = A.n, A.nnz
+    ptr0 = np.asarray(A.ptr, dtype=np.uint32)
+    row = np.repeat(np.arange(n, dtype=np.uint32), np.diff(ptr0))
+    perm = np.random.default_rng(6).permutation(nz)
+    row, col, val = row[perm], np.asarray(A.col, np.uint32)[perm], np.asarray(A.val)[perm]
+    dev = lambda a: torch.as_tensor(a.view(np.int32) if a.dtype == np.uint32 else a).cuda()
+    drow, dcol, dval = dev(row), dev(col), dev(val)
+    optr = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+    ocol = torch.empty(nz, dtype=torch.int32, device="cuda")
+    oval = torch.empty(nz, dtype=torch.float64, device="cuda")
+    L = _lib.load()
+    torch.cuda.synchronize()
+    call = lambda: _lib.check(L.hbicg_coo2csr_device(drow.data_ptr(), dcol.data_ptr(), dval.data_ptr(), nz, n, n,
+                                                     optr.data_ptr(), ocol.data_ptr(), oval.data_ptr(), None),
+                              "hbicg_coo2csr_device")
+    call()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        call()
+        times.append(time.perf_counter() - t0)
+    t = min(times)
+    ok = bool(np.array_equal(optr.cpu().numpy().view(np.uint32), ptr0))
+    alg = 28.0 * nz + 4.0 * (n + 1)     # row, col, val in; col, val, ptr out
+    out = {"workload": f"COO -> CSR of the 3-D 7-point {args.grid}^3 operator (n={n}, nnz={nz}), "
+                       f"entries in seeded random order, on the device",
+           "ms": round(t * 1e3, 3), "entries_per_s": round(nz / t, 1),
+           "algorithmic_GBps": round(alg / t / 1e9, 1), "hbm_frac": round(alg / t / 1e9 / HBM_PEAK_GBPS, 4),
+           "radix_passes": (max(n - 1, 1).bit_length() + 7) // 8, "ptr_matches": ok,
+           "note": "wall time of the call (scratch allocation, range check, radix passes, gather, "
+                   "row pointers, stream synchronisation); best of 5"}
+    del drow, dcol, dval, optr, ocol, oval
+    if args.cpu_iters > 0:
+        import pyoracle as O
+        t0 = time.perf_counter()
+        O.coo2csr(n, row, col, val)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(nz / tc, 1), "unit": "entries/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle coo2csr (the reference's stable merge sort by row, "
+                                         f"restated) on the same {nz} entries: {tc:.2f} s"}
+    log(f"ingest: {t * 1e3:.2f} ms for {nz} entries, ptr matches {ok}")
+    return out
+
+
+def _driver_problem(n_side, M):
+    """The reference drivers' conventions (src/main_seed_diff.c:113-135,
+    src/main_repeat.c:113-120): b = 1, sigma_i = (i+1) * 0.01 / M, x0 = 0."""
+    import numpy as np
+    sigma = (np.arange(M) + 1) * (0.01 / M)
+    return np.ones(n_side ** 3), sigma
+
+
+def _timed_solves(A, b_host, sigma, seed, variant, tol, reps, log):
+    import numpy as np
+    import torch
+
+    from hbicg import Plan, make_params
+    M = len(sigma)
+    graph = 1 if variant == "dynamic" else 0
+    window = 1 if variant == "normal" else 64
+    plan = Plan(A, M, make_params(tol=tol, max_iter=2000, window=window, graph=graph))
+    b = torch.as_tensor(b_host).cuda()
+    x = torch.zeros((M, A.n), dtype=torch.float64, device="cuda")
+    plan.solve(b, sigma, seed, x)
+    times = []
+    for _ in range(reps):
+        x.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k, _, rep = plan.solve(b, sigma, seed, x)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    res = float(plan.true_residuals(b, sigma, x).max())
+    plan.close()
+    del x, b
+    return k, rep, times, res
+
+
+def sweep_mode(args):
+    """Shift counts M = 1, 2, 4, .. --sweep-max, each with the normal and
+    dynamic variants (src/main_seed_diff.c:113-155: seed M/2 - 1), on the 3-D
+    7-point operator of side --sweep-grid."""
+    import numpy as np
+
+    from hbicg import problems as P
+    g = args.sweep_grid
+    A = P.conv_diff_3d(g, g, g, diag=6.0, conv=0.2)
+    rows = []
+    M = 1
+    while M <= args.sweep_max:
+        b, sigma = _driver_problem(g, M)
+        seed = max(M // 2 - 1, 0)
+        for variant in ("normal", "dynamic"):
+            k, rep, times, res = _timed_solves(A, b, sigma, seed, variant, args.tol, args.steps, log)
+            t = float(np.median(times))
+            row = {"M": M, "variant": variant, "seed": seed, "iterations": k - 1, "ms": round(t * 1e3, 3),
+                   "shift_solves_per_s": round(M / t, 2), "seed_switches": rep["n_switch"],
+                   "all_converged": bool(rep["stop_count"] == M), "true_residual_max": res}
+            log(json.dumps(row))
+            rows.append(row)
+        M *= 2
+    top = [r for r in rows if r["variant"] == "dynamic"][-1]
+    print(json.dumps({
+        "metric": METRIC, "value": top["shift_solves_per_s"], "unit": "shift-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": top["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"shift-count sweep on conv-diff 3-D 7-point {g}^3 (n={A.n}), b = 1, "
+                               f"sigma_i = (i+1)*0.01/M, seed M/2-1, relres {args.tol:g}; value = dynamic at "
+                               f"M={top['M']} (median of {args.steps})", "mode": "sweep"},
+        "sweep": rows}))
+
+
+def repeat_mode(args):
+    """One family of --repeat-shifts shifts (seed M/2 - 1, as the reference's
+    SEED 511 of 1024) on the --grid operator, the dynamic variant timed
+    --repeats times in a row (src/main_repeat.c:130-177)."""
+    import numpy as np
+
+    from hbicg import problems as P
+    g, M = args.grid, args.repeat_shifts
+    A = P.conv_diff_3d(g, g, g, diag=6.0, conv=0.2)
+    b, sigma = _driver_problem(g, M)
+    seed = max(M // 2 - 1, 0)
+    k, rep, times, res = _timed_solves(A, b, sigma, seed, "dynamic", args.tol, args.repeats, log)
+    ms = [round(t * 1e3, 3) for t in times]
+    log(f"repeat: {ms} ms")
+    mean = float(np.mean(times))
+    print(json.dumps({
+        "metric": METRIC, "value": round(M / mean, 3), "unit": "shift-solves/s", "n_gpus": 1,
+        "steps": args.repeats, "warmup": 1, "ms_per_step": round(mean * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"repeat timing: conv-diff 3-D 7-point {g}^3 (n={A.n}), M={M} shifts "
+                               f"sigma_i = (i+1)*0.01/M, seed {seed}, b = 1, relres {args.tol:g}, dynamic",
+                   "mode": "repeat"},
+        "repeat_ms": ms, "iterations": k - 1, "seed_switches": rep["n_switch"],
+        "all_converged": bool(rep["stop_count"] == M), "true_residual_max": res}))
 
 
 def batched_leg(args, log):
@@ -182,6 +354,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding of the "
                                    "shift set is designed but not built", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.mode == "sweep":
+        return sweep_mode(args)
+    if args.mode == "repeat":
+        return repeat_mode(args)
     import numpy as np
     import torch
 
@@ -303,6 +479,9 @@ def main():
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(args, log)
+    ingest = None
+    if not args.no_ingest:
+        ingest = ingest_leg(args, log)
 
     line = {
         "metric": METRIC,
@@ -336,6 +515,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "ingest": ingest,
         "gen_s": round(t_gen, 2),
     }
     print(json.dumps(line))
