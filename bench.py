@@ -212,7 +212,7 @@ def _timed_solves(A, b_host, sigma, seed, variant, tol, reps, log):
     from hbicg import Plan, make_params
     M = len(sigma)
     graph = 1 if variant == "dynamic" else 0
-    window = 1 if variant == "normal" else 64
+    window = 1 if variant == "normal" else 0
     plan = Plan(A, M, make_params(tol=tol, max_iter=2000, window=window, graph=graph))
     b = torch.as_tensor(b_host).cuda()
     x = torch.zeros((M, A.n), dtype=torch.float64, device="cuda")
@@ -380,7 +380,7 @@ def main():
     dev = torch.device("cuda:0")
     M, n = prob.M, prob.n
     graph = 1 if args.variant == "dynamic" else 0
-    window = args.window if args.window > 0 else (1 if args.variant == "normal" else 64)
+    window = args.window if args.window > 0 else (1 if args.variant == "normal" else 0)
     params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph)
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)
@@ -406,6 +406,7 @@ def main():
 
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step")
     rep, stop_iter, switches = plan.report()
+    window = rep["window"]
     true_res = plan.true_residuals(b, prob.sigma, x)
     iters = ks[-1] - 1
     # shift updates the per-iteration formulation performs (one per active

@@ -194,15 +194,29 @@ void hbicg_plan_destroy(hbicg_plan *pl)
  * per-iteration updates) and the largest the arithmetic mode supports for
  * the others.  Exact-order results do not depend on the window, so an exact
  * plan clamps it to the exact replay's limit. */
-static int window_for(const hbicg_params *p, int variant)
+/* Shift-update window of a plan.  Exact order: at most HB_WMAX_EXACT.  A
+ * window the caller sets: as given, at most cap.  Default: normal 1;
+ * static/dynamic the smaller of HB_WAUTO and max_iter (a solve that ends
+ * inside its first window flushes once, at the end), halved while its ring
+ * buffers (2W+1 n-vectors) would take more than a quarter of the free
+ * device memory (at least 16). */
+static int window_for(const hbicg_params *p, int variant, int cap, size_t n, size_t free_bytes)
 {
-    const int wmax = p->exact_order ? HB_WMAX_EXACT : HB_WMAX;
+    const int wmax = p->exact_order ? HB_WMAX_EXACT : cap;
     if (p->window > 0) return p->window < wmax ? p->window : wmax;
-    return variant == 0 ? 1 : wmax;
+    if (variant == 0) return 1;
+    if (p->exact_order) return wmax;
+    int w = HB_WAUTO < wmax ? HB_WAUTO : wmax;
+    if (p->max_iter < w) w = p->max_iter;
+    while (w > 16 && (2.0 * w + 1.0) * (double)n * 8.0 > 0.25 * (double)free_bytes) w /= 2;
+    return w;
 }
 
+/* w_alloc: ring storage for windows up to this (the batched solver's
+ * autotuner); w_cap: the largest window the plan may choose. */
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
-                                 const hbicg_params *p, void *stream, int variant, int w_alloc)
+                                 const hbicg_params *p, void *stream, int variant, int w_alloc,
+                                 int w_cap)
 {
     int rc = 0;
     unsigned *hptr = NULL, *hblk = NULL;
@@ -222,7 +236,11 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->n = n;
     pl->M = M;
     pl->nz = A->nz;
-    pl->W = window_for(p, variant);
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+        pl->W = window_for(p, variant, w_cap, (size_t)n, free_b);
+    }
     pl->cap = p->max_iter + 1;
     pl->tol = p->tol;
     pl->growth_limit = p->growth_limit > 0.0 ? p->growth_limit : 0.0;
@@ -382,7 +400,7 @@ fail:
 hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,
                               const hbicg_params *p, void *stream)
 {
-    return plan_create_v(A, a_on_device, M, p ? p : params(), stream, 1, 0);
+    return plan_create_v(A, a_on_device, M, p ? p : params(), stream, 1, 0, HB_WMAX);
 }
 
 /* Enqueues one chunk of iterations followed by the done-flag publication. */
@@ -595,6 +613,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->status = st.status;
     R->M = pl->M;
     R->n = pl->n;
+    R->window = pl->W;
     R->dot_r = st.dot_r;
     R->dot_zero = st.dot_zero;
     R->rTr = st.rTr;
@@ -760,7 +779,7 @@ static int dropin(CSR_Matrix *A_diag, CSR_Matrix *A_offd, INFO_Matrix *A_info, d
     if (!A_diag->ptr || (A_diag->nz > 0 && (!A_diag->val || !A_diag->col))) return HBICG_E_ARG;
     const double t0 = now_ms();
     hbicg_csr A = {A_diag->val, A_diag->col, A_diag->ptr, A_diag->rows, A_diag->nz};
-    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0);
+    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0, HB_WMAX);
     if (!pl) return g_errno ? g_errno : HBICG_E_HIP;
     const double t1 = now_ms();
     const int k = hbicg_plan_solve(pl, r, sigma, seed, x_set, r, 0, NULL);
@@ -870,7 +889,7 @@ static int batch_layout(hbicg_batch *bt, int W, int group_cap)
     int rc = 0, *tab = NULL;
     size_t tiles = 0;
     if (!bt->exact) group_cap = 0;
-    if (W > (bt->exact ? HB_WMAX_EXACT : HB_WMAX)) return HBICG_E_ARG;
+    if (W > (bt->exact ? HB_WMAX_EXACT : HB_WBATCH)) return HBICG_E_ARG;
     for (int p = 0; p < bt->nprob; ++p) {
         const hbicg_plan *pl = bt->plans[p];
         const int G = group_cap > 0 && group_cap < pl->M ? group_cap : pl->M;
@@ -932,9 +951,9 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     bt->hdevs = (hb_dev *)calloc((size_t)nprob, sizeof(hb_dev));
     if (!bt->plans || !bt->hdevs) { rc = HBICG_E_NOMEM; goto fail; }
     size_t nsp = 0, nvec = 0, ncoef = 0;
-    const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WMAX;
+    const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WBATCH;
     for (int q = 0; q < nprob; ++q) {
-        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax);
+        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax);
         if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
         bt->hdevs[q] = bt->plans[q]->dev;
         nsp += (size_t)bt->plans[q]->nblk;
@@ -1105,6 +1124,7 @@ int hbicg_batch_solve(hbicg_batch *bt, const double *const *b, const double *con
         R->status = st.status;
         R->M = pl->M;
         R->n = pl->n;
+        R->window = bt->hdevs[p].W;
         R->dot_r = st.dot_r;
         R->dot_zero = st.dot_zero;
         R->rTr = st.rTr;

@@ -672,6 +672,9 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         const int spike = sh_spike;
         const int fl = do_switch || done_next || len >= d.W || spike;
         st->flush = fl;
+        /* the solve's last flush leaves p_j unused: x_j only */
+        const int xonly = fl && done_next && !do_switch && !spike;
+        st->flush_xonly = xonly;
         if (fl) {
             st->flush_k0 = win_k0;
             st->flush_len = len;
@@ -685,7 +688,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
          * vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1} and x_j, p_j of every
          * listed shift, read and written once */
         double bytes = 32.0;
-        if (fl) bytes += 8.0 * (2 * len + 1) + 32.0 * st->flush_cnt;
+        if (fl) bytes += 8.0 * (2 * len + 1) + (xonly ? 16.0 : 32.0) * st->flush_cnt;
         st->shift_bytes += bytes * (double)d.n;
     }
     if (d.growth_limit > 0.0) {
@@ -1035,25 +1038,30 @@ DEV void flush_coef_body(const hb_dev &d, int pos, const double *__restrict__ co
             blk[base + (e / 2) * (2 * HB_UG) + (e % 2) * HB_UG] = on ? c[e] : id;
         }
     }
-    if (!pch && t > 2 * Lc) return;
-    dd xk = {0.0, 0.0}, pk = {pch ? 1.0 : 0.0, 0.0};
     const int nb = min(lv, Lc);
-    for (int b = 0; b < nb; ++b) {
-        const double *c = coef + ((size_t)b * M + pos) * 6;
-        const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
-        /* x += c0 g_{2b+1} + c1 p ;  p = c4 (p + c2 g_{2b+1} + c3 g_{2b}) + c5 g_{2b+2} */
-        dd xn = dd_add(xk, dd_mul(pk, c1));
-        dd tp = pk;
-        if (t == 2 * b + 1) { xn = dd_add(xn, c0); tp = dd_add(tp, c2); }
-        if (t == 2 * b) tp = dd_add(tp, c3);
-        dd pn = dd_mul(tp, c4);
-        if (t == 2 * b + 2) pn = dd_add(pn, c5);
-        xk = xn;
-        pk = pn;
+    /* thread HB_BLOCK-1 carries the p_j component; the others carry the
+     * g_k components k = t, t + HB_BLOCK-1, ... (k <= 2Lc).  Component k
+     * is zero before step (k-1)/2, so its walk starts there. */
+    for (int kc = pch ? -1 : t; kc <= 2 * Lc; kc += HB_BLOCK - 1) {
+        dd xk = {0.0, 0.0}, pk = {pch ? 1.0 : 0.0, 0.0};
+        for (int b = kc > 0 ? (kc - 1) / 2 : 0; b < nb; ++b) {
+            const double *c = coef + ((size_t)b * M + pos) * 6;
+            const double c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
+            /* x += c0 g_{2b+1} + c1 p ;  p = c4 (p + c2 g_{2b+1} + c3 g_{2b}) + c5 g_{2b+2} */
+            dd xn = dd_add(xk, dd_mul(pk, c1));
+            dd tp = pk;
+            if (kc == 2 * b + 1) { xn = dd_add(xn, c0); tp = dd_add(tp, c2); }
+            if (kc == 2 * b) tp = dd_add(tp, c3);
+            dd pn = dd_mul(tp, c4);
+            if (kc == 2 * b + 2) pn = dd_add(pn, c5);
+            xk = xn;
+            pk = pn;
+        }
+        const int row = pch ? 0 : 1 + kc;
+        blk[(size_t)row * (2 * HB_UG) + pos % HB_UG] = xk.hi + xk.lo;
+        blk[(size_t)row * (2 * HB_UG) + HB_UG + pos % HB_UG] = pk.hi + pk.lo;
+        if (pch) break;
     }
-    const int row = pch ? 0 : 1 + t;
-    blk[(size_t)row * (2 * HB_UG) + pos % HB_UG] = xk.hi + xk.lo;
-    blk[(size_t)row * (2 * HB_UG) + HB_UG + pos % HB_UG] = pk.hi + pk.lo;
 }
 
 __global__ __launch_bounds__(HB_BLOCK) void k_flush_coef(hb_dev d)
@@ -1061,36 +1069,23 @@ __global__ __launch_bounds__(HB_BLOCK) void k_flush_coef(hb_dev d)
     flush_coef_body(d, blockIdx.x, d.coef, d.ccoef);
 }
 
-/* Tile bx: 256 elements.  Every thread first applies the seed p update of
- * the finished iteration to its element (src/solver_dynamic.c:208-210).  At
- * a flush the four wavefronts then walk the tile's four 64-element
- * sub-tiles together, wavefront w taking shift chunks w, w+4, ...: the
- * wavefronts read the same g_k slices at about the same time, so each g_k
- * comes from HBM once and from cache for the other chunks.  The chunk's
- * coefficients are uniform across the wavefront and arrive through scalar
- * loads; the wavefront index goes through readfirstlane so the compiler
- * knows it is uniform. */
-template <int UG>
-DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
-                       double *__restrict__ pset, const double *__restrict__ rring,
-                       const double *__restrict__ qring, const double *__restrict__ svec,
-                       const double *__restrict__ ccoef, const int *__restrict__ flist)
+/* The window flush of tile bx (256 elements): the four wavefronts walk the
+ * tile's four 64-element sub-tiles together, wavefront w taking shift
+ * chunks w, w+4, ...: the wavefronts read the same g_k slices at about the
+ * same time, so each g_k comes from HBM once and from cache for the other
+ * chunks.  The chunk's coefficients are uniform across the wavefront and
+ * arrive through scalar loads; the wavefront index goes through
+ * readfirstlane so the compiler knows it is uniform.  XONLY (the solve's
+ * last flush): p_j is neither formed nor written, halving the arithmetic. */
+template <int UG, bool XONLY>
+DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double *__restrict__ pset,
+                     const double *__restrict__ rring, const double *__restrict__ qring,
+                     const double *__restrict__ ccoef, const int *__restrict__ flist)
 {
     const hb_state *st = d.st;
-    if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     const int n = d.n, W = d.W;
-    const int i0 = bx * HB_BLOCK + threadIdx.x;
-    if (i0 < n) {
-        double *ps = pset + off(st->prev_seed, n) + i0;
-        const double pb = st->pb, pbw = st->pbw;
-        const double r = rring[off(st->kprev % (W + 1), n) + i0];
-        double pe = pb * *ps;
-        pe = pe + r;
-        *ps = pe + pbw * svec[i0];
-    }
-    if (!st->flush) return;
     const int k0 = st->flush_k0, cnt = st->flush_cnt;
-    const int Lc = st->flush_len - st->flush_tail, tail = st->flush_tail;
+    const int Lc = st->flush_len - st->flush_tail, tail = XONLY ? 0 : st->flush_tail;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
@@ -1116,7 +1111,7 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 x[u] = __builtin_fma(cb[u], p[u], x[u]);
-                p[u] = cb[UG + u] * p[u];
+                if (!XONLY) p[u] = cb[UG + u] * p[u];
             }
             int rs = (k0 - 1) % (W + 1), qs = k0 % W;
             double gr = rring[off(rs, n) + i];
@@ -1125,7 +1120,7 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[u], gr, x[u]);
-                    p[u] = __builtin_fma(ck[UG + u], gr, p[u]);
+                    if (!XONLY) p[u] = __builtin_fma(ck[UG + u], gr, p[u]);
                 }
             }
             for (int b = 0; b < Lc; ++b) {
@@ -1137,12 +1132,12 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[u], gq, x[u]);
-                    p[u] = __builtin_fma(ck[UG + u], gq, p[u]);
+                    if (!XONLY) p[u] = __builtin_fma(ck[UG + u], gq, p[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[2 * UG + u], gr, x[u]);
-                    p[u] = __builtin_fma(ck[3 * UG + u], gr, p[u]);
+                    if (!XONLY) p[u] = __builtin_fma(ck[3 * UG + u], gr, p[u]);
                 }
             }
             if (tail) {
@@ -1167,11 +1162,37 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
                 if (u < nu) {
                     const int j = flist[c * UG + u];
                     __builtin_nontemporal_store(x[u], xset + off(j, n) + i);
-                    __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
+                    if (!XONLY) __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
                 }
             }
         }
     }
+}
+
+/* Tile bx: every thread first applies the seed p update of the finished
+ * iteration to its element (src/solver_dynamic.c:208-210), then the tile
+ * takes part in the window flush when one is due. */
+template <int UG>
+DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
+                       double *__restrict__ pset, const double *__restrict__ rring,
+                       const double *__restrict__ qring, const double *__restrict__ svec,
+                       const double *__restrict__ ccoef, const int *__restrict__ flist)
+{
+    const hb_state *st = d.st;
+    if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
+    const int n = d.n, W = d.W;
+    const int i0 = bx * HB_BLOCK + threadIdx.x;
+    if (i0 < n) {
+        double *ps = pset + off(st->prev_seed, n) + i0;
+        const double pb = st->pb, pbw = st->pbw;
+        const double r = rring[off(st->kprev % (W + 1), n) + i0];
+        double pe = pb * *ps;
+        pe = pe + r;
+        *ps = pe + pbw * svec[i0];
+    }
+    if (!st->flush) return;
+    if (st->flush_xonly) flush_apply<UG, true>(d, bx, xset, pset, rring, qring, ccoef, flist);
+    else flush_apply<UG, false>(d, bx, xset, pset, rring, qring, ccoef, flist);
 }
 
 DEV void shift_cc_tile(const hb_dev &d, int bx)

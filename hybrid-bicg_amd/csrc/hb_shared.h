@@ -13,7 +13,9 @@ extern "C" {
 #define HB_BLOCK        256   /* threads per workgroup of every kernel        */
 #define HB_NNZ_CAP      2048  /* nonzeros staged in LDS by one SpMV workgroup */
 #define HB_ROWS_CAP     256   /* rows per SpMV workgroup                       */
-#define HB_WMAX         64    /* largest shift-update window                   */
+#define HB_WMAX         1024  /* largest shift-update window                   */
+#define HB_WAUTO        512   /* default window of static/dynamic, memory permitting */
+#define HB_WBATCH       64    /* largest window of the batched solver          */
 #define HB_WMAX_EXACT   16    /* largest window of the exact-order replay      */
 #define HB_UG           16    /* shifts per chunk of the combined-coefficient flush */
 #define HB_CC_ROWS(W)   (2 * (W) + 5) /* rows of one chunk's coefficient block  */
@@ -55,6 +57,7 @@ typedef struct hb_state {
     int k4_epoch;      /* k when the shift kernel last ran (runs while < k) */
     int status;        /* HB_ST_* bits                                    */
     int flush_tail;    /* 1: the flush applies its last iteration on its own */
+    int flush_xonly;   /* 1: last flush of the solve: x_j only, p_j not needed */
 } hb_state;
 
 /* Every pointer below addresses global device memory.  Device code sees

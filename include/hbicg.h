@@ -31,8 +31,11 @@ extern "C" {
 typedef struct hbicg_params {
     double tol;          /* stopping rule, default EPS (1e-12)                           */
     int    max_iter;     /* iteration cap, default MAX_ITER (2000)                       */
-    int    window;       /* shift-update window 1..64 (exact_order: capped at 16); 0 = variant
-                            default: normal 1, static/dynamic 64 (exact_order 16)           */
+    int    window;       /* shift-update window 1..1024 (exact_order: capped at 16); 0 = variant
+                            default: normal 1, static/dynamic min(512, max_iter), halved
+                            while its 2W+1 ring vectors would take over a quarter of the
+                            free device memory (exact_order 16).  x_j and p_j are written
+                            at a window's end; the solve's last window writes x_j only  */
     int    exact_order;  /* 1: serial dots, row-sequential SpMV and per-iteration replay of
                             the shift updates (reference order and roundings, any window);
                             0: tree reductions and the combined-coefficient window flush
@@ -61,6 +64,7 @@ typedef struct hbicg_report {
     double t_setup_ms;   /* plan creation (allocation, upload, SpMV binning) */
     double t_solve_ms;   /* device time of init + iteration loop              */
     double t_total_ms;   /* wall time of the whole call                        */
+    int    window;       /* shift-update window the solve ran with             */
 } hbicg_report;
 
 void hbicg_default_params(hbicg_params *p);
@@ -157,7 +161,7 @@ typedef struct hbicg_tuning {
 } hbicg_tuning;
 
 /* A[p] (host, a_on_device = 0, or device memory), M[p] shifts each.
- * params: tol, max_iter, window (0: 64, exact_order 16), exact_order, graph
+ * params: tol, max_iter, window (0 or above 64: 64, exact_order 16), exact_order, graph
  * (0 off), chunk.  Ring storage is sized for the largest window the mode
  * allows, so autotuning can change it. */
 hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, const int *M,

@@ -54,7 +54,7 @@ def test_params_roundtrip():
     r = _lib.Params()
     assert L.hbicg_get_params(C.byref(r)) == 0
     assert (r.tol, r.max_iter, r.window, r.exact_order) == (1e-10, 77, 4, 1)
-    bad = make_params(window=65)
+    bad = make_params(window=1025)
     assert L.hbicg_set_params(C.byref(bad)) == -1
     bad = make_params(tol=-1.0)
     assert L.hbicg_set_params(C.byref(bad)) == -1

@@ -30,7 +30,8 @@ def test_batch_equals_plain_bitwise(small_batch):
     bt = Batch(small_batch, params)
     k, xs, ms = bt.solve()
     for i, p in enumerate(small_batch):
-        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1))
+        # the batch's default window is 64 (the plan's default is longer)
+        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1, window=64))
         kp, xp, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, p.seed)
         rep, stop, sw = bt.report(i)
         prep, pstop, psw = pl.report()

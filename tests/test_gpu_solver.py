@@ -104,14 +104,25 @@ def test_window_bitwise_independent_exact(c0, window):
 
 
 @pytest.mark.parametrize("seed", [0, 8])
-@pytest.mark.parametrize("window", [3, 8, 16, 33, 64])
+@pytest.mark.parametrize("window", [3, 8, 16, 33, 64, 256, 1024])
 def test_window_independent_default(c0, oracle_c0, window, seed):
     """Default mode: the combined-coefficient flush regroups each window's
-    sums; results move only by roundings as the window changes."""
+    sums; results move only by roundings as the window changes.  Windows of
+    256 and 1024 outlast the solve (about 120 iterations): one flush at the
+    end (or at the seed switch and at the end), x_j only at the last."""
     mk = lambda w: make_params(tol=c0.tol, window=w)
     base = dropin_solve(c0.A, c0.b, c0.sigma, seed, "static", mk(1))
     got = dropin_solve(c0.A, c0.b, c0.sigma, seed, "static", mk(window))
+    assert got.report["window"] == window
     _same_up_to_window(got, base, seed, oracle_c0[seed].switches)
+
+
+def test_default_window_choice(c0):
+    """static/dynamic default to a window of min(512, max_iter) when the
+    ring buffers fit (they do at n = 4096); normal runs window 1."""
+    for variant, mi, w in (("static", 2000, 512), ("dynamic", 300, 300), ("normal", 2000, 1)):
+        got = dropin_solve(c0.A, c0.b, c0.sigma, 0, variant, make_params(tol=c0.tol, max_iter=mi))
+        assert got.report["window"] == w, (variant, got.report["window"])
 
 
 @pytest.mark.parametrize("seed", [0, 8])
