@@ -31,7 +31,8 @@
  *
  * Every floating-point operation is rounded separately (the library is
  * built with -ffp-contract=off) so that each elementwise step reproduces the
- * reference's own sequence of roundings.  Reductions: workgroup partials,
+ * reference's own sequence of roundings; the default mode's window flush
+ * and double-double coefficients ask for fused multiply-adds explicitly.  Reductions: workgroup partials,
  * combined in a fixed order by a one-workgroup finalize (deterministic).
  * With d.exact set, every dot is summed serially from 0.0 in ascending index
  * order and every SpMV row sequentially, the reference's order
