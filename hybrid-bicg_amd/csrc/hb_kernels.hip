@@ -1170,6 +1170,85 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
     }
 }
 
+/* The solve's last flush (x_j only) of tile bx.  Each step of the window
+ * loop waits for its g_k loads (cache hits, but hundreds of cycles), so a
+ * lane carries two elements, i and i + 64: every wait then covers twice
+ * the multiply-adds, and the loop runs half as many steps. */
+template <int UG>
+DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, const double *__restrict__ pset,
+                        const double *__restrict__ rring, const double *__restrict__ qring,
+                        const double *__restrict__ ccoef, const int *__restrict__ flist)
+{
+    const hb_state *st = d.st;
+    const int n = d.n, W = d.W;
+    const int k0 = st->flush_k0, cnt = st->flush_cnt;
+    const int Lc = st->flush_len - st->flush_tail;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nch = (cnt + UG - 1) / UG;
+    const size_t cstride = (size_t)HB_CC_ROWS(W) * (2 * UG);
+    for (int sub = 0; sub < HB_BLOCK / 64; sub += 2) {
+        const int i0 = bx * HB_BLOCK + sub * 64 + lane;
+        if (i0 >= n) break;
+        const bool has1 = i0 + 64 < n;
+        const int i1 = has1 ? i0 + 64 : i0;   /* a clamped duplicate when past the end */
+        for (int c = wave; c < nch; c += HB_BLOCK / 64) {
+            const double *__restrict__ cb = ccoef + (size_t)c * cstride;
+            const int nu = min(UG, cnt - c * UG);
+            double x0[UG], x1[UG];
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                if (u < nu) {
+                    const int j = flist[c * UG + u];
+                    x0[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pset + off(j, n) + i0),
+                                          __builtin_nontemporal_load(xset + off(j, n) + i0));
+                    x1[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pset + off(j, n) + i1),
+                                          __builtin_nontemporal_load(xset + off(j, n) + i1));
+                } else {
+                    x0[u] = 0.0;
+                    x1[u] = 0.0;
+                }
+            }
+            int rs = (k0 - 1) % (W + 1), qs = k0 % W;
+            {
+                const double *gp = rring + off(rs, n);
+                const double g0 = gp[i0], g1 = gp[i1];
+                const double *__restrict__ ck = cb + 2 * UG;
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    x0[u] = __builtin_fma(ck[u], g0, x0[u]);
+                    x1[u] = __builtin_fma(ck[u], g1, x1[u]);
+                }
+            }
+            for (int b = 0; b < Lc; ++b) {
+                rs = rs + 1 == W + 1 ? 0 : rs + 1;
+                const double *qp = qring + off(qs, n), *rp = rring + off(rs, n);
+                const double q0 = qp[i0], q1 = qp[i1], r0 = rp[i0], r1 = rp[i1];
+                qs = qs + 1 == W ? 0 : qs + 1;
+                const double *__restrict__ ck = cb + (size_t)(2 * b + 2) * (2 * UG);
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    x0[u] = __builtin_fma(ck[u], q0, x0[u]);
+                    x1[u] = __builtin_fma(ck[u], q1, x1[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    x0[u] = __builtin_fma(ck[2 * UG + u], r0, x0[u]);
+                    x1[u] = __builtin_fma(ck[2 * UG + u], r1, x1[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UG; ++u) {
+                if (u < nu) {
+                    const int j = flist[c * UG + u];
+                    __builtin_nontemporal_store(x0[u], xset + off(j, n) + i0);
+                    if (has1) __builtin_nontemporal_store(x1[u], xset + off(j, n) + i1);
+                }
+            }
+        }
+    }
+}
+
 /* Tile bx: every thread first applies the seed p update of the finished
  * iteration to its element (src/solver_dynamic.c:208-210), then the tile
  * takes part in the window flush when one is due. */
@@ -1192,7 +1271,7 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
         *ps = pe + pbw * svec[i0];
     }
     if (!st->flush) return;
-    if (st->flush_xonly) flush_apply<UG, true>(d, bx, xset, pset, rring, qring, ccoef, flist);
+    if (st->flush_xonly) flush_apply_x2<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
     else flush_apply<UG, false>(d, bx, xset, pset, rring, qring, ccoef, flist);
 }
 
