@@ -105,7 +105,7 @@ struct hbicg_plan {
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
-    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *ccoef, *grow, *part, *trace;
+    double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *ccoef, *grow, *part, *trace, *p_snap;
     int *stop, *stop_iter, *act_len, *flist, *fpos, *switch_k, *switch_to;
     hb_state *st;
     int *flag_host;      /* pinned: {done, k} */
@@ -171,7 +171,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -321,6 +321,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->grow = (double *)dalloc((size_t)2 * M * 8, &rc);
     if (pl->want_trace) pl->trace = (double *)dalloc(cap * (size_t)M * 8, &rc);
     pl->part = (double *)dalloc((size_t)maxparts * 2 * 8, &rc);
+    pl->p_snap = (double *)dalloc(nb, &rc);
     pl->stop = (int *)dalloc((size_t)M * 4, &rc);
     pl->stop_iter = (int *)dalloc((size_t)M * 4, &rc);
     pl->act_len = (int *)dalloc((size_t)M * 4, &rc);
@@ -359,7 +360,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->stop = pl->stop;
     d->stop_iter = pl->stop_iter;
     d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
-    d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st;
+    d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st; d->p_snap = pl->p_snap;
 
     /* 16-bit column offsets for the tiles whose entries stay within 32767
      * columns of the tile's first row */
@@ -623,13 +624,16 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
         const double n = (double)pl->n, nz = (double)pl->nz;
         const double spmv = 8.0 * nz + pl->col_bytes + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
         /* algorithmic bytes per launch: SpMV 1 gathers p_s and reads r^
-         * (writes s), SpMV 2 reads r, s and r^ (writes q and y), the seed
-         * update reads q, y, p_s, x_s, r^ (writes x_s, r); the finalizes
-         * read the workgroup partial pairs; the shift kernel's bytes are
-         * counted on the device (seed p update plus the window flushes) */
+         * (writes s), SpMV 2 reads r and s (writes q and y), the seed update
+         * reads q, y, r^ and writes r (exact order: also p_s and x_s in,
+         * x_s out; the default mode defers x_s to the window flush); the
+         * finalizes read the workgroup partial pairs; the shift kernel's
+         * bytes are counted on the device (seed p update plus the window
+         * flushes) */
         const double fin_spmv = 16.0 * pl->nblk, fin_seed = 16.0 * pl->dev.vec_grid;
+        const double seed_upd = (pl->exact ? 56.0 : 32.0) * n;
         const double per_launch[HB_NSTAGE] = {spmv + 24.0 * n, fin_spmv, spmv + 32.0 * n, fin_spmv,
-                                              56.0 * n, fin_seed, 0.0, 0.0};
+                                              seed_upd, fin_seed, 0.0, 0.0};
         const int iters = st.k - 1;
         for (int sg = 0; sg < HB_NSTAGE; ++sg) {
             hbicg_kernel_stat *K = &pl->kstat[sg];

@@ -152,6 +152,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
     __shared__ unsigned rp[HB_ROWS_CAP + 1];
     const int n = d.n;
     int switched = 0;
+    bool snap = false;   /* SEED1: keep p_s as the window-start p of the seed */
     double sig, zscale = 1.0, nalpha = 0.0;
     const double *xa;
     double *rprev = nullptr, *qout = nullptr;
@@ -168,6 +169,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             xa = xbase + off(seed, n);
             switched = st->switched;
             zscale = st->zscale;
+            snap = !d.exact && k == st->win_k0;
         } else {
             xa = rprev;
             nalpha = -d.aa[k];
@@ -197,6 +199,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         if (MODE == SP_SEED1) {
             const double si = t + sig * a;
             out1[i] = si;
+            if (snap) d.p_snap[i] = a;
             part0 += bb * si;
             if (switched) {
                 const double rr = c * zscale;
@@ -482,7 +485,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     const int tid = threadIdx.x;
     /* every value this stage reads, loaded before its first store */
     const int k = st->k, s = st->seed, M = d.M;
-    const int win_k0 = st->win_k0;
+    const int win_k0 = st->win_k0, win_seedpos = st->win_seedpos;
     const double rTr_old = st->rTr_old, dot_zero = st->dot_zero;
     const double ak = d.aa[k], wk = d.wa[k], bkm1 = d.ba[k - 1], akm1 = d.aa[k - 1];
     const double sig_s = d.sigma[s];
@@ -504,6 +507,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     const double thr = (d.tol * d.tol) * dot_zero;
     const int slot = k - win_k0;
     double *coef = d.coef + (size_t)slot * (size_t)M * 6;
+    int seedpos = slot == 0 ? -1 : win_seedpos;
 
     if (slot == 0) {
         /* A window opens: its shift list is every shift updated in the
@@ -531,7 +535,18 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             base += sh_nc[HB_BLOCK - 1];
             __syncthreads();
         }
-        if (tid == 0) st->flush_cnt = base;
+        /* default mode: the seed's own x update (x_s += alpha~ p_s +
+         * omega~ q, src/solver_dynamic.c:194-195) is deferred too, the seed
+         * listed last with the coefficients of its recurrence */
+        seedpos = d.exact ? -1 : base;
+        if (tid == 0) {
+            if (seedpos >= 0) {
+                d.flist[seedpos] = s;
+                d.fpos[s] = seedpos;
+            }
+            st->win_seedpos = seedpos;
+            st->flush_cnt = base + (seedpos >= 0 ? 1 : 0);
+        }
     }
 
     /* Opt-in guard of the default mode (growth_limit > 0): how large the
@@ -586,6 +601,15 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             zp = fabs(1.0 / (znew * pik));
         } else {
             zp = 1.0;
+            if (seedpos >= 0) {
+                /* the shift recurrence at sigma_j = sigma_s (pi = zeta = 1):
+                 * x += omega~ q + alpha~ p,
+                 * p = beta~ (p + (omega~/alpha~) q - (omega~/alpha~) r_old) + r */
+                double *c = coef + (size_t)seedpos * 6;
+                const double wa = wk / ak;
+                c[0] = wk; c[1] = ak; c[2] = wa; c[3] = -wa; c[4] = beta; c[5] = 1.0;
+                d.act_len[seedpos] = slot + 1;
+            }
         }
         /* the residual estimate the reference prints under
          * DISPLAY_SIGMA_RESIDUAL (src/solver_dynamic.c:286-292) */
@@ -716,13 +740,18 @@ DEV void seed_update_body(const hb_dev &d, int b, const double *__restrict__ qri
     const double *ps = pset + off(s, n);
     double *xs = xset + off(s, n);
     double part0 = 0.0, part1 = 0.0;
+    /* exact order updates x_s here, in the reference's order; the default
+     * mode defers it to the window flush (the seed is listed there) */
+    const bool upd_x = d.exact;
 #pragma unroll 2
     for (int i = b * HB_BLOCK + threadIdx.x; i < n; i += nb * HB_BLOCK) {
         const double qi = q[i];
-        double xi = xs[i];
-        xi = xi + a * ps[i];
-        xi = xi + w * qi;
-        xs[i] = xi;
+        if (upd_x) {
+            double xi = xs[i];
+            xi = xi + a * ps[i];
+            xi = xi + w * qi;
+            xs[i] = xi;
+        }
         const double ri = qi + nw * y[i];
         rn[i] = ri;
         part0 += ri * ri;
@@ -1087,6 +1116,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
     const int n = d.n, W = d.W;
     const int k0 = st->flush_k0, cnt = st->flush_cnt;
     const int Lc = st->flush_len - st->flush_tail, tail = XONLY ? 0 : st->flush_tail;
+    const int sp = st->win_seedpos;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
@@ -1103,7 +1133,9 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
                 if (u < nu) {
                     const int j = flist[c * UG + u];
                     x[u] = __builtin_nontemporal_load(xset + off(j, n) + i);
-                    p[u] = __builtin_nontemporal_load(pset + off(j, n) + i);
+                    /* the seed's window-start p is the snapshot (its p_s
+                     * has moved on every iteration) */
+                    p[u] = __builtin_nontemporal_load(c * UG + u == sp ? d.p_snap + i : pset + off(j, n) + i);
                 } else {
                     x[u] = 0.0;
                     p[u] = 0.0;
@@ -1163,7 +1195,9 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
                 if (u < nu) {
                     const int j = flist[c * UG + u];
                     __builtin_nontemporal_store(x[u], xset + off(j, n) + i);
-                    if (!XONLY) __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
+                    /* the seed keeps its own p_s (the regrouped one is
+                     * only its x's companion) */
+                    if (!XONLY && c * UG + u != sp) __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
                 }
             }
         }
@@ -1183,6 +1217,7 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
     const int n = d.n, W = d.W;
     const int k0 = st->flush_k0, cnt = st->flush_cnt;
     const int Lc = st->flush_len - st->flush_tail;
+    const int sp = st->win_seedpos;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
@@ -1200,9 +1235,10 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
                     const int j = flist[c * UG + u];
-                    x0[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pset + off(j, n) + i0),
+                    const double *pj = c * UG + u == sp ? d.p_snap : pset + off(j, n);
+                    x0[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i0),
                                           __builtin_nontemporal_load(xset + off(j, n) + i0));
-                    x1[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pset + off(j, n) + i1),
+                    x1[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i1),
                                           __builtin_nontemporal_load(xset + off(j, n) + i1));
                 } else {
                     x0[u] = 0.0;

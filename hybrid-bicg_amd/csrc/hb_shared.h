@@ -58,6 +58,8 @@ typedef struct hb_state {
     int status;        /* HB_ST_* bits                                    */
     int flush_tail;    /* 1: the flush applies its last iteration on its own */
     int flush_xonly;   /* 1: last flush of the solve: x_j only, p_j not needed */
+    int win_seedpos;   /* default mode: list position of the seed in the pending
+                          window (its x is deferred like a shift's), else -1 */
 } hb_state;
 
 /* Every pointer below addresses global device memory.  Device code sees
@@ -107,6 +109,7 @@ typedef struct hb_dev {
     HB_G int    *fpos;         /* [M] position of shift j in the list      */
     HB_G int    *switch_k, *switch_to; /* [M] switch log                   */
     HB_G double *part;         /* workgroup partial sums [max(nblk,vec_grid)*2] */
+    HB_G double *p_snap;       /* [n] the seed's p at the pending window's start */
     HB_G double *trace;        /* [cap][M] estimated residual per iteration and shift,
                                   -1 once stopped (NULL: not recorded)     */
     HB_G hb_state *st;
