@@ -407,6 +407,28 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step")
     rep, stop_iter, switches = plan.report()
     window = rep["window"]
+    value_table = rep["value_table"]
+
+    # the same solve with the matrix read as 8-byte values (no value table),
+    # for comparison: the format changes bytes, not results
+    general = None
+    if value_table:
+        gplan = Plan(prob.A, M, make_params(tol=prob.tol, max_iter=prob.max_iter, window=window,
+                                            graph=graph, value_index=False), device=dev)
+        xg = torch.zeros_like(x)
+        gplan.solve(b, prob.sigma, prob.seed, xg)
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        for _ in range(args.steps):
+            xg.zero_()
+            gplan.solve(b, prob.sigma, prob.seed, xg)
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - tg) / args.steps
+        same = bool(torch.equal(xg, x))
+        general = {"ms_per_step": round(tg * 1e3, 3), "value": round(M / tg, 3), "x_bitwise_equal": same}
+        log(f"8-byte values: {tg * 1e3:.1f} ms/step, x bitwise equal {same}")
+        gplan.close()
+        del xg
     true_res = plan.true_residuals(b, prob.sigma, x)
     iters = ks[-1] - 1
     # shift updates the per-iteration formulation performs (one per active
@@ -499,7 +521,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"conv-diff 3-D 7-point {args.grid}^3 (n={n}, nnz={prob.A.nnz}), "
                                f"{M} shifts log-spaced [1e-3,1], relres {prob.tol:g}, seed=smallest shift",
-                   "variant": args.variant, "window": window, "hipgraph": bool(graph)},
+                   "variant": args.variant, "window": window, "hipgraph": bool(graph),
+                   "spmv_value_table": value_table},
+        "without_value_table": general,
         "iterations": iters,
         "seed_switches": switches,
         "true_residual_max": float(true_res.max()),

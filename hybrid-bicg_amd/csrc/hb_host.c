@@ -11,6 +11,7 @@
 #define _POSIX_C_SOURCE 199309L
 #include <hip/hip_runtime_api.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -41,6 +42,7 @@ void hbicg_default_params(hbicg_params *p)
     p->profile = 0;
     p->growth_limit = 0.0;
     p->trace = 0;
+    p->value_index = 1;
 }
 
 static const hbicg_params *params(void)
@@ -102,6 +104,9 @@ struct hbicg_plan {
     short *col16;
     unsigned *cflag;
     double col_bytes;    /* column-index bytes per SpMV (16-bit tiles count 2 per entry) */
+    double val_bytes;    /* value bytes per SpMV (8 per entry, 1 through a value table) */
+    unsigned char *vidx; /* value table of the SpMV, when the matrix has one */
+    double *vdict;
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
@@ -171,7 +176,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -212,11 +217,39 @@ static int window_for(const hbicg_params *p, int variant, int cap, size_t n, siz
     return w;
 }
 
+/* Value table of a matrix with at most HB_VDICT distinct values (compared
+ * as bit patterns, so -0.0 and 0.0 stay apart): dict gets them in order of
+ * first appearance, idx[e] the table position of val[e].  Returns the table
+ * size, or 0 when the matrix has more distinct values. */
+static int build_value_table(const double *val, size_t nz, double *dict, unsigned char *idx)
+{
+    enum { SLOTS = 1024 };   /* open addressing, at most a quarter full */
+    uint64_t key[SLOTS];
+    short pos[SLOTS];
+    for (int h = 0; h < SLOTS; ++h) pos[h] = -1;
+    int nd = 0;
+    for (size_t e = 0; e < nz; ++e) {
+        uint64_t b;
+        memcpy(&b, &val[e], sizeof b);
+        unsigned h = (unsigned)((b * 0x9E3779B97F4A7C15ull) >> 54);
+        while (pos[h] >= 0 && key[h] != b) h = (h + 1) & (SLOTS - 1);
+        if (pos[h] < 0) {
+            if (nd == HB_VDICT) return 0;
+            key[h] = b;
+            pos[h] = (short)nd;
+            dict[nd++] = val[e];
+        }
+        idx[e] = (unsigned char)pos[h];
+    }
+    return nd;
+}
+
 /* w_alloc: ring storage for windows up to this (the batched solver's
- * autotuner); w_cap: the largest window the plan may choose. */
+ * autotuner); w_cap: the largest window the plan may choose; allow_vi: the
+ * SpMV may read the matrix through a value table (not in batches). */
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
                                  const hbicg_params *p, void *stream, int variant, int w_alloc,
-                                 int w_cap)
+                                 int w_cap, int allow_vi)
 {
     int rc = 0;
     unsigned *hptr = NULL, *hblk = NULL;
@@ -387,6 +420,54 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
         free(hf);
     }
 
+    /* value table: a matrix with at most HB_VDICT distinct values is read
+     * as one byte per entry (device matrices: a prefix is checked first) */
+    pl->val_bytes = 8.0 * (double)pl->nz;
+    if (allow_vi && pl->nz > 0) {
+        const size_t nzs = pl->nz;
+        double *tmp = NULL, dict[HB_VDICT];
+        unsigned char *hidx = (unsigned char *)malloc(nzs);
+        int nd = 0;
+        if (!hidx) { rc = HBICG_E_NOMEM; goto fail; }
+        if (a_on_device) {
+            const size_t pre = nzs < 65536 ? nzs : 65536;
+            tmp = (double *)malloc(nzs * sizeof(double));
+            if (!tmp || hipMemcpy(tmp, A->val, pre * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+                free(hidx);
+                free(tmp);
+                rc = tmp ? HBICG_E_HIP : HBICG_E_NOMEM;
+                goto fail;
+            }
+            if (build_value_table(tmp, pre, dict, hidx) > 0) {
+                if (pre < nzs && hipMemcpy(tmp + pre, A->val + pre, (nzs - pre) * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+                    free(hidx);
+                    free(tmp);
+                    rc = HBICG_E_HIP;
+                    goto fail;
+                }
+                nd = build_value_table(tmp, nzs, dict, hidx);
+            }
+        } else {
+            nd = build_value_table(A->val, nzs, dict, hidx);
+        }
+        if (nd > 0) {
+            pl->vidx = (unsigned char *)dalloc(nzs, &rc);
+            pl->vdict = (double *)dalloc(HB_VDICT * sizeof(double), &rc);
+            if (rc == 0 && (hipMemcpy(pl->vidx, hidx, nzs, hipMemcpyHostToDevice) != hipSuccess ||
+                            hipMemcpy(pl->vdict, dict, (size_t)nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+                rc = HBICG_E_HIP;
+            if (rc == 0) {
+                d->vidx = pl->vidx;
+                d->vdict = pl->vdict;
+                d->nvdict = nd;
+                pl->val_bytes = (double)nzs;
+            }
+        }
+        free(hidx);
+        free(tmp);
+        if (rc) goto fail;
+    }
+
     if (a_on_device) free(hptr);
     free(hblk);
     return pl;
@@ -401,7 +482,8 @@ fail:
 hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,
                               const hbicg_params *p, void *stream)
 {
-    return plan_create_v(A, a_on_device, M, p ? p : params(), stream, 1, 0, HB_WMAX);
+    const hbicg_params *pp = p ? p : params();
+    return plan_create_v(A, a_on_device, M, pp, stream, 1, 0, HB_WMAX, pp->value_index);
 }
 
 /* Enqueues one chunk of iterations followed by the done-flag publication. */
@@ -615,14 +697,15 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->M = pl->M;
     R->n = pl->n;
     R->window = pl->W;
+    R->value_table = pl->dev.nvdict;
     R->dot_r = st.dot_r;
     R->dot_zero = st.dot_zero;
     R->rTr = st.rTr;
     R->t_solve_ms = ms;
     R->t_total_ms = now_ms() - t_start;
     if (pl->profile) {
-        const double n = (double)pl->n, nz = (double)pl->nz;
-        const double spmv = 8.0 * nz + pl->col_bytes + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
+        const double n = (double)pl->n;
+        const double spmv = pl->val_bytes + pl->col_bytes + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
         /* algorithmic bytes per launch: SpMV 1 gathers p_s and reads r^
          * (writes s), SpMV 2 reads r and s (writes q and y), the seed update
          * reads q, y, r^ and writes r (exact order: also p_s and x_s in,
@@ -783,7 +866,7 @@ static int dropin(CSR_Matrix *A_diag, CSR_Matrix *A_offd, INFO_Matrix *A_info, d
     if (!A_diag->ptr || (A_diag->nz > 0 && (!A_diag->val || !A_diag->col))) return HBICG_E_ARG;
     const double t0 = now_ms();
     hbicg_csr A = {A_diag->val, A_diag->col, A_diag->ptr, A_diag->rows, A_diag->nz};
-    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0, HB_WMAX);
+    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0, HB_WMAX, p->value_index);
     if (!pl) return g_errno ? g_errno : HBICG_E_HIP;
     const double t1 = now_ms();
     const int k = hbicg_plan_solve(pl, r, sigma, seed, x_set, r, 0, NULL);
@@ -957,7 +1040,7 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     size_t nsp = 0, nvec = 0, ncoef = 0;
     const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WBATCH;
     for (int q = 0; q < nprob; ++q) {
-        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax);
+        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax, 0);
         if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
         bt->hdevs[q] = bt->plans[q]->dev;
         nsp += (size_t)bt->plans[q]->nblk;

@@ -39,6 +39,7 @@
  * (src/openmp_vector.c:68-74, src/openmp_matrix.c:47-58).
  */
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <math.h>
 #include "hb_shared.h"
 #include "hb_launch.h"
@@ -138,7 +139,7 @@ DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
  * gathers r_{k-1} (in rring) and s, PLAIN gathers x. */
 enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 
-template <int MODE, bool EXACT>
+template <int MODE, bool EXACT, bool VI = false>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ col, const short *__restrict__ col16,
                    const unsigned *__restrict__ cflag, const unsigned *__restrict__ ptr,
@@ -230,153 +231,179 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         }
     };
 
-    if (N <= HB_NNZ_CAP) {
-        /* rows reduced by T lanes each, T the largest power of two with
-         * T*R <= HB_BLOCK (at most 64; 1 in exact order): every row has one
-         * group of lanes, and lane 0 of the group owns its epilogue */
-        int T = 1;
-        if (!EXACT) {
-            const int q = HB_BLOCK / R;
-            while ((T << 1) <= q && (T << 1) <= 64) T <<= 1;
-        }
-        const int row = tid / T, lane = tid & (T - 1);
-        const unsigned i_me = r0 + (unsigned)min(row, R - 1);
-        double oa = 0.0, ob = 0.0, oc = 0.0;
-        operands(i_me, oa, ob, oc);
+    /* VI: the matrix is read as one byte per entry into its table of
+     * distinct values (at most HB_VDICT, staged in LDS) instead of eight
+     * bytes per entry; the products are the same doubles */
+    __shared__ double vd[VI ? HB_VDICT : 1];
+    auto tile = [&]() {
+        const unsigned char *__restrict__ vidx = d.vidx;
+        auto vat = [&](unsigned e) -> double {
+            if constexpr (VI) return vd[vidx[e]];
+            else return val[e];
+        };
+        if (N <= HB_NNZ_CAP) {
+            /* rows reduced by T lanes each, T the largest power of two with
+             * T*R <= HB_BLOCK (at most 64; 1 in exact order): every row has one
+             * group of lanes, and lane 0 of the group owns its epilogue */
+            int T = 1;
+            if (!EXACT) {
+                const int q = HB_BLOCK / R;
+                while ((T << 1) <= q && (T << 1) <= 64) T <<= 1;
+            }
+            const int row = tid / T, lane = tid & (T - 1);
+            const unsigned i_me = r0 + (unsigned)min(row, R - 1);
+            double oa = 0.0, ob = 0.0, oc = 0.0;
+            operands(i_me, oa, ob, oc);
 
-        constexpr int U = HB_NNZ_CAP / HB_BLOCK;
-        static_assert(U * HB_BLOCK == HB_NNZ_CAP, "tile capacity");
-        if (MODE == SP_SEED2 && T == 1 && narrow && N > 0) {
-            /* One row per thread.  Values and 16-bit column offsets are
-             * staged in LDS, then each thread walks its own row: the lanes of
-             * a wave gather the same stencil slot of consecutive rows, i.e.
-             * consecutive addresses, where the entry-per-lane order touches
-             * several cache lines per instruction.  Two gathers per entry
-             * (r and s), four entries in flight.  The row sum runs in CSR
-             * order with the products rounded on their own, as in the LDS
-             * products path below. */
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int e = tid + u * HB_BLOCK;
-                const unsigned ec = e0 + (unsigned)min(e, N - 1);
-                const double vv = val[ec];
-                const short cc = col16[ec];
-                prod[e < N ? e : HB_NNZ_CAP] = vv;
-                scol[e < N ? e : HB_NNZ_CAP] = cc;
+            constexpr int U = HB_NNZ_CAP / HB_BLOCK;
+            static_assert(U * HB_BLOCK == HB_NNZ_CAP, "tile capacity");
+            if (MODE == SP_SEED2 && T == 1 && narrow && N > 0) {
+                /* One row per thread.  Values and 16-bit column offsets are
+                 * staged in LDS, then each thread walks its own row: the lanes of
+                 * a wave gather the same stencil slot of consecutive rows, i.e.
+                 * consecutive addresses, where the entry-per-lane order touches
+                 * several cache lines per instruction.  Two gathers per entry
+                 * (r and s), four entries in flight.  The row sum runs in CSR
+                 * order with the products rounded on their own, as in the LDS
+                 * products path below. */
+    #pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int e = tid + u * HB_BLOCK;
+                    const unsigned ec = e0 + (unsigned)min(e, N - 1);
+                    const double vv = vat(ec);
+                    const short cc = col16[ec];
+                    prod[e < N ? e : HB_NNZ_CAP] = vv;
+                    scol[e < N ? e : HB_NNZ_CAP] = cc;
+                }
+                for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
+                __syncthreads();
+                if (row < R) {
+                    constexpr int CH = 4;
+                    const int lo = (int)rp[row], hi = (int)rp[row + 1];
+                    double acc = 0.0;
+                    for (int base = lo; base < hi; base += CH) {
+                        double vv[CH], xv[CH];
+    #pragma unroll
+                        for (int j = 0; j < CH; ++j) {
+                            const int e = min(base + j, hi - 1);
+                            vv[j] = prod[e];
+                            xv[j] = xval(r0 + (unsigned)(int)scol[e]);
+                        }
+    #pragma unroll
+                        for (int j = 0; j < CH; ++j)
+                            if (base + j < hi) acc += vv[j] * xv[j];
+                    }
+                    epilogue(i_me, acc, oa, ob, oc);
+                }
+                return;
+            }
+            if (MODE == SP_SEED2) {
+                /* two gathers per entry (r and s): entries in rounds of four */
+                if (narrow) {
+    #pragma unroll 4
+                    for (int e = tid; e < N; e += HB_BLOCK)
+                        prod[e] = vat(e0 + e) * xval(r0 + (unsigned)(int)col16[e0 + e]);
+                } else {
+    #pragma unroll 4
+                    for (int e = tid; e < N; e += HB_BLOCK) prod[e] = vat(e0 + e) * xval(col[e0 + e]);
+                }
+            } else if (N > 0) {
+                /* A thread owns entries tid + u*HB_BLOCK, u < U.  All of its
+                 * column and value loads are issued together (indices clamped
+                 * into the tile, so no load is predicated), then all of its
+                 * gathers: two memory latencies per tile, not two per entry. */
+                double v[U];
+                unsigned c[U];
+                if (narrow) {
+                    /* 16-bit column offsets from the tile's first row: 2 bytes
+                     * per entry instead of 4 */
+    #pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const unsigned e = e0 + (unsigned)min(tid + u * HB_BLOCK, N - 1);
+                        c[u] = r0 + (unsigned)(int)col16[e];
+                        v[u] = vat(e);
+                    }
+                } else {
+    #pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const unsigned e = e0 + (unsigned)min(tid + u * HB_BLOCK, N - 1);
+                        c[u] = col[e];
+                        v[u] = vat(e);
+                    }
+                }
+                /* entries past the tile end go to the spare slot prod[HB_NNZ_CAP]
+                 * (no branch, so the gathers are not serialised behind one) */
+    #pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int e = tid + u * HB_BLOCK;
+                    prod[e < N ? e : HB_NNZ_CAP] = v[u] * xval(c[u]);
+                }
             }
             for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
             __syncthreads();
             if (row < R) {
-                constexpr int CH = 4;
-                const int lo = (int)rp[row], hi = (int)rp[row + 1];
                 double acc = 0.0;
-                for (int base = lo; base < hi; base += CH) {
-                    double vv[CH], xv[CH];
-#pragma unroll
-                    for (int j = 0; j < CH; ++j) {
-                        const int e = min(base + j, hi - 1);
-                        vv[j] = prod[e];
-                        xv[j] = xval(r0 + (unsigned)(int)scol[e]);
-                    }
-#pragma unroll
-                    for (int j = 0; j < CH; ++j)
-                        if (base + j < hi) acc += vv[j] * xv[j];
-                }
-                epilogue(i_me, acc, oa, ob, oc);
+                const int hi = (int)rp[row + 1];
+                for (int e = (int)rp[row] + lane; e < hi; e += T) acc += prod[e];
+                for (int o = T >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+                if (lane == 0) epilogue(i_me, acc, oa, ob, oc);
             }
-            goto tile_done;
-        }
-        if (MODE == SP_SEED2) {
-            /* two gathers per entry (r and s): entries in rounds of four */
-            if (narrow) {
-#pragma unroll 4
-                for (int e = tid; e < N; e += HB_BLOCK)
-                    prod[e] = val[e0 + e] * xval(r0 + (unsigned)(int)col16[e0 + e]);
-            } else {
-#pragma unroll 4
-                for (int e = tid; e < N; e += HB_BLOCK) prod[e] = val[e0 + e] * xval(col[e0 + e]);
-            }
-        } else if (N > 0) {
-            /* A thread owns entries tid + u*HB_BLOCK, u < U.  All of its
-             * column and value loads are issued together (indices clamped
-             * into the tile, so no load is predicated), then all of its
-             * gathers: two memory latencies per tile, not two per entry. */
-            double v[U];
-            unsigned c[U];
-            if (narrow) {
-                /* 16-bit column offsets from the tile's first row: 2 bytes
-                 * per entry instead of 4 */
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const unsigned e = e0 + (unsigned)min(tid + u * HB_BLOCK, N - 1);
-                    c[u] = r0 + (unsigned)(int)col16[e];
-                    v[u] = val[e];
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const unsigned e = e0 + (unsigned)min(tid + u * HB_BLOCK, N - 1);
-                    c[u] = col[e];
-                    v[u] = val[e];
-                }
-            }
-            /* entries past the tile end go to the spare slot prod[HB_NNZ_CAP]
-             * (no branch, so the gathers are not serialised behind one) */
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int e = tid + u * HB_BLOCK;
-                prod[e < N ? e : HB_NNZ_CAP] = v[u] * xval(c[u]);
-            }
-        }
-        for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
-        __syncthreads();
-        if (row < R) {
-            double acc = 0.0;
-            const int hi = (int)rp[row + 1];
-            for (int e = (int)rp[row] + lane; e < hi; e += T) acc += prod[e];
-            for (int o = T >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-            if (lane == 0) epilogue(i_me, acc, oa, ob, oc);
-        }
-    } else {
-        /* one row longer than the LDS stage */
-        double acc = 0.0;
-        if (EXACT) {
-            if (tid == 0)
-                for (int e = 0; e < N; ++e) acc += val[e0 + e] * xval(col[e0 + e]);
         } else {
-            for (int e = tid; e < N; e += HB_BLOCK) acc += val[e0 + e] * xval(col[e0 + e]);
-            double dummy = 0.0;
-            block_sum2(acc, dummy);
+            /* one row longer than the LDS stage */
+            double acc = 0.0;
+            if (EXACT) {
+                if (tid == 0)
+                    for (int e = 0; e < N; ++e) acc += vat(e0 + e) * xval(col[e0 + e]);
+            } else {
+                for (int e = tid; e < N; e += HB_BLOCK) acc += vat(e0 + e) * xval(col[e0 + e]);
+                double dummy = 0.0;
+                block_sum2(acc, dummy);
+            }
+            if (tid == 0) {
+                double oa = 0.0, ob = 0.0, oc = 0.0;
+                operands(r0, oa, ob, oc);
+                epilogue(r0, acc, oa, ob, oc);
+            }
         }
-        if (tid == 0) {
-            double oa = 0.0, ob = 0.0, oc = 0.0;
-            operands(r0, oa, ob, oc);
-            epilogue(r0, acc, oa, ob, oc);
-        }
+    };
+    if (VI) {
+        if (tid < d.nvdict) vd[tid] = d.vdict[tid];
+        __syncthreads();
     }
+    tile();
 
-tile_done:
     if (MODE == SP_PLAIN) return;
 
     block_sum2(part0, part1);
     if (tid == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
 }
 
-template <int MODE, bool EXACT>
+template <int MODE, bool EXACT, bool VI = false>
 DEV void spmv_tile(const hb_dev &d, unsigned b)
 {
     if (MODE == SP_SEED1)
-        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, d.p_set, d.rring, nullptr,
-                               d.rhat, d.s, nullptr, d.part, 0.0);
+        spmv_body<MODE, EXACT, VI>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, d.p_set, d.rring, nullptr,
+                                   d.rhat, d.s, nullptr, d.part, 0.0);
     else
-        spmv_body<MODE, EXACT>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
-                               d.y, d.qring, d.part, 0.0);
+        spmv_body<MODE, EXACT, VI>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
+                                   d.y, d.qring, d.part, 0.0);
 }
 
-template <int MODE, bool EXACT>
+/* Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups round
+ * robin over the eight XCDs, each with its own L2).  Tile order: XCD x gets
+ * the contiguous tiles [start(x), start(x+1)), so a tile's column halo (the
+ * rows around it, and the rows a plane away for stencils) was mostly
+ * fetched into the same L2 by its neighbours.  A bijection on 0 .. nb-1. */
+DEV unsigned xcd_tile(unsigned b, unsigned nb)
+{
+    const unsigned q = nb / 8, r = nb % 8, x = b % 8;
+    return x * q + min(x, r) + b / 8;
+}
+
+template <int MODE, bool EXACT, bool VI>
 __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
 {
-    spmv_tile<MODE, EXACT>(d, blockIdx.x);
+    spmv_tile<MODE, EXACT, VI>(d, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 /* p[0 .. count) = v (a fill kernel: usable where a memset cannot express v) */
@@ -1457,8 +1484,9 @@ template <int MODE, bool EXACT>
 __global__ __launch_bounds__(HB_BLOCK) void kb_spmv(const hb_dev *__restrict__ devs,
                                                     const int *__restrict__ tab)
 {
-    const hb_dev &d = devs[tab[2 * blockIdx.x]];
-    spmv_tile<MODE, EXACT>(d, tab[2 * blockIdx.x + 1]);
+    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    const hb_dev &d = devs[tab[2 * t]];
+    spmv_tile<MODE, EXACT>(d, tab[2 * t + 1]);
 }
 
 template <int MODE, bool EXACT>
@@ -1581,16 +1609,26 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
     const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK), one(1);
     switch (stage) {
     case HB_STAGE_SPMV1:
-        if (d->exact) hipLaunchKernelGGL((k_spmv<SP_SEED1, true>), gs, bl, 0, s, *d);
-        else hipLaunchKernelGGL((k_spmv<SP_SEED1, false>), gs, bl, 0, s, *d);
+        if (d->exact) {
+            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED1, true, true>), gs, bl, 0, s, *d);
+            else hipLaunchKernelGGL((k_spmv<SP_SEED1, true, false>), gs, bl, 0, s, *d);
+        } else {
+            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED1, false, true>), gs, bl, 0, s, *d);
+            else hipLaunchKernelGGL((k_spmv<SP_SEED1, false, false>), gs, bl, 0, s, *d);
+        }
         break;
     case HB_STAGE_FIN1:
         if (d->exact) hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, true>), one, bl, 0, s, *d);
         else hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), one, bl, 0, s, *d);
         break;
     case HB_STAGE_SPMV2:
-        if (d->exact) hipLaunchKernelGGL((k_spmv<SP_SEED2, true>), gs, bl, 0, s, *d);
-        else hipLaunchKernelGGL((k_spmv<SP_SEED2, false>), gs, bl, 0, s, *d);
+        if (d->exact) {
+            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, true, true>), gs, bl, 0, s, *d);
+            else hipLaunchKernelGGL((k_spmv<SP_SEED2, true, false>), gs, bl, 0, s, *d);
+        } else {
+            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true>), gs, bl, 0, s, *d);
+            else hipLaunchKernelGGL((k_spmv<SP_SEED2, false, false>), gs, bl, 0, s, *d);
+        }
         break;
     case HB_STAGE_FIN2:
         if (d->exact) hipLaunchKernelGGL((k_fin_spmv<SP_SEED2, true>), one, bl, 0, s, *d);

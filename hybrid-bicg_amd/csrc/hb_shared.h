@@ -18,6 +18,7 @@ extern "C" {
 #define HB_WBATCH       64    /* largest window of the batched solver          */
 #define HB_WMAX_EXACT   16    /* largest window of the exact-order replay      */
 #define HB_UG           16    /* shifts per chunk of the combined-coefficient flush */
+#define HB_VDICT        256   /* largest value table of the value-indexed SpMV */
 #define HB_CC_ROWS(W)   (2 * (W) + 5) /* rows of one chunk's coefficient block  */
 
 /* Error bits in hb_state.status */
@@ -89,6 +90,9 @@ typedef struct hb_dev {
     HB_G const unsigned *blk;  /* SpMV workgroup row starts [nblk+1]       */
     HB_G const short    *col16;/* [nz] col - (first row of the entry's tile), where it fits */
     HB_G const unsigned *cflag;      /* [nblk] 1: the tile's col16 entries are exact */
+    HB_G const unsigned char *vidx;  /* [nz] index into vdict, or NULL: val is read */
+    HB_G const double   *vdict;      /* [nvdict <= HB_VDICT] the matrix's distinct values */
+    int nvdict;
     HB_G const double   *sigma;/* [M]                                      */
     HB_G const double   *rhat; /* [n]   shadow residual r^ = b             */
     HB_G double *rring;        /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */

@@ -51,6 +51,9 @@ typedef struct hbicg_params {
                             p exceeds this multiple of one iteration's own terms (0: off;
                             the regrouped coefficients are exact to double precision)    */
     int    trace;        /* 1: record the per-iteration residual estimate of every shift  */
+    int    value_index;  /* 1 (default): a matrix with at most 256 distinct values is read
+                            by the SpMV as one byte per entry into a table of its values
+                            (lossless; results unchanged); 0: always 8-byte values     */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -65,6 +68,7 @@ typedef struct hbicg_report {
     double t_solve_ms;   /* device time of init + iteration loop              */
     double t_total_ms;   /* wall time of the whole call                        */
     int    window;       /* shift-update window the solve ran with             */
+    int    value_table;  /* distinct values the SpMV read through its table (0: none) */
 } hbicg_report;
 
 void hbicg_default_params(hbicg_params *p);

@@ -226,3 +226,29 @@ def test_sweep_driver_conventions_1024_shifts(c0):
         print(variant, "true residual max", tr.max(), "oracle", tro.max(), flush=True)
         assert int((tr > 10 * c0.tol).sum()) <= n_above
         assert tr.max() <= 10 * max(tro.max(), 10 * c0.tol)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_value_table_bitwise(c0, exact):
+    """A matrix with few distinct values is read through a one-byte value
+    table; the products are the same doubles, so every result is bitwise
+    equal to reading the 8-byte values (value_index=0)."""
+    mk = lambda vi: make_params(tol=c0.tol, exact_order=exact, value_index=vi)
+    a = dropin_solve(c0.A, c0.b, c0.sigma, 8, "dynamic", mk(True))
+    b = dropin_solve(c0.A, c0.b, c0.sigma, 8, "dynamic", mk(False))
+    assert a.report["value_table"] == 3 and b.report["value_table"] == 0
+    assert a.k == b.k and a.switches == b.switches
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.r, b.r)
+
+
+def test_value_table_not_used_for_many_values():
+    """More than 256 distinct values: the SpMV reads the 8-byte values."""
+    rng = np.random.default_rng(3)
+    n = 600
+    A = P.conv_diff_2d(20, 30)
+    A = P.CSR(ptr=A.ptr, col=A.col, val=A.val + 1e-3 * rng.random(len(A.val)), n=A.n)
+    b = rng.standard_normal(A.n)
+    got = dropin_solve(A, b, np.array([0.0, 0.1]), 0, "dynamic", make_params(tol=1e-10))
+    assert got.report["value_table"] == 0
+    ref = O.solve(A, b, np.array([0.0, 0.1]), 0, tol=1e-10)
+    assert abs(got.k - ref.k) <= 2
