@@ -42,7 +42,7 @@ void hbicg_default_params(hbicg_params *p)
     p->profile = 0;
     p->growth_limit = 0.0;
     p->trace = 0;
-    p->value_index = 1;
+    p->value_index = 0;
 }
 
 static const hbicg_params *params(void)

@@ -51,9 +51,10 @@ typedef struct hbicg_params {
                             p exceeds this multiple of one iteration's own terms (0: off;
                             the regrouped coefficients are exact to double precision)    */
     int    trace;        /* 1: record the per-iteration residual estimate of every shift  */
-    int    value_index;  /* 1 (default): a matrix with at most 256 distinct values is read
-                            by the SpMV as one byte per entry into a table of its values
-                            (lossless; results unchanged); 0: always 8-byte values     */
+    int    value_index;  /* 1: a matrix with at most 256 distinct values is read by the
+                            SpMV as one byte per entry into a table of its values (lossless;
+                            results unchanged; 5% faster solve at 128^3, the SpMV then no
+                            longer bandwidth-bound); 0 (default): 8-byte values           */
 } hbicg_params;
 
 typedef struct hbicg_report {
