@@ -1484,7 +1484,10 @@ template <int MODE, bool EXACT>
 __global__ __launch_bounds__(HB_BLOCK) void kb_spmv(const hb_dev *__restrict__ devs,
                                                     const int *__restrict__ tab)
 {
-    const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+    /* hardware order: problems finish at different iterations, and the
+     * tiles still working must stay spread over all XCDs (an XCD-contiguous
+     * order measured 14% slower on the 256-problem batch) */
+    const unsigned t = blockIdx.x;
     const hb_dev &d = devs[tab[2 * t]];
     spmv_tile<MODE, EXACT>(d, tab[2 * t + 1]);
 }
