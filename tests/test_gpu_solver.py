@@ -252,3 +252,27 @@ def test_value_table_not_used_for_many_values():
     assert got.report["value_table"] == 0
     ref = O.solve(A, b, np.array([0.0, 0.1]), 0, tol=1e-10)
     assert abs(got.k - ref.k) <= 2
+
+
+@pytest.mark.parametrize("M,seed", [(1, 0), (17, 16), (33, 5)])
+def test_ragged_sizes(M, seed):
+    """n = 1517 (not a multiple of 64: partial tiles in every kernel) and
+    shift counts 1, 17 and 33 (partial chunks of 16 in the flush): exact
+    order equals the oracle bit for bit; the default order meets the parity
+    bar, with every shift's true residual recomputed."""
+    A = P.conv_diff_2d(37, 41, diag=4.01, conv=0.3)
+    rng = np.random.default_rng(M)
+    b = rng.uniform(-1.0, 1.0, A.n)
+    sigma = np.logspace(-2, 0, M) if M > 1 else np.array([0.05])
+    print(f"case ragged n={A.n} M={M} seed={seed}", flush=True)
+    ref = O.solve(A, b, sigma, seed, tol=1e-10)
+    ex = dropin_solve(A, b, sigma, seed, "dynamic", make_params(tol=1e-10, exact_order=True))
+    assert ex.k == ref.k and ex.switches == ref.switches
+    assert np.array_equal(ex.x, ref.x) and np.array_equal(ex.r, ref.r)
+    for variant in ("normal", "static", "dynamic"):
+        got = dropin_solve(A, b, sigma, seed, variant, make_params(tol=1e-10))
+        assert abs(got.k - ref.k) <= 2 and len(got.switches) == len(ref.switches)
+        rel = np.linalg.norm(got.x - ref.x, axis=1) / np.linalg.norm(ref.x, axis=1)
+        assert rel.max() <= 1e-9, (variant, rel.max())
+        tr = O.true_residuals(A, got.x, b, sigma)
+        assert tr.max() <= 10 * 1e-10, (variant, tr.max())
