@@ -1132,9 +1132,9 @@ __global__ __launch_bounds__(HB_BLOCK) void k_flush_coef(hb_dev d)
  * same time, so each g_k comes from HBM once and from cache for the other
  * chunks.  The chunk's coefficients are uniform across the wavefront and
  * arrive through scalar loads; the wavefront index goes through
- * readfirstlane so the compiler knows it is uniform.  XONLY (the solve's
- * last flush): p_j is neither formed nor written, halving the arithmetic. */
-template <int UG, bool XONLY>
+ * readfirstlane so the compiler knows it is uniform.  The solve's last
+ * flush, which needs x_j only, is flush_apply_x2 below. */
+template <int UG>
 DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double *__restrict__ pset,
                      const double *__restrict__ rring, const double *__restrict__ qring,
                      const double *__restrict__ ccoef, const int *__restrict__ flist)
@@ -1142,7 +1142,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
     const hb_state *st = d.st;
     const int n = d.n, W = d.W;
     const int k0 = st->flush_k0, cnt = st->flush_cnt;
-    const int Lc = st->flush_len - st->flush_tail, tail = XONLY ? 0 : st->flush_tail;
+    const int Lc = st->flush_len - st->flush_tail, tail = st->flush_tail;
     const int sp = st->win_seedpos;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1171,7 +1171,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 x[u] = __builtin_fma(cb[u], p[u], x[u]);
-                if (!XONLY) p[u] = cb[UG + u] * p[u];
+                p[u] = cb[UG + u] * p[u];
             }
             int rs = (k0 - 1) % (W + 1), qs = k0 % W;
             double gr = rring[off(rs, n) + i];
@@ -1180,7 +1180,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[u], gr, x[u]);
-                    if (!XONLY) p[u] = __builtin_fma(ck[UG + u], gr, p[u]);
+                    p[u] = __builtin_fma(ck[UG + u], gr, p[u]);
                 }
             }
             for (int b = 0; b < Lc; ++b) {
@@ -1192,12 +1192,12 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[u], gq, x[u]);
-                    if (!XONLY) p[u] = __builtin_fma(ck[UG + u], gq, p[u]);
+                    p[u] = __builtin_fma(ck[UG + u], gq, p[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[2 * UG + u], gr, x[u]);
-                    if (!XONLY) p[u] = __builtin_fma(ck[3 * UG + u], gr, p[u]);
+                    p[u] = __builtin_fma(ck[3 * UG + u], gr, p[u]);
                 }
             }
             if (tail) {
@@ -1224,7 +1224,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
                     __builtin_nontemporal_store(x[u], xset + off(j, n) + i);
                     /* the seed keeps its own p_s (the regrouped one is
                      * only its x's companion) */
-                    if (!XONLY && c * UG + u != sp) __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
+                    if (c * UG + u != sp) __builtin_nontemporal_store(p[u], pset + off(j, n) + i);
                 }
             }
         }
@@ -1335,7 +1335,7 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
     }
     if (!st->flush) return;
     if (st->flush_xonly) flush_apply_x2<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
-    else flush_apply<UG, false>(d, bx, xset, pset, rring, qring, ccoef, flist);
+    else flush_apply<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
 }
 
 DEV void shift_cc_tile(const hb_dev &d, int bx)
