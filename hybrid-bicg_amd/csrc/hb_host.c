@@ -105,8 +105,10 @@ struct hbicg_plan {
     unsigned *cflag;
     double col_bytes;    /* column-index bytes per SpMV (16-bit tiles count 2 per entry) */
     double val_bytes;    /* value bytes per SpMV (8 per entry, 1 through a value table) */
-    unsigned char *vidx; /* value table of the SpMV, when the matrix has one */
-    double *vdict;
+    unsigned char *vidx; /* value table of the SpMV, when the matrix has one, with */
+    double *vdict;       /* col16 in the same tile-padded layout (col16p, tpad) */
+    short *col16p;
+    unsigned *tpad;
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
@@ -176,7 +178,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -450,19 +452,60 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
         } else {
             nd = build_value_table(A->val, nzs, dict, hidx);
         }
+        unsigned *htp = NULL;
+        unsigned char *hvp = NULL;
+        short *hc16 = NULL, *hcp = NULL;
         if (nd > 0) {
-            pl->vidx = (unsigned char *)dalloc(nzs, &rc);
-            pl->vdict = (double *)dalloc(HB_VDICT * sizeof(double), &rc);
-            if (rc == 0 && (hipMemcpy(pl->vidx, hidx, nzs, hipMemcpyHostToDevice) != hipSuccess ||
+            /* tile-padded layout: tile t's entries start at tpad[t], a
+             * multiple of 8, so the kernel fetches 8 indices and 8 columns
+             * per load; the gaps are zero */
+            htp = (unsigned *)malloc(((size_t)pl->nblk + 1) * 4);
+            size_t P = 0;
+            if (htp) {
+                for (int t = 0; t < pl->nblk; ++t) {
+                    htp[t] = (unsigned)P;
+                    P += ((size_t)(hptr[hblk[t + 1]] - hptr[hblk[t]]) + 7) & ~(size_t)7;
+                }
+                htp[pl->nblk] = (unsigned)P;
+            }
+            if (!htp) rc = HBICG_E_NOMEM;
+            else if (P > 0xffffffffu) rc = HBICG_E_ARG;
+            if (rc == 0) {
+                hvp = (unsigned char *)calloc(P, 1);
+                hcp = (short *)calloc(P, 2);
+                hc16 = (short *)malloc(nzs * 2);
+                if (!hvp || !hcp || !hc16) rc = HBICG_E_NOMEM;
+            }
+            if (rc == 0 && hipMemcpy(hc16, pl->col16, nzs * 2, hipMemcpyDeviceToHost) != hipSuccess) rc = HBICG_E_HIP;
+            if (rc == 0) {
+                for (int t = 0; t < pl->nblk; ++t) {
+                    const size_t a = hptr[hblk[t]], len = hptr[hblk[t + 1]] - a;
+                    memcpy(hvp + htp[t], hidx + a, len);
+                    memcpy(hcp + htp[t], hc16 + a, len * 2);
+                }
+                pl->vidx = (unsigned char *)dalloc(P, &rc);
+                pl->col16p = (short *)dalloc(P * 2, &rc);
+                pl->tpad = (unsigned *)dalloc(((size_t)pl->nblk + 1) * 4, &rc);
+                pl->vdict = (double *)dalloc(HB_VDICT * sizeof(double), &rc);
+            }
+            if (rc == 0 && (hipMemcpy(pl->vidx, hvp, P, hipMemcpyHostToDevice) != hipSuccess ||
+                            hipMemcpy(pl->col16p, hcp, P * 2, hipMemcpyHostToDevice) != hipSuccess ||
+                            hipMemcpy(pl->tpad, htp, ((size_t)pl->nblk + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
                             hipMemcpy(pl->vdict, dict, (size_t)nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
                 rc = HBICG_E_HIP;
             if (rc == 0) {
                 d->vidx = pl->vidx;
+                d->col16p = pl->col16p;
+                d->tpad = pl->tpad;
                 d->vdict = pl->vdict;
                 d->nvdict = nd;
                 pl->val_bytes = (double)nzs;
             }
         }
+        free(htp);
+        free(hvp);
+        free(hcp);
+        free(hc16);
         free(hidx);
         free(tmp);
         if (rc) goto fail;

@@ -148,8 +148,9 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const double *__restrict__ rhat, double *__restrict__ out1,
                    double *__restrict__ qring, double *__restrict__ part, double shift)
 {
-    __shared__ double prod[HB_NNZ_CAP + 1];
-    __shared__ short scol[HB_NNZ_CAP + 1];
+    __shared__ double prod[VI ? 1 : HB_NNZ_CAP + 1];
+    __shared__ __attribute__((aligned(16))) short scol[HB_NNZ_CAP + 8];
+    __shared__ __attribute__((aligned(16))) unsigned char sidx[VI ? HB_NNZ_CAP + 8 : 1];
     __shared__ unsigned rp[HB_ROWS_CAP + 1];
     const int n = d.n;
     int switched = 0;
@@ -235,12 +236,85 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
      * distinct values (at most HB_VDICT, staged in LDS) instead of eight
      * bytes per entry; the products are the same doubles */
     __shared__ double vd[VI ? HB_VDICT : 1];
+    /* VI layout: tile b's indices and 16-bit columns start at tpad[b] (a
+     * multiple of 8), so a thread fetches 8 entries of each with one 8- and
+     * one 16-byte load.  Stencil tiles (one row per thread) stage them in LDS
+     * and walk their rows there; other tiles read them where they lie, T
+     * lanes per row.  Products and their summation order are those of the
+     * 8-byte path, so results are bitwise the same. */
+    auto tile_vi = [&]() {
+        const unsigned char *__restrict__ vidxp = d.vidx;
+        const short *__restrict__ col16p = d.col16p;
+        const unsigned pb = d.tpad[b];
+        if (N <= HB_NNZ_CAP) {
+            int T = 1;
+            if (!EXACT) {
+                const int q = HB_BLOCK / R;
+                while ((T << 1) <= q && (T << 1) <= 64) T <<= 1;
+            }
+            const int row = tid / T, lane = tid & (T - 1);
+            const unsigned i_me = r0 + (unsigned)min(row, R - 1);
+            double oa = 0.0, ob = 0.0, oc = 0.0;
+            operands(i_me, oa, ob, oc);
+            if (T == 1 && narrow) {
+                if (8 * tid < N) {
+                    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+                    typedef int i4 __attribute__((ext_vector_type(4)));
+                    const u2 v8 = *(const u2 *)(vidxp + pb + 8 * tid);
+                    const i4 c8 = *(const i4 *)(col16p + pb + 8 * tid);
+                    *(u2 *)(sidx + 8 * tid) = v8;
+                    *(i4 *)(scol + 8 * tid) = c8;
+                }
+                for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
+                __syncthreads();
+                if (row < R) {
+                    constexpr int CH = 4;
+                    const int lo = (int)rp[row], hi = (int)rp[row + 1];
+                    double acc = 0.0;
+                    for (int base = lo; base < hi; base += CH) {
+                        double vv[CH], xv[CH];
+#pragma unroll
+                        for (int j = 0; j < CH; ++j) {
+                            const int e = min(base + j, hi - 1);
+                            vv[j] = vd[sidx[e]];
+                            xv[j] = xval(r0 + (unsigned)(int)scol[e]);
+                        }
+#pragma unroll
+                        for (int j = 0; j < CH; ++j)
+                            if (base + j < hi) acc += vv[j] * xv[j];
+                    }
+                    epilogue(i_me, acc, oa, ob, oc);
+                }
+            } else if (row < R) {
+                const int lo = (int)(ptr[i_me] - e0), hi = (int)(ptr[i_me + 1] - e0);
+                double acc = 0.0;
+                for (int e = lo + lane; e < hi; e += T) {
+                    const unsigned cc = narrow ? r0 + (unsigned)(int)col16p[pb + e] : col[e0 + e];
+                    acc += vd[vidxp[pb + e]] * xval(cc);
+                }
+                for (int o = T >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+                if (lane == 0) epilogue(i_me, acc, oa, ob, oc);
+            }
+        } else {
+            /* one row longer than the LDS stage */
+            double acc = 0.0;
+            if (EXACT) {
+                if (tid == 0)
+                    for (int e = 0; e < N; ++e) acc += vd[vidxp[pb + e]] * xval(col[e0 + e]);
+            } else {
+                for (int e = tid; e < N; e += HB_BLOCK) acc += vd[vidxp[pb + e]] * xval(col[e0 + e]);
+                double dummy = 0.0;
+                block_sum2(acc, dummy);
+            }
+            if (tid == 0) {
+                double oa = 0.0, ob = 0.0, oc = 0.0;
+                operands(r0, oa, ob, oc);
+                epilogue(r0, acc, oa, ob, oc);
+            }
+        }
+    };
     auto tile = [&]() {
-        const unsigned char *__restrict__ vidx = d.vidx;
-        auto vat = [&](unsigned e) -> double {
-            if constexpr (VI) return vd[vidx[e]];
-            else return val[e];
-        };
+        auto vat = [&](unsigned e) -> double { return val[e]; };
         if (N <= HB_NNZ_CAP) {
             /* rows reduced by T lanes each, T the largest power of two with
              * T*R <= HB_BLOCK (at most 64; 1 in exact order): every row has one
@@ -366,11 +440,13 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             }
         }
     };
-    if (VI) {
+    if constexpr (VI) {
         if (tid < d.nvdict) vd[tid] = d.vdict[tid];
         __syncthreads();
+        tile_vi();
+    } else {
+        tile();
     }
-    tile();
 
     if (MODE == SP_PLAIN) return;
 

@@ -90,7 +90,10 @@ typedef struct hb_dev {
     HB_G const unsigned *blk;  /* SpMV workgroup row starts [nblk+1]       */
     HB_G const short    *col16;/* [nz] col - (first row of the entry's tile), where it fits */
     HB_G const unsigned *cflag;      /* [nblk] 1: the tile's col16 entries are exact */
-    HB_G const unsigned char *vidx;  /* [nz] index into vdict, or NULL: val is read */
+    HB_G const unsigned char *vidx;  /* value table indices, tile-padded (tpad), or NULL */
+    HB_G const short    *col16p;     /* col16 in the same tile-padded layout (with vidx) */
+    HB_G const unsigned *tpad;       /* [nblk+1] tile b's entries start at tpad[b], a
+                                        multiple of 8, in vidx and col16p            */
     HB_G const double   *vdict;      /* [nvdict <= HB_VDICT] the matrix's distinct values */
     int nvdict;
     HB_G const double   *sigma;/* [M]                                      */
