@@ -105,10 +105,15 @@ struct hbicg_plan {
     unsigned *cflag;
     double col_bytes;    /* column-index bytes per SpMV (16-bit tiles count 2 per entry) */
     double val_bytes;    /* value bytes per SpMV (8 per entry, 1 through a value table) */
+    double ptr_bytes;    /* row-pointer bytes per SpMV (diagonal tiles read none)    */
     unsigned char *vidx; /* value table of the SpMV, when the matrix has one, with */
     double *vdict;       /* col16 in the same tile-padded layout (col16p, tpad) */
     short *col16p;
     unsigned *tpad;
+    unsigned *dtile;     /* diagonal tiles of the value-table SpMV: records, offsets, */
+    int *doff;           /* index bytes                                               */
+    unsigned char *didx;
+    int n_diag_tiles;
     int own_matrix, nblk;
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
@@ -178,7 +183,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -244,6 +249,194 @@ static int build_value_table(const double *val, size_t nz, double *dict, unsigne
         idx[e] = (unsigned char)pos[h];
     }
     return nd;
+}
+
+/* Diagonals of tile [r0, r1): a tile of more than HB_BLOCK/2 rows whose rows
+ * list their columns in strictly ascending order and whose entries lie on at
+ * most HB_DIAG_K diagonals (column - row) is stored by diagonals.  off gets
+ * the diagonals in ascending order; returns their number, or 0 where the
+ * tile does not qualify.  A row's entries then come in the order of the
+ * diagonals, which is the row's CSR order. */
+static int tile_diagonals(const unsigned *ptr, const unsigned *col, unsigned r0, unsigned r1, int *off)
+{
+    int k = 0;
+    if (r1 - r0 <= HB_BLOCK / 2 || ptr[r1] == ptr[r0]) return 0;
+    for (unsigned i = r0; i < r1; ++i) {
+        for (unsigned e = ptr[i]; e < ptr[i + 1]; ++e) {
+            if (e > ptr[i] && col[e] <= col[e - 1]) return 0;
+            const long long o = (long long)col[e] - (long long)i;
+            int s = 0;
+            while (s < k && off[s] < o) ++s;
+            if (s < k && off[s] == o) continue;
+            if (k == HB_DIAG_K) return 0;
+            for (int t = k; t > s; --t) off[t] = off[t - 1];
+            off[s] = (int)o;
+            ++k;
+        }
+    }
+    return k;
+}
+
+/* Matrix layouts of the value-table SpMV.  A matrix with at most HB_VDICT
+ * distinct values (device matrices: a prefix is checked first) is read
+ * through a table of them, one index byte per entry.  Tiles that qualify
+ * (tile_diagonals, and at most HB_VDICT - 1 values, HB_DIAG_NONE marking a
+ * row's missing diagonal) are stored by diagonals: row i of tile t has its
+ * KP = 8 or 16 index bytes at dtile[2t] + KP*i, K = dtile[2t+1] > 0 diagonals
+ * at doff[HB_DIAG_K*t ...]; the kernel reads no columns and no row pointers
+ * for them, unless KP*rows exceeds the 3 bytes per entry of the CSR form.
+ * The other tiles keep CSR order, their index bytes and 16-bit columns in a
+ * tile-padded copy: tile t at tpad[t], a multiple of 8.  Leaves the plan
+ * unchanged where the matrix has more values. */
+static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_device, const unsigned *hptr,
+                              const unsigned *hblk)
+{
+    const size_t nzs = pl->nz;
+    const int nb = pl->nblk;
+    hb_dev *d = &pl->dev;
+    double dict[HB_VDICT];
+    int rc = 0, nd = 0;
+    double *tmp = NULL;
+    unsigned *tcol = NULL, *htp = NULL, *hdt = NULL, *hf = NULL;
+    int *hoff = NULL;
+    unsigned char *hidx = (unsigned char *)malloc(nzs), *hvp = NULL, *hdi = NULL;
+    short *hc16 = NULL, *hcp = NULL;
+    if (!hidx) return HBICG_E_NOMEM;
+    if (a_on_device) {
+        const size_t pre = nzs < 65536 ? nzs : 65536;
+        tmp = (double *)malloc(nzs * sizeof(double));
+        if (!tmp) { rc = HBICG_E_NOMEM; goto out; }
+        if (hipMemcpy(tmp, A->val, pre * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = HBICG_E_HIP; goto out; }
+        if (build_value_table(tmp, pre, dict, hidx) > 0) {
+            if (pre < nzs && hipMemcpy(tmp + pre, A->val + pre, (nzs - pre) * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = HBICG_E_HIP;
+                goto out;
+            }
+            nd = build_value_table(tmp, nzs, dict, hidx);
+        }
+    } else {
+        nd = build_value_table(A->val, nzs, dict, hidx);
+    }
+    if (nd == 0) goto out;
+
+    const unsigned *hcol = A->col;
+    if (a_on_device) {
+        tcol = (unsigned *)malloc(nzs * 4);
+        if (!tcol) { rc = HBICG_E_NOMEM; goto out; }
+        if (hipMemcpy(tcol, A->col, nzs * 4, hipMemcpyDeviceToHost) != hipSuccess) { rc = HBICG_E_HIP; goto out; }
+        hcol = tcol;
+    }
+    htp = (unsigned *)malloc(((size_t)nb + 1) * 4);
+    hdt = (unsigned *)calloc((size_t)nb * 2, 4);
+    hoff = (int *)calloc((size_t)nb * HB_DIAG_K, sizeof(int));
+    hf = (unsigned *)malloc((size_t)nb * 4);
+    hc16 = (short *)malloc(nzs * 2);
+    if (!htp || !hdt || !hoff || !hf || !hc16) { rc = HBICG_E_NOMEM; goto out; }
+    if (hipMemcpy(hc16, pl->col16, nzs * 2, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(hf, pl->cflag, (size_t)nb * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = HBICG_E_HIP;
+        goto out;
+    }
+    size_t P = 0, D = 0;
+    double vb = 0.0, cb = 0.0, pb = 0.0;
+    int ndiag = 0;
+    for (int t = 0; t < nb; ++t) {
+        const unsigned r0 = hblk[t], r1 = hblk[t + 1];
+        const size_t R = r1 - r0, N = hptr[r1] - hptr[r0];
+        int k = nd < HB_VDICT ? tile_diagonals(hptr, hcol, r0, r1, hoff + (size_t)HB_DIAG_K * t) : 0;
+        const size_t KP = k <= 8 ? 8 : 16;
+        if (k > 0 && KP * R > 3 * N) k = 0;
+        htp[t] = (unsigned)P;
+        if (k > 0) {
+            D = (D + 15) & ~(size_t)15;
+            hdt[2 * t] = (unsigned)D;
+            hdt[2 * t + 1] = (unsigned)k;
+            D += KP * R;
+            vb += (double)(KP * R) + 4.0 * k + 8.0;
+            pb += 8.0;
+            ++ndiag;
+        } else {
+            memset(hoff + (size_t)HB_DIAG_K * t, 0, HB_DIAG_K * sizeof(int));
+            P += (N + 7) & ~(size_t)7;
+            vb += (double)N;
+            cb += (hf[t] ? 2.0 : 4.0) * (double)N;
+            pb += 4.0 * ((double)R + 1.0);
+        }
+        if (P > 0xffffffffu || D > 0xffffffffu) { rc = HBICG_E_ARG; goto out; }
+    }
+    htp[nb] = (unsigned)P;
+    hvp = (unsigned char *)calloc(P ? P : 1, 1);
+    hcp = (short *)calloc(P ? P : 1, 2);
+    hdi = (unsigned char *)malloc(D ? D : 1);
+    if (!hvp || !hcp || !hdi) { rc = HBICG_E_NOMEM; goto out; }
+    memset(hdi, HB_DIAG_NONE, D ? D : 1);
+    for (int t = 0; t < nb; ++t) {
+        const unsigned r0 = hblk[t], r1 = hblk[t + 1];
+        if (hdt[2 * t + 1]) {
+            const int *off = hoff + (size_t)HB_DIAG_K * t;
+            const size_t KP = hdt[2 * t + 1] <= 8 ? 8 : 16;
+            for (unsigned i = r0; i < r1; ++i) {
+                unsigned char *dst = hdi + hdt[2 * t] + KP * (i - r0);
+                int s = 0;
+                for (unsigned e = hptr[i]; e < hptr[i + 1]; ++e, ++s) {
+                    const long long o = (long long)hcol[e] - (long long)i;
+                    while (off[s] != o) ++s;
+                    dst[s] = hidx[e];
+                }
+            }
+        } else {
+            const size_t a = hptr[r0], len = hptr[r1] - a;
+            memcpy(hvp + htp[t], hidx + a, len);
+            memcpy(hcp + htp[t], hc16 + a, len * 2);
+        }
+    }
+    pl->vidx = (unsigned char *)dalloc(P, &rc);
+    pl->col16p = (short *)dalloc(P * 2, &rc);
+    pl->tpad = (unsigned *)dalloc(((size_t)nb + 1) * 4, &rc);
+    pl->vdict = (double *)dalloc(HB_VDICT * sizeof(double), &rc);
+    if (ndiag) {
+        pl->dtile = (unsigned *)dalloc((size_t)nb * 8, &rc);
+        pl->doff = (int *)dalloc((size_t)nb * HB_DIAG_K * sizeof(int), &rc);
+        pl->didx = (unsigned char *)dalloc(D, &rc);
+    }
+    if (rc) goto out;
+    if (hipMemcpy(pl->vidx, hvp, P, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->col16p, hcp, P * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->tpad, htp, ((size_t)nb + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->vdict, dict, (size_t)nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        (ndiag && (hipMemcpy(pl->dtile, hdt, (size_t)nb * 8, hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(pl->doff, hoff, (size_t)nb * HB_DIAG_K * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(pl->didx, hdi, D, hipMemcpyHostToDevice) != hipSuccess))) {
+        rc = HBICG_E_HIP;
+        goto out;
+    }
+    d->vidx = pl->vidx;
+    d->col16p = pl->col16p;
+    d->tpad = pl->tpad;
+    d->vdict = pl->vdict;
+    d->nvdict = nd;
+    if (ndiag) {
+        d->dtile = pl->dtile;
+        d->doff = pl->doff;
+        d->didx = pl->didx;
+    }
+    pl->n_diag_tiles = ndiag;
+    pl->val_bytes = vb;
+    pl->col_bytes = cb + 4.0 * (double)nb;
+    pl->ptr_bytes = pb;
+out:
+    free(tmp);
+    free(tcol);
+    free(htp);
+    free(hdt);
+    free(hf);
+    free(hoff);
+    free(hidx);
+    free(hvp);
+    free(hdi);
+    free(hc16);
+    free(hcp);
+    return rc;
 }
 
 /* w_alloc: ring storage for windows up to this (the batched solver's
@@ -422,94 +615,9 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
         free(hf);
     }
 
-    /* value table: a matrix with at most HB_VDICT distinct values is read
-     * as one byte per entry (device matrices: a prefix is checked first) */
     pl->val_bytes = 8.0 * (double)pl->nz;
-    if (allow_vi && pl->nz > 0) {
-        const size_t nzs = pl->nz;
-        double *tmp = NULL, dict[HB_VDICT];
-        unsigned char *hidx = (unsigned char *)malloc(nzs);
-        int nd = 0;
-        if (!hidx) { rc = HBICG_E_NOMEM; goto fail; }
-        if (a_on_device) {
-            const size_t pre = nzs < 65536 ? nzs : 65536;
-            tmp = (double *)malloc(nzs * sizeof(double));
-            if (!tmp || hipMemcpy(tmp, A->val, pre * 8, hipMemcpyDeviceToHost) != hipSuccess) {
-                free(hidx);
-                free(tmp);
-                rc = tmp ? HBICG_E_HIP : HBICG_E_NOMEM;
-                goto fail;
-            }
-            if (build_value_table(tmp, pre, dict, hidx) > 0) {
-                if (pre < nzs && hipMemcpy(tmp + pre, A->val + pre, (nzs - pre) * 8, hipMemcpyDeviceToHost) != hipSuccess) {
-                    free(hidx);
-                    free(tmp);
-                    rc = HBICG_E_HIP;
-                    goto fail;
-                }
-                nd = build_value_table(tmp, nzs, dict, hidx);
-            }
-        } else {
-            nd = build_value_table(A->val, nzs, dict, hidx);
-        }
-        unsigned *htp = NULL;
-        unsigned char *hvp = NULL;
-        short *hc16 = NULL, *hcp = NULL;
-        if (nd > 0) {
-            /* tile-padded layout: tile t's entries start at tpad[t], a
-             * multiple of 8, so the kernel fetches 8 indices and 8 columns
-             * per load; the gaps are zero */
-            htp = (unsigned *)malloc(((size_t)pl->nblk + 1) * 4);
-            size_t P = 0;
-            if (htp) {
-                for (int t = 0; t < pl->nblk; ++t) {
-                    htp[t] = (unsigned)P;
-                    P += ((size_t)(hptr[hblk[t + 1]] - hptr[hblk[t]]) + 7) & ~(size_t)7;
-                }
-                htp[pl->nblk] = (unsigned)P;
-            }
-            if (!htp) rc = HBICG_E_NOMEM;
-            else if (P > 0xffffffffu) rc = HBICG_E_ARG;
-            if (rc == 0) {
-                hvp = (unsigned char *)calloc(P, 1);
-                hcp = (short *)calloc(P, 2);
-                hc16 = (short *)malloc(nzs * 2);
-                if (!hvp || !hcp || !hc16) rc = HBICG_E_NOMEM;
-            }
-            if (rc == 0 && hipMemcpy(hc16, pl->col16, nzs * 2, hipMemcpyDeviceToHost) != hipSuccess) rc = HBICG_E_HIP;
-            if (rc == 0) {
-                for (int t = 0; t < pl->nblk; ++t) {
-                    const size_t a = hptr[hblk[t]], len = hptr[hblk[t + 1]] - a;
-                    memcpy(hvp + htp[t], hidx + a, len);
-                    memcpy(hcp + htp[t], hc16 + a, len * 2);
-                }
-                pl->vidx = (unsigned char *)dalloc(P, &rc);
-                pl->col16p = (short *)dalloc(P * 2, &rc);
-                pl->tpad = (unsigned *)dalloc(((size_t)pl->nblk + 1) * 4, &rc);
-                pl->vdict = (double *)dalloc(HB_VDICT * sizeof(double), &rc);
-            }
-            if (rc == 0 && (hipMemcpy(pl->vidx, hvp, P, hipMemcpyHostToDevice) != hipSuccess ||
-                            hipMemcpy(pl->col16p, hcp, P * 2, hipMemcpyHostToDevice) != hipSuccess ||
-                            hipMemcpy(pl->tpad, htp, ((size_t)pl->nblk + 1) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                            hipMemcpy(pl->vdict, dict, (size_t)nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
-                rc = HBICG_E_HIP;
-            if (rc == 0) {
-                d->vidx = pl->vidx;
-                d->col16p = pl->col16p;
-                d->tpad = pl->tpad;
-                d->vdict = pl->vdict;
-                d->nvdict = nd;
-                pl->val_bytes = (double)nzs;
-            }
-        }
-        free(htp);
-        free(hvp);
-        free(hcp);
-        free(hc16);
-        free(hidx);
-        free(tmp);
-        if (rc) goto fail;
-    }
+    pl->ptr_bytes = 4.0 * ((double)n + 1.0);
+    if (allow_vi && pl->nz > 0 && (rc = build_value_layout(pl, A, a_on_device, hptr, hblk)) != 0) goto fail;
 
     if (a_on_device) free(hptr);
     free(hblk);
@@ -741,6 +849,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->n = pl->n;
     R->window = pl->W;
     R->value_table = pl->dev.nvdict;
+    R->diag_tiles = pl->n_diag_tiles;
     R->dot_r = st.dot_r;
     R->dot_zero = st.dot_zero;
     R->rTr = st.rTr;
@@ -748,7 +857,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->t_total_ms = now_ms() - t_start;
     if (pl->profile) {
         const double n = (double)pl->n;
-        const double spmv = pl->val_bytes + pl->col_bytes + 4.0 * (n + 1.0) + 4.0 * (pl->nblk + 1.0);
+        const double spmv = pl->val_bytes + pl->col_bytes + pl->ptr_bytes + 4.0 * (pl->nblk + 1.0);
         /* algorithmic bytes per launch: SpMV 1 gathers p_s and reads r^
          * (writes s), SpMV 2 reads r and s (writes q and y), the seed update
          * reads q, y, r^ and writes r (exact order: also p_s and x_s in,

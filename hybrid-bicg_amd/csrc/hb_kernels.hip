@@ -146,7 +146,11 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ blk, const double *__restrict__ xbase,
                    double *__restrict__ rring, const double *__restrict__ svec,
                    const double *__restrict__ rhat, double *__restrict__ out1,
-                   double *__restrict__ qring, double *__restrict__ part, double shift)
+                   double *__restrict__ qring, double *__restrict__ part, double shift,
+                   const unsigned char *__restrict__ vidx = nullptr, const short *__restrict__ col16p = nullptr,
+                   const unsigned *__restrict__ tpad = nullptr, const double *__restrict__ vdict = nullptr,
+                   const unsigned *__restrict__ dtile = nullptr, const int *__restrict__ doff = nullptr,
+                   const unsigned char *__restrict__ didx = nullptr)
 {
     __shared__ double prod[VI ? 1 : HB_NNZ_CAP + 1];
     __shared__ __attribute__((aligned(16))) short scol[HB_NNZ_CAP + 8];
@@ -242,10 +246,67 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
      * and walk their rows there; other tiles read them where they lie, T
      * lanes per row.  Products and their summation order are those of the
      * 8-byte path, so results are bitwise the same. */
-    auto tile_vi = [&]() {
-        const unsigned char *__restrict__ vidxp = d.vidx;
-        const short *__restrict__ col16p = d.col16p;
-        const unsigned pb = d.tpad[b];
+    auto tile_vi = [&](unsigned dk, unsigned dbase) {
+        {
+            if (dk) {
+                /* Diagonal tile (more than HB_BLOCK/2 rows, so T = 1 as in
+                 * the CSR form): thread tid owns row r0 + tid, whose KP index
+                 * bytes (one per diagonal, HB_DIAG_NONE where the row has no
+                 * entry) come in one 8- or 16-byte load; column = row +
+                 * diagonal.  The row sum skips the missing diagonals and runs
+                 * over the others in ascending order, i.e. in CSR order. */
+                const unsigned i_me = r0 + (unsigned)min(tid, R - 1);
+                double oa = 0.0, ob = 0.0, oc = 0.0;
+                operands(i_me, oa, ob, oc);
+                if (tid < R) {
+                    const int *__restrict__ offs = doff + (size_t)HB_DIAG_K * b;
+                    int o[HB_DIAG_K];
+#pragma unroll
+                    for (int j = 0; j < HB_DIAG_K; ++j) o[j] = offs[j];
+                    unsigned w[4];
+                    if (dk <= 8) {
+                        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+                        const u2 v = *(const u2 *)(didx + dbase + 8u * (unsigned)tid);
+                        w[0] = v.x; w[1] = v.y; w[2] = 0xffffffffu; w[3] = 0xffffffffu;
+                    } else {
+                        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+                        const u4 v = *(const u4 *)(didx + dbase + 16u * (unsigned)tid);
+                        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+                    }
+                    /* eight diagonals at a time: every gather is issued
+                     * before the first product, and the sum selects
+                     * (no branch) so that no load waits inside one */
+                    double acc = 0.0;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (8 * h >= (int)dk) break;
+                        double vv[8], xv[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            /* the gather address does not wait for the index
+                             * bytes: row + diagonal, clamped into [0, n) for
+                             * a missing diagonal or a padding slot (diagonal
+                             * 0); rows of a tile lie within HB_ROWS_CAP of
+                             * each other, so the sum cannot overflow */
+                            const int cs = (int)i_me + o[8 * h + j];
+                            xv[j] = xval((unsigned)min(max(cs, 0), n - 1));
+                        }
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) vv[j] = vd[(w[2 * h + j / 4] >> (8 * (j % 4))) & 255u];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const unsigned ix = (w[2 * h + j / 4] >> (8 * (j % 4))) & 255u;
+                            const double t = acc + vv[j] * xv[j];
+                            acc = ix != HB_DIAG_NONE ? t : acc;
+                        }
+                    }
+                    epilogue(i_me, acc, oa, ob, oc);
+                }
+                return;
+            }
+        }
+        const unsigned char *__restrict__ vidxp = vidx;
+        const unsigned pb = tpad[b];
         if (N <= HB_NNZ_CAP) {
             int T = 1;
             if (!EXACT) {
@@ -441,9 +502,15 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         }
     };
     if constexpr (VI) {
-        if (tid < d.nvdict) vd[tid] = d.vdict[tid];
+        /* the tile's diagonal record is read before the barrier */
+        unsigned dk = 0, dbase = 0;
+        if (dtile) {
+            dk = dtile[2 * b + 1];
+            dbase = dtile[2 * b];
+        }
+        if (tid < d.nvdict) vd[tid] = vdict[tid];
         __syncthreads();
-        tile_vi();
+        tile_vi(dk, dbase);
     } else {
         tile();
     }
@@ -459,10 +526,12 @@ DEV void spmv_tile(const hb_dev &d, unsigned b)
 {
     if (MODE == SP_SEED1)
         spmv_body<MODE, EXACT, VI>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, d.p_set, d.rring, nullptr,
-                                   d.rhat, d.s, nullptr, d.part, 0.0);
+                                   d.rhat, d.s, nullptr, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
+                                   d.doff, d.didx);
     else
         spmv_body<MODE, EXACT, VI>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
-                                   d.y, d.qring, d.part, 0.0);
+                                   d.y, d.qring, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile, d.doff,
+                                   d.didx);
 }
 
 /* Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups round

@@ -19,6 +19,8 @@ extern "C" {
 #define HB_WMAX_EXACT   16    /* largest window of the exact-order replay      */
 #define HB_UG           16    /* shifts per chunk of the combined-coefficient flush */
 #define HB_VDICT        256   /* largest value table of the value-indexed SpMV */
+#define HB_DIAG_K       16    /* most diagonals of a diagonal tile             */
+#define HB_DIAG_NONE    255   /* index byte of a diagonal a row has no entry on */
 #define HB_CC_ROWS(W)   (2 * (W) + 5) /* rows of one chunk's coefficient block  */
 
 /* Error bits in hb_state.status */
@@ -94,6 +96,10 @@ typedef struct hb_dev {
     HB_G const short    *col16p;     /* col16 in the same tile-padded layout (with vidx) */
     HB_G const unsigned *tpad;       /* [nblk+1] tile b's entries start at tpad[b], a
                                         multiple of 8, in vidx and col16p            */
+    HB_G const unsigned *dtile;      /* [2*nblk] diagonal tiles: {first index byte, K};
+                                        K = 0: CSR tile.  NULL: no diagonal tiles    */
+    HB_G const int      *doff;       /* [HB_DIAG_K*nblk] diagonals (column - row)    */
+    HB_G const unsigned char *didx;  /* index bytes of the diagonal tiles            */
     HB_G const double   *vdict;      /* [nvdict <= HB_VDICT] the matrix's distinct values */
     int nvdict;
     HB_G const double   *sigma;/* [M]                                      */

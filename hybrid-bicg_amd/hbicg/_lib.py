@@ -60,7 +60,8 @@ class Report(C.Structure):
                 ("seed_final", C.c_int), ("status", C.c_int), ("M", C.c_int), ("n", C.c_int),
                 ("dot_r", C.c_double), ("dot_zero", C.c_double), ("rTr", C.c_double),
                 ("t_setup_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_total_ms", C.c_double),
-                ("window", C.c_int), ("value_table", C.c_int)]
+                ("window", C.c_int), ("value_table", C.c_int),
+                ("diag_tiles", C.c_int)]
 
 
 class Tuning(C.Structure):

@@ -70,6 +70,7 @@ typedef struct hbicg_report {
     double t_total_ms;   /* wall time of the whole call                        */
     int    window;       /* shift-update window the solve ran with             */
     int    value_table;  /* distinct values the SpMV read through its table (0: none) */
+    int    diag_tiles;   /* SpMV tiles stored by diagonals (value table only)         */
 } hbicg_report;
 
 void hbicg_default_params(hbicg_params *p);

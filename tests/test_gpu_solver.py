@@ -237,8 +237,85 @@ def test_value_table_bitwise(c0, exact):
     a = dropin_solve(c0.A, c0.b, c0.sigma, 8, "dynamic", mk(True))
     b = dropin_solve(c0.A, c0.b, c0.sigma, 8, "dynamic", mk(False))
     assert a.report["value_table"] == 3 and b.report["value_table"] == 0
+    assert a.report["diag_tiles"] > 0 and b.report["diag_tiles"] == 0
     assert a.k == b.k and a.switches == b.switches
     assert np.array_equal(a.x, b.x) and np.array_equal(a.r, b.r)
+
+
+def _rows_csr(rows, n):
+    """CSR from per-row lists of (column, value), in the given order."""
+    ptr = np.zeros(n + 1, np.uint32)
+    ptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c, _ in r], np.uint32)
+    val = np.array([v for r in rows for _, v in r], np.float64)
+    return P.CSR(ptr=ptr, col=col, val=val, n=n)
+
+
+def _mixed_tile_matrix():
+    """5-point 64x64 operator edited so that its 256-row tiles take every
+    path of the value-table SpMV: rows of tile 3 list their entries in
+    descending column order (CSR form), one row of tile 5 has entries on 20
+    more diagonals (more than 16: CSR form), every row of tile 7 has two more
+    diagonals and one of them four more (11 diagonals: 16 index bytes per
+    row), boundary rows miss diagonals."""
+    A = P.conv_diff_2d(64, 64)
+    n = A.n
+    ptr = np.asarray(A.ptr, np.int64)
+    rows = [list(zip(A.col[ptr[i]:ptr[i + 1]].tolist(), A.val[ptr[i]:ptr[i + 1]].tolist())) for i in range(n)]
+    for i in range(3 * 256, 4 * 256):
+        rows[i] = sorted(rows[i], key=lambda e: -e[0])
+    r = 5 * 256 + 100
+    rows[r] = sorted(rows[r] + [(r + o, 0.01) for o in range(2, 22) if o != 64], key=lambda e: e[0])
+    for i in range(7 * 256, 8 * 256):
+        extra = [(i - 2, -0.02), (i + 2, 0.02)] + ([(i + o, 0.01) for o in (-4, -3, 3, 4)] if i == 7 * 256 + 9 else [])
+        rows[i] = sorted(rows[i] + extra, key=lambda e: e[0])
+    return _rows_csr(rows, n)
+
+
+@pytest.mark.parametrize("exact", [False, True])
+def test_value_table_diagonal_tiles(exact):
+    """Tiles stored by diagonals (one index byte per row and diagonal, no
+    columns) and tiles kept in CSR form in one matrix: results bitwise equal
+    to reading the 8-byte values; in exact order also equal to the oracle."""
+    A = _mixed_tile_matrix()
+    plain = P.conv_diff_2d(64, 64)
+    rng = np.random.default_rng(21)
+    b = rng.uniform(-1.0, 1.0, A.n)
+    sigma = np.logspace(-2, 0, 9)
+    print(f"case diagonal tiles exact={exact}", flush=True)
+    mk = lambda vi: make_params(tol=1e-10, exact_order=exact, value_index=vi)
+    a = dropin_solve(A, b, sigma, 0, "dynamic", mk(True))
+    c = dropin_solve(A, b, sigma, 0, "dynamic", mk(False))
+    full = dropin_solve(plain, b, sigma, 0, "dynamic", mk(True))
+    assert a.report["value_table"] > 0 and c.report["value_table"] == 0
+    assert full.report["diag_tiles"] == 16
+    assert 0 < a.report["diag_tiles"] <= 14, a.report["diag_tiles"]
+    assert a.k == c.k and a.switches == c.switches
+    assert np.array_equal(a.x, c.x) and np.array_equal(a.r, c.r)
+    if exact:
+        ref = O.solve(A, b, sigma, 0, tol=1e-10)
+        assert a.k == ref.k and np.array_equal(a.x, ref.x) and np.array_equal(a.r, ref.r)
+
+
+def test_value_table_256_values_csr_tiles():
+    """256 distinct values fill the table; no index byte is left to mark a
+    missing diagonal, so every tile stays in CSR form, bitwise equal to
+    reading the 8-byte values."""
+    A = P.conv_diff_2d(64, 64)
+    ptr = np.asarray(A.ptr, np.int64)
+    val = np.empty(len(A.val))
+    for i in range(A.n):
+        for e in range(ptr[i], ptr[i + 1]):
+            val[e] = 4.0 + (i % 128) / 1000.0 if A.col[e] == i else -(0.5 + (e % 128) / 1000.0)
+    A = P.CSR(ptr=A.ptr, col=A.col, val=val, n=A.n)
+    assert len(np.unique(val)) == 256
+    rng = np.random.default_rng(22)
+    b = rng.uniform(-1.0, 1.0, A.n)
+    sigma = np.logspace(-2, 0, 5)
+    a = dropin_solve(A, b, sigma, 0, "dynamic", make_params(tol=1e-10, value_index=True))
+    c = dropin_solve(A, b, sigma, 0, "dynamic", make_params(tol=1e-10, value_index=False))
+    assert a.report["value_table"] == 256 and a.report["diag_tiles"] == 0
+    assert a.k == c.k and np.array_equal(a.x, c.x)
 
 
 def test_value_table_not_used_for_many_values():
