@@ -166,16 +166,23 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         hb_state *st = d.st;
         /* the shift kernel runs once per scalar stage: it compares k with
          * the epoch written here, after the previous shift kernel finished */
-        if (MODE == SP_SEED1 && b == 0 && threadIdx.x == 0) st->k4_epoch = st->k;
-        if (st->done) return;
-        const int k = st->k, seed = st->seed;
+        /* every state word is read before that store, so the loads issue
+         * together (a store through st would order the reads after it) */
+        const int done = st->done, k = st->k, seed = st->seed;
+        const int sw = MODE == SP_SEED1 ? st->switched : 0;
+        const double zs = MODE == SP_SEED1 ? st->zscale : 1.0;
+        const int wk0 = MODE == SP_SEED1 ? st->win_k0 : 0;
+        if (MODE == SP_SEED1 && b == 0 && threadIdx.x == 0) st->k4_epoch = k;
+        /* k >= 1, seed >= 0 and switched >= 0 always hold; testing them
+         * keeps their loads ahead of the branch, in one round trip with done */
+        if (done | (k < 1) | (seed < 0) | (sw < 0)) return;
         sig = d.sigma[seed];
         rprev = rring + off((k - 1) % (d.W + 1), n);
         if (MODE == SP_SEED1) {
             xa = xbase + off(seed, n);
-            switched = st->switched;
-            zscale = st->zscale;
-            snap = !d.exact && k == st->win_k0;
+            switched = sw;
+            zscale = zs;
+            snap = !d.exact && k == wk0;
         } else {
             xa = rprev;
             nalpha = -d.aa[k];
@@ -246,65 +253,65 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
      * and walk their rows there; other tiles read them where they lie, T
      * lanes per row.  Products and their summation order are those of the
      * 8-byte path, so results are bitwise the same. */
-    auto tile_vi = [&](unsigned dk, unsigned dbase) {
-        {
-            if (dk) {
-                /* Diagonal tile (more than HB_BLOCK/2 rows, so T = 1 as in
-                 * the CSR form): thread tid owns row r0 + tid, whose KP index
-                 * bytes (one per diagonal, HB_DIAG_NONE where the row has no
-                 * entry) come in one 8- or 16-byte load; column = row +
-                 * diagonal.  The row sum skips the missing diagonals and runs
-                 * over the others in ascending order, i.e. in CSR order. */
-                const unsigned i_me = r0 + (unsigned)min(tid, R - 1);
-                double oa = 0.0, ob = 0.0, oc = 0.0;
-                operands(i_me, oa, ob, oc);
-                if (tid < R) {
-                    const int *__restrict__ offs = doff + (size_t)HB_DIAG_K * b;
-                    int o[HB_DIAG_K];
+    /* Diagonal tile (more than HB_BLOCK/2 rows, so T = 1 as in the CSR
+     * form): thread tid owns row r0 + tid, whose index bytes (one per
+     * diagonal, HB_DIAG_NONE where the row has no entry) come in one 8- or
+     * 16-byte load; column = row + diagonal.  The row sum skips the missing
+     * diagonals and runs over the others in ascending order, i.e. in CSR
+     * order.  Eight diagonals at a time, every gather is issued before the
+     * value table is staged (its barrier then overlaps their latency), and
+     * the sum selects rather than branches, so no load waits inside a branch.
+     * vdv: this thread's value-table entry, loaded at kernel start. */
+    auto tile_dia = [&](unsigned dk, unsigned dbase, double vdv) {
+        const unsigned i_me = r0 + (unsigned)min(tid, R - 1);
+        double oa = 0.0, ob = 0.0, oc = 0.0;
+        operands(i_me, oa, ob, oc);
+        const int *__restrict__ offs = doff + (size_t)HB_DIAG_K * b;
+        int o[HB_DIAG_K];
 #pragma unroll
-                    for (int j = 0; j < HB_DIAG_K; ++j) o[j] = offs[j];
-                    unsigned w[4];
-                    if (dk <= 8) {
-                        typedef unsigned u2 __attribute__((ext_vector_type(2)));
-                        const u2 v = *(const u2 *)(didx + dbase + 8u * (unsigned)tid);
-                        w[0] = v.x; w[1] = v.y; w[2] = 0xffffffffu; w[3] = 0xffffffffu;
-                    } else {
-                        typedef unsigned u4 __attribute__((ext_vector_type(4)));
-                        const u4 v = *(const u4 *)(didx + dbase + 16u * (unsigned)tid);
-                        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
-                    }
-                    /* eight diagonals at a time: every gather is issued
-                     * before the first product, and the sum selects
-                     * (no branch) so that no load waits inside one */
-                    double acc = 0.0;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        if (8 * h >= (int)dk) break;
-                        double vv[8], xv[8];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            /* the gather address does not wait for the index
-                             * bytes: row + diagonal, clamped into [0, n) for
-                             * a missing diagonal or a padding slot (diagonal
-                             * 0); rows of a tile lie within HB_ROWS_CAP of
-                             * each other, so the sum cannot overflow */
-                            const int cs = (int)i_me + o[8 * h + j];
-                            xv[j] = xval((unsigned)min(max(cs, 0), n - 1));
-                        }
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) vv[j] = vd[(w[2 * h + j / 4] >> (8 * (j % 4))) & 255u];
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const unsigned ix = (w[2 * h + j / 4] >> (8 * (j % 4))) & 255u;
-                            const double t = acc + vv[j] * xv[j];
-                            acc = ix != HB_DIAG_NONE ? t : acc;
-                        }
-                    }
-                    epilogue(i_me, acc, oa, ob, oc);
-                }
-                return;
-            }
+        for (int j = 0; j < HB_DIAG_K; ++j) o[j] = offs[j];
+        unsigned w[4];
+        if (dk <= 8) {
+            typedef unsigned u2 __attribute__((ext_vector_type(2)));
+            const u2 v = *(const u2 *)(didx + dbase + 8u * (unsigned)min(tid, R - 1));
+            w[0] = v.x; w[1] = v.y; w[2] = 0xffffffffu; w[3] = 0xffffffffu;
+        } else {
+            typedef unsigned u4 __attribute__((ext_vector_type(4)));
+            const u4 v = *(const u4 *)(didx + dbase + 16u * (unsigned)min(tid, R - 1));
+            w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
         }
+        double xv[8];
+        /* the gather address does not wait for the index bytes: row +
+         * diagonal, clamped into [0, n) for a missing diagonal or a padding
+         * slot (diagonal 0); rows of a tile lie within HB_ROWS_CAP of each
+         * other, so the sum cannot overflow */
+        auto gather8 = [&](int j0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int cs = (int)i_me + o[j0 + j];
+                xv[j] = xval((unsigned)min(max(cs, 0), n - 1));
+            }
+        };
+        double acc = 0.0;
+        auto sum8 = [&](int h) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const unsigned ix = (w[2 * h + j / 4] >> (8 * (j % 4))) & 255u;
+                const double t = acc + vd[ix] * xv[j];
+                acc = ix != HB_DIAG_NONE ? t : acc;
+            }
+        };
+        gather8(0);
+        if (tid < d.nvdict) vd[tid] = vdv;
+        __syncthreads();
+        sum8(0);
+        if (dk > 8) {
+            gather8(8);
+            sum8(1);
+        }
+        if (tid < R) epilogue(i_me, acc, oa, ob, oc);
+    };
+    auto tile_vi = [&]() {
         const unsigned char *__restrict__ vidxp = vidx;
         const unsigned pb = tpad[b];
         if (N <= HB_NNZ_CAP) {
@@ -502,15 +509,19 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         }
     };
     if constexpr (VI) {
-        /* the tile's diagonal record is read before the barrier */
+        const double vdv = tid < d.nvdict ? vdict[tid] : 0.0;
         unsigned dk = 0, dbase = 0;
         if (dtile) {
             dk = dtile[2 * b + 1];
             dbase = dtile[2 * b];
         }
-        if (tid < d.nvdict) vd[tid] = vdict[tid];
-        __syncthreads();
-        tile_vi(dk, dbase);
+        if (dk) {
+            tile_dia(dk, dbase, vdv);
+        } else {
+            if (tid < d.nvdict) vd[tid] = vdv;
+            __syncthreads();
+            tile_vi();
+        }
     } else {
         tile();
     }
