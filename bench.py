@@ -14,6 +14,14 @@ Printed: one JSON line with the contract fields plus
                 between two events, so no per-launch event gap is counted);
                 traffic: HBM bytes per launch from the committed rocprofv3
                 FETCH_SIZE / WRITE_SIZE summary of this workload, when there;
+  window_flush_roofline
+                the window flushes' multiply-adds (twice, as flops) over their
+                duration against the FP64 peak: the flush is FMA-bound;
+  other_spmv_format
+                the same solve with the other SpMV matrix format (by default
+                the matrix is read through its table of 3 distinct values, by
+                diagonal tiles; --value-index 0 reads 8-byte CSR values), x
+                compared bit for bit;
   cpu_baseline  the oracle (CPU restatement of the reference) timed on a
                 bounded sample of the same solve on this host, extrapolated
                 to the full solve from its seed-step and per-shift timers;
@@ -38,6 +46,7 @@ sys.path.insert(0, os.path.join(ROOT, "hybrid-bicg_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBPS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
 METRIC = "shift-solves/s @ relres 1e-10; SpMV %HBM peak; batched & adjoint vs plain"
 
 
@@ -67,6 +76,9 @@ def parse():
     ap.add_argument("--shifts", type=int, default=64)
     ap.add_argument("--variant", default="dynamic", choices=["normal", "static", "dynamic"])
     ap.add_argument("--window", type=int, default=0, help="shift-update window (0: variant default)")
+    ap.add_argument("--value-index", type=int, default=1, choices=[0, 1],
+                    help="1 (library default): SpMV reads the matrix through its value table "
+                         "(and diagonal tiles) where it qualifies; 0: 8-byte CSR values")
     ap.add_argument("--cpu-iters", type=int, default=40, help="oracle iterations timed (0: skip)")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-problem batched leg")
@@ -381,7 +393,8 @@ def main():
     M, n = prob.M, prob.n
     graph = 1 if args.variant == "dynamic" else 0
     window = args.window if args.window > 0 else (1 if args.variant == "normal" else 0)
-    params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph)
+    params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph,
+                         value_index=bool(args.value_index))
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)
     x = torch.zeros((M, n), dtype=torch.float64, device=dev)
@@ -407,14 +420,15 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step")
     rep, stop_iter, switches = plan.report()
     window = rep["window"]
-    value_table = rep["value_table"]
+    value_table, diag_tiles = rep["value_table"], rep["diag_tiles"]
 
-    # the same solve with the matrix read through a one-byte value table
-    # (value_index=1), for comparison: the format changes bytes, not results
+    # the same solve with the other SpMV format (8-byte CSR values, or the
+    # value table with diagonal tiles), for comparison: the format changes
+    # bytes, not results
     general = None
-    if not value_table:
+    if value_table or not args.value_index:
         gplan = Plan(prob.A, M, make_params(tol=prob.tol, max_iter=prob.max_iter, window=window,
-                                            graph=graph, value_index=True), device=dev)
+                                            graph=graph, value_index=not value_table), device=dev)
         xg = torch.zeros_like(x)
         gplan.solve(b, prob.sigma, prob.seed, xg)
         torch.cuda.synchronize()
@@ -425,9 +439,11 @@ def main():
         torch.cuda.synchronize()
         tg = (time.perf_counter() - tg) / args.steps
         same = bool(torch.equal(xg, x))
+        grep = gplan.report()[0]
         general = {"ms_per_step": round(tg * 1e3, 3), "value": round(M / tg, 3), "x_bitwise_equal": same,
-                   "value_table": gplan.report()[0]["value_table"]}
-        log(f"value table: {tg * 1e3:.1f} ms/step, x bitwise equal {same}")
+                   "value_index": int(not value_table), "value_table": grep["value_table"],
+                   "diag_tiles": grep["diag_tiles"]}
+        log(f"other SpMV format: {tg * 1e3:.1f} ms/step, x bitwise equal {same}")
         gplan.close()
         del xg
     true_res = plan.true_residuals(b, prob.sigma, x)
@@ -445,10 +461,10 @@ def main():
     ref_equiv = ref_bytes / (ms_per_step * 1e-3) / 1e9
 
     # profiled solve of the same workload: HIP events around every kernel
-    kernels, roofline = {}, None
+    kernels, roofline, flush_roofline = {}, None, None
     if not args.no_profile:
         pparams = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=0,
-                              profile=True)
+                              profile=True, value_index=bool(args.value_index))
         pplan = Plan(prob.A, M, pparams, device=dev)
         xp = torch.zeros_like(x)
         pplan.solve(b, prob.sigma, prob.seed, xp)
@@ -473,8 +489,17 @@ def main():
             kd["avg_ms"] = round(kd["total_ms"] / max(kd["launches"], 1), 5)
         spmv_ms = kernels["spmv_seed"]["total_ms"] + kernels["spmv_q_fused"]["total_ms"]
         spmv_b = kernels["spmv_seed"]["bytes"] + kernels["spmv_q_fused"]["bytes"]
-        sh_ms = kernels["shift_window"]["total_ms"]
-        sh_b = kernels["shift_window"]["bytes"]
+        sh_ms = kernels["seed_p_update"]["total_ms"] + kernels["window_flush"]["total_ms"]
+        sh_b = kernels["seed_p_update"]["bytes"] + kernels["window_flush"]["bytes"]
+        wf = kernels["window_flush"]
+        if wf["launches"]:
+            # the flush is a multiply-add stream over the window vectors:
+            # bounded by the FP64 rate, not by HBM
+            tf = wf.get("flops", 0.0) / (wf["total_ms"] * 1e-3) / 1e12
+            flush_roofline = {"kernel": "window_flush", "bound": "fp64", "achieved": round(tf, 2),
+                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": round(tf / FP64_PEAK_TFLOPS, 4), "launches": wf["launches"],
+                              "ms": round(wf["total_ms"], 3), "flops": wf.get("flops", 0.0)}
         spmv_shift_frac = (spmv_b + sh_b) / ((spmv_ms + sh_ms) * 1e-3) / 1e9 / HBM_PEAK_GBPS
     else:
         spmv_shift_frac = None
@@ -523,8 +548,8 @@ def main():
         "config": {"workload": f"conv-diff 3-D 7-point {args.grid}^3 (n={n}, nnz={prob.A.nnz}), "
                                f"{M} shifts log-spaced [1e-3,1], relres {prob.tol:g}, seed=smallest shift",
                    "variant": args.variant, "window": window, "hipgraph": bool(graph),
-                   "spmv_value_table": value_table},
-        "with_value_table": general,
+                   "spmv_value_table": value_table, "spmv_diag_tiles": diag_tiles},
+        "other_spmv_format": general,
         "iterations": iters,
         "seed_switches": switches,
         "true_residual_max": float(true_res.max()),
@@ -537,6 +562,7 @@ def main():
                     "of every active shift read and written each iteration) must move for this "
                     "solve, divided by the whole solve time measured here"},
         "roofline": roofline,
+        "window_flush_roofline": flush_roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -42,7 +42,7 @@ void hbicg_default_params(hbicg_params *p)
     p->profile = 0;
     p->growth_limit = 0.0;
     p->trace = 0;
-    p->value_index = 0;
+    p->value_index = 1;
 }
 
 static const hbicg_params *params(void)
@@ -132,7 +132,9 @@ struct hbicg_plan {
     hipEvent_t *pev;          /* [2][chunk][HB_NSTAGE + 3] */
     double *ptimes;           /* [iterations launched][HB_NSTAGE + 1] */
     int ptimes_cap;
-    hbicg_kernel_stat kstat[HB_NSTAGE];
+    hbicg_kernel_stat kstat[HB_NSTAGE + 1];
+    int *flush_log;           /* device [cap], params.profile */
+    int *flush_host;          /* host copy */
     /* last report */
     hbicg_report rep;
     int *h_stop_iter, *h_switch_k, *h_switch_to;
@@ -183,7 +185,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->flush_log, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -197,6 +199,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
         free(pl->pev);
     }
     free(pl->ptimes);
+    free(pl->flush_host);
     if (pl->own_stream && pl->stream) hipStreamDestroy(pl->stream);
     free(pl->h_stop_iter); free(pl->h_switch_k); free(pl->h_switch_to);
     free(pl);
@@ -548,6 +551,11 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->ccoef = (double *)dalloc(ccn * 8, &rc);
     pl->grow = (double *)dalloc((size_t)2 * M * 8, &rc);
     if (pl->want_trace) pl->trace = (double *)dalloc(cap * (size_t)M * 8, &rc);
+    if (pl->profile) {
+        pl->flush_log = (int *)dalloc(cap * sizeof(int), &rc);
+        pl->flush_host = (int *)calloc(cap, sizeof(int));
+        if (!pl->flush_host) rc = HBICG_E_NOMEM;
+    }
     pl->part = (double *)dalloc((size_t)maxparts * 2 * 8, &rc);
     pl->p_snap = (double *)dalloc(nb, &rc);
     pl->stop = (int *)dalloc((size_t)M * 4, &rc);
@@ -585,6 +593,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->s = pl->s; d->y = pl->y; d->p_set = pl->p_set; d->x_set = NULL;
     d->aa = pl->aa; d->ba = pl->ba; d->wa = pl->wa; d->pi = pl->pi; d->eta = pl->eta;
     d->zeta = pl->zeta; d->coef = pl->coef; d->ccoef = pl->ccoef; d->grow = pl->grow; d->trace = pl->trace;
+    d->flush_log = pl->flush_log;
     d->stop = pl->stop;
     d->stop_iter = pl->stop_iter;
     d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
@@ -817,6 +826,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
         rc = HBICG_E_HIP;
         goto fail;
     }
+    if (pl->flush_log) HIPCK(hipMemsetAsync(pl->flush_log, 0, (size_t)pl->cap * sizeof(int), pl->stream));
     if (hb_k_init(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
     if ((rc = run_loop(pl)) != 0) goto fail;
     if (hb_k_finalize(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
@@ -884,8 +894,29 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
                 }
             }
             K->bytes = per_launch[sg] * iters;
+            K->flops = 0.0;
         }
-        pl->kstat[HB_STAGE_SHIFT].bytes = st.shift_bytes;
+        /* the shift kernel's launches split in two: [7] the seed p update
+         * alone (p, r, s in, p out), [8] the launches that also ran a window
+         * flush (the device-counted rest of the bytes, and its multiply-adds) */
+        HIPCK(hipMemcpy(pl->flush_host, pl->flush_log, (size_t)pl->cap * sizeof(int), hipMemcpyDeviceToHost));
+        hbicg_kernel_stat *Kp = &pl->kstat[HB_STAGE_SHIFT], *Kf = &pl->kstat[HB_NSTAGE];
+        memset(Kf, 0, sizeof(*Kf));
+        Kp->total_ms = 0.0;
+        Kp->launches = 0;
+        for (int it = 0; it < iters && it < pl->ptimes_cap; ++it) {
+            const double t = pl->ptimes[(size_t)it * HB_PT + HB_STAGE_SHIFT];
+            if (it + 1 < pl->cap && pl->flush_host[it + 1]) {
+                Kf->launches += 1;
+                Kf->total_ms += t;
+            } else {
+                Kp->launches += 1;
+                Kp->total_ms += t;
+            }
+        }
+        Kp->bytes = 32.0 * n * Kp->launches;
+        Kf->bytes = st.shift_bytes - Kp->bytes;
+        Kf->flops = 2.0 * st.flush_fma * n;
     }
     if (rep) *rep = *R;
     return st.k;
@@ -911,7 +942,7 @@ fail:
 int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax)
 {
     if (!pl || !stats) return HBICG_E_ARG;
-    const int m = nmax < HB_NSTAGE ? nmax : HB_NSTAGE;
+    const int m = nmax < HB_NSTAGE + 1 ? nmax : HB_NSTAGE + 1;
     for (int i = 0; i < m; ++i) stats[i] = pl->kstat[i];
     return m;
 }

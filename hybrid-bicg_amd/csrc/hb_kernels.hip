@@ -898,6 +898,13 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         double bytes = 32.0;
         if (fl) bytes += 8.0 * (2 * len + 1) + (xonly ? 16.0 : 32.0) * st->flush_cnt;
         st->shift_bytes += bytes * (double)d.n;
+        /* multiply-adds per row and listed shift: x_j takes p_j, r_{k0-1}
+         * and two window vectors per iteration; p_j as many again; a tail
+         * iteration applied on its own six more */
+        if (fl)
+            st->flush_fma += (double)st->flush_cnt *
+                             ((2.0 * (len - spike) + 2.0) * (xonly ? 1.0 : 2.0) + (spike ? 6.0 : 0.0));
+        if (d.flush_log && k < d.cap) d.flush_log[k] = fl;
     }
     if (d.growth_limit > 0.0) {
         /* a closing window leaves p_j materialised: the next window's bound
@@ -1569,6 +1576,7 @@ DEV void init_fin_body(const hb_dev &d)
         st->status = 0;
         st->zscale = 1.0;
         st->shift_bytes = 0.0;
+        st->flush_fma = 0.0;
     }
 }
 

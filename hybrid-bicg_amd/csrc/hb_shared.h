@@ -43,6 +43,7 @@ typedef struct hb_state {
     double zscale;     /* 1/(zeta_m pi_m[k]) to apply to r after a switch */
     double pb, pbw;    /* beta~[k] and -beta~[k]*omega~[k] for the seed p update */
     double shift_bytes;/* algorithmic HBM bytes the shift kernel moved (summed) */
+    double flush_fma;  /* multiply-adds of the window flushes, per row of x_j (summed) */
     int k;             /* iteration about to run (1-based, reference k)   */
     int kprev;         /* iteration the shift kernel works for            */
     int seed;          /* current seed index                              */
@@ -123,6 +124,8 @@ typedef struct hb_dev {
     HB_G int    *switch_k, *switch_to; /* [M] switch log                   */
     HB_G double *part;         /* workgroup partial sums [max(nblk,vec_grid)*2] */
     HB_G double *p_snap;       /* [n] the seed's p at the pending window's start */
+    HB_G int *flush_log;       /* [cap] params.profile: 1 where iteration k's shift
+                                  kernel applied a window flush, else NULL       */
     HB_G double *trace;        /* [cap][M] estimated residual per iteration and shift,
                                   -1 once stopped (NULL: not recorded)     */
     HB_G hb_state *st;

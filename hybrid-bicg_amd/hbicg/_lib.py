@@ -51,7 +51,7 @@ class Params(C.Structure):
 class KernelStat(C.Structure):
     """include/hbicg.h hbicg_kernel_stat."""
     _fields_ = [("launches", C.c_int), ("total_ms", C.c_double), ("bytes", C.c_double),
-                ("replays", C.c_int), ("replay_ms", C.c_double)]
+                ("replays", C.c_int), ("replay_ms", C.c_double), ("flops", C.c_double)]
 
 
 class Report(C.Structure):

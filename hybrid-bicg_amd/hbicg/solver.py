@@ -19,7 +19,7 @@ VARIANTS = ("normal", "static", "dynamic")
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
                 device=0, exit_on_error=False, upload_x=True, profile=False,
-                growth_limit=0.0, trace=False, value_index=False) -> _lib.Params:
+                growth_limit=0.0, trace=False, value_index=True) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -190,7 +190,7 @@ class Plan:
         return out[:rows]
 
     KERNELS = ("spmv_seed", "fin_spmv_seed", "spmv_q_fused", "fin_spmv_q", "seed_update",
-               "scalar_stage", "flush_coef", "shift_window")
+               "scalar_stage", "flush_coef", "seed_p_update", "window_flush")
 
     def kernel_stats(self):
         """Per-kernel {launches, total_ms, bytes, gbps} of the last profiled
@@ -206,6 +206,8 @@ class Plan:
                                     "bytes": st.bytes, "gbps": gbps}
             if st.replays:
                 out[self.KERNELS[i]].update(replays=st.replays, replay_ms=st.replay_ms)
+            if st.flops:
+                out[self.KERNELS[i]].update(flops=st.flops)
         return out
 
     def true_residuals(self, b, sigma, x):

@@ -51,10 +51,12 @@ typedef struct hbicg_params {
                             p exceeds this multiple of one iteration's own terms (0: off;
                             the regrouped coefficients are exact to double precision)    */
     int    trace;        /* 1: record the per-iteration residual estimate of every shift  */
-    int    value_index;  /* 1: a matrix with at most 256 distinct values is read by the
-                            SpMV as one byte per entry into a table of its values (lossless;
-                            results unchanged; 5% faster solve at 128^3, the SpMV then no
-                            longer bandwidth-bound); 0 (default): 8-byte values           */
+    int    value_index;  /* 1 (default): a matrix with at most 256 distinct values is read
+                            by the SpMV as one byte per entry into a table of its values,
+                            and its tiles whose entries lie on at most 16 diagonals as one
+                            byte per row and diagonal (no columns, no row pointers); the
+                            products and their order are those of the CSR read, so results
+                            are bitwise unchanged.  0: 8-byte CSR values                   */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -131,16 +133,20 @@ int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
 /* Per-kernel statistics of the plan's last solve run with params.profile:
  * one entry per launch of an iteration: [0] seed SpMV 1, [1] its finalize
  * (alpha~), [2] fused q + SpMV 2, [3] its finalize (omega~), [4] seed
- * update, [5] scalar stage, [6] combined window coefficients, [7] seed p
- * update + window flush.  launches counts executed iterations,
- * total_ms the summed HIP-event durations, bytes the algorithmic HBM bytes
- * those launches had to move.  Returns the number of entries filled. */
+ * update, [5] scalar stage, [6] combined window coefficients, [7] the shift
+ * kernel's launches that ran the seed p update alone, [8] those that also
+ * applied a window flush.  launches counts the launches (executed
+ * iterations), total_ms the summed HIP-event durations, bytes the
+ * algorithmic HBM bytes those launches had to move, flops ([8] only) twice
+ * the flush's multiply-adds.  Returns the number of entries filled (at most
+ * 9). */
 typedef struct hbicg_kernel_stat {
     int    launches;
     double total_ms;
     double bytes;
     int    replays;      /* [2] only: extra back-to-back launches timed on their own */
     double replay_ms;    /*   (same inputs and outputs, no event gaps between them)  */
+    double flops;
 } hbicg_kernel_stat;
 int hbicg_plan_kernel_stats(hbicg_plan *pl, hbicg_kernel_stat *stats, int nmax);
 
