@@ -28,12 +28,18 @@ def oracle_c0(c0):
     return {s: O.solve(c0.A, c0.b, c0.sigma, s, tol=c0.tol) for s in (0, 8, 15)}
 
 
+@pytest.mark.parametrize("value_index", [True, False])
 @pytest.mark.parametrize("variant", ["normal", "static", "dynamic"])
 @pytest.mark.parametrize("seed", [0, 8, 15])
-def test_exact_order_bitwise_c0(c0, oracle_c0, variant, seed):
-    print(f"case exact c0 variant={variant} seed={seed}", flush=True)
+def test_exact_order_bitwise_c0(c0, oracle_c0, variant, seed, value_index):
+    """Exact order equals the oracle bit for bit, with the SpMV reading the
+    matrix through its value table and diagonal tiles (the default) and
+    reading 8-byte CSR values."""
+    print(f"case exact c0 variant={variant} seed={seed} value_index={value_index}", flush=True)
     ref = oracle_c0[seed]
-    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, make_params(tol=c0.tol, exact_order=True))
+    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant,
+                       make_params(tol=c0.tol, exact_order=True, value_index=value_index))
+    assert got.report["value_table"] == (3 if value_index else 0)
     assert got.k == ref.k
     np.testing.assert_array_equal(got.stop_iter, ref.stop_iter)
     assert got.switches == ref.switches
