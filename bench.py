@@ -28,7 +28,12 @@ Printed: one JSON line with the contract fields plus
   batched, adjoint, ingest
                 the 256-problem batch against per-problem plain solves, the
                 forward + adjoint solve of the 64^3 / 32-shift loss, and GPU
-                COO -> CSR of the 128^3 operator's shuffled entries.
+                COO -> CSR of the 128^3 operator's shuffled entries;
+  dropin_host_buffers
+                the headline workload through shifted_lopbicg_dynamic on host
+                arrays: set-up, PCIe transfers and solve (never the value);
+  powerlaw      the 1e6-row power-law matrix (31M nonzeros, 128 shifts):
+                device-resident solve time and its SpMVs' bandwidth.
 
 --mode sweep   shift counts 1, 2, .. 1024 x {normal, dynamic} on a 64^3
                operator with the reference sweep driver's conventions
@@ -87,6 +92,10 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device copy (counter calibration under rocprofv3)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the GPU COO->CSR ingest leg")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) drop-in call of the headline workload")
+    ap.add_argument("--no-powerlaw", action="store_true",
+                    help="skip the 1e6-row power-law matrix leg (128 shifts)")
     ap.add_argument("--mode", default="headline", choices=["headline", "sweep", "repeat"],
                     help="headline: the 128^3 / 64-shift solve and its legs; sweep: shift counts "
                          "1..--sweep-max x {normal, dynamic} (src/main_seed_diff.c); repeat: one "
@@ -207,6 +216,76 @@ def ingest_leg(args, log):
                                          f"restated) on the same {nz} entries: {tc:.2f} s"}
     log(f"ingest: {t * 1e3:.2f} ms for {nz} entries, ptr matches {ok}")
     return out
+
+
+def dropin_leg(args, prob, log):
+    """The headline workload through the reference's entry point
+    shifted_lopbicg_dynamic on host arrays (src/solver.h:33-35): plan set-up
+    (upload of A, value-table layout), upload of b and x_loc_set, the solve,
+    and the download of x_loc_set and r, as hbicg_report times them.  This
+    is the PCIe-inclusive rate; the headline value is the device-resident
+    solve."""
+    from hbicg import dropin_solve, make_params
+    p = make_params(tol=prob.tol, max_iter=prob.max_iter, value_index=bool(args.value_index))
+    best = None
+    for _ in range(2):
+        res = dropin_solve(prob.A, prob.b, prob.sigma, prob.seed, "dynamic", p)
+        rep = res.report
+        if best is None or rep["t_total_ms"] < best["t_total_ms"]:
+            best = rep
+    log(f"drop-in (host buffers): {best['t_total_ms']:.1f} ms total, {best['t_solve_ms']:.1f} ms solve")
+    return {"t_setup_ms": round(best["t_setup_ms"], 2), "t_solve_ms": round(best["t_solve_ms"], 2),
+            "t_total_ms": round(best["t_total_ms"], 2),
+            "shift_solves_per_s": round(prob.M / (best["t_total_ms"] * 1e-3), 2), "iterations": best["k"] - 1,
+            "note": "one shifted_lopbicg_dynamic call on host arrays, best of 2: plan set-up, "
+                    "uploads, solve, downloads (x_loc_set is 1.07 GB each way)"}
+
+
+def powerlaw_leg(args, log):
+    """The irregular matrix: n = 1e6 rows with power-law lengths on
+    [1, 2000] (31M nonzeros, values uniform, so no value table), 128 shifts
+    uniform in [0, 0.5], relative residual 1e-10, cap 2000 iterations.  The
+    smallest shifts do not reach the rule within the cap, so the solve runs
+    the cap, like the reference.  Device-resident timing, plus the SpMV
+    kernels' bandwidth from a profiled solve."""
+    import torch
+
+    from hbicg import Plan, make_params
+    from hbicg import problems as P
+    t = time.perf_counter()
+    prob = P.config2()
+    log(f"power-law matrix generated in {time.perf_counter() - t:.1f} s")
+    dev = torch.device("cuda:0")
+    b = torch.as_tensor(prob.b).to(dev)
+    x = torch.zeros((prob.M, prob.A.n), dtype=torch.float64, device=dev)
+    pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=prob.max_iter, graph=1), device=dev)
+    pl.solve(b, prob.sigma, prob.seed, x)
+    torch.cuda.synchronize()
+    x.zero_()
+    t = time.perf_counter()
+    k, _, rep = pl.solve(b, prob.sigma, prob.seed, x)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3
+    pl.close()
+    pp = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=prob.max_iter, graph=0, profile=True),
+              device=dev)
+    xp = torch.zeros_like(x)
+    pp.solve(b, prob.sigma, prob.seed, xp)
+    ks = pp.kernel_stats()
+    pp.close()
+    spmv = {}
+    for name in ("spmv_seed", "spmv_q_fused"):
+        kd = ks[name]
+        avg = kd["replay_ms"] / kd["replays"] if kd.get("replays") else kd["total_ms"] / max(kd["launches"], 1)
+        per = kd["bytes"] / max(kd["launches"], 1)
+        spmv[name] = {"avg_launch_ms": round(avg, 5), "algorithmic_bytes_per_launch": per,
+                      "GBps": round(per / (avg * 1e-3) / 1e9, 1),
+                      "frac": round(per / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+    log(f"power-law: {ms:.1f} ms for {k - 1} iterations, {rep['stop_count']}/{prob.M} shifts stopped")
+    return {"workload": f"power-law rows on [1,2000] (n={prob.A.n}, nnz={prob.A.nnz}), {prob.M} shifts "
+                        f"uniform in [0,0.5], relres {prob.tol:g}, cap {prob.max_iter}",
+            "ms_per_solve": round(ms, 2), "iterations": k - 1, "ms_per_iteration": round(ms / (k - 1), 4),
+            "shifts_stopped": rep["stop_count"], "value_table": rep["value_table"], "spmv": spmv}
 
 
 def _driver_problem(n_side, M):
@@ -531,6 +610,12 @@ def main():
     ingest = None
     if not args.no_ingest:
         ingest = ingest_leg(args, log)
+    dropin = None
+    if not args.no_dropin:
+        dropin = dropin_leg(args, prob, log)
+    powerlaw = None
+    if not args.no_powerlaw:
+        powerlaw = powerlaw_leg(args, log)
 
     line = {
         "metric": METRIC,
@@ -568,6 +653,8 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "ingest": ingest,
+        "dropin_host_buffers": dropin,
+        "powerlaw": powerlaw,
         "gen_s": round(t_gen, 2),
     }
     print(json.dumps(line))

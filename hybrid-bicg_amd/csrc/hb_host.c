@@ -43,6 +43,7 @@ void hbicg_default_params(hbicg_params *p)
     p->growth_limit = 0.0;
     p->trace = 0;
     p->value_index = 1;
+    p->q_mode = 0;
 }
 
 static const hbicg_params *params(void)
@@ -57,7 +58,7 @@ static const hbicg_params *params(void)
 int hbicg_set_params(const hbicg_params *p)
 {
     if (!p || !(p->tol > 0.0) || p->max_iter < 1 || p->window < 0 || p->window > HB_WMAX ||
-        p->chunk < 0)
+        p->chunk < 0 || p->q_mode < 0 || p->q_mode > 2)
         return HBICG_E_ARG;
     g_params = *p;
     g_params_set = 1;
@@ -453,7 +454,8 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     unsigned *hptr = NULL, *hblk = NULL;
     hbicg_plan *pl = NULL;
     g_errno = 0;
-    if (!A || !A->ptr || !p || M <= 0 || A->n == 0 || p->window < 0 || p->window > HB_WMAX) {
+    if (!A || !A->ptr || !p || M <= 0 || A->n == 0 || p->window < 0 || p->window > HB_WMAX ||
+        p->q_mode < 0 || p->q_mode > 2) {
         g_errno = HBICG_E_ARG;
         return NULL;
     }
@@ -617,11 +619,17 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
             rc = HBICG_E_HIP;
             goto fail;
         }
-        double cb = 4.0 * (double)pl->nblk;
-        for (int t = 0; t < pl->nblk; ++t)
-            cb += (hf[t] ? 2.0 : 4.0) * (double)(hptr[hblk[t + 1]] - hptr[hblk[t]]);
+        double cb = 4.0 * (double)pl->nblk, narrow_nz = 0.0;
+        for (int t = 0; t < pl->nblk; ++t) {
+            const double len = (double)(hptr[hblk[t + 1]] - hptr[hblk[t]]);
+            cb += (hf[t] ? 2.0 : 4.0) * len;
+            if (hf[t]) narrow_nz += len;
+        }
         pl->col_bytes = cb;
         free(hf);
+        /* columns scattered beyond 16-bit offsets for most entries: SpMV 2
+         * gathers q formed beforehand rather than r and s at every entry */
+        d->q_unfused = !pl->exact && (p->q_mode == 2 || (p->q_mode == 0 && 2.0 * narrow_nz < (double)pl->nz));
     }
 
     pl->val_bytes = 8.0 * (double)pl->nz;
@@ -860,6 +868,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->window = pl->W;
     R->value_table = pl->dev.nvdict;
     R->diag_tiles = pl->n_diag_tiles;
+    R->q_unfused = pl->dev.q_unfused;
     R->dot_r = st.dot_r;
     R->dot_zero = st.dot_zero;
     R->rTr = st.rTr;
@@ -877,7 +886,10 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
          * flushes) */
         const double fin_spmv = 16.0 * pl->nblk, fin_seed = 16.0 * pl->dev.vec_grid;
         const double seed_upd = (pl->exact ? 56.0 : 32.0) * n;
-        const double per_launch[HB_NSTAGE] = {spmv + 24.0 * n, fin_spmv, spmv + 32.0 * n, fin_spmv,
+        /* SpMV 2 with q formed beforehand: r, s in and q out (k_qform), q
+         * gathered, y out */
+        const double spmv2 = spmv + (pl->dev.q_unfused ? 40.0 : 32.0) * n;
+        const double per_launch[HB_NSTAGE] = {spmv + 24.0 * n, fin_spmv, spmv2, fin_spmv,
                                               seed_upd, fin_seed, 0.0, 0.0};
         const int iters = st.k - 1;
         for (int sg = 0; sg < HB_NSTAGE; ++sg) {

@@ -139,7 +139,7 @@ DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
  * gathers r_{k-1} (in rring) and s, PLAIN gathers x. */
 enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 
-template <int MODE, bool EXACT, bool VI = false>
+template <int MODE, bool EXACT, bool VI = false, bool UNF = false>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ col, const short *__restrict__ col16,
                    const unsigned *__restrict__ cflag, const unsigned *__restrict__ ptr,
@@ -184,16 +184,17 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             zscale = zs;
             snap = !d.exact && k == wk0;
         } else {
-            xa = rprev;
             nalpha = -d.aa[k];
             qout = qring + off(k % d.W, n);
+            /* UNF: q was formed for every row by k_qform; gather it */
+            xa = UNF ? qout : rprev;
         }
     } else {
         sig = shift;
         xa = xbase;
     }
     auto xval = [&](unsigned c) -> double {
-        if (MODE == SP_SEED2) return xa[c] + nalpha * svec[c];
+        if (MODE == SP_SEED2 && !UNF) return xa[c] + nalpha * svec[c];
         return xa[c];
     };
 
@@ -220,8 +221,9 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                 part1 += bb * rr;
             }
         } else if (MODE == SP_SEED2) {
-            const double qi = a + nalpha * bb;
-            qout[i] = qi;
+            /* UNF: a is q_i, formed by k_qform as r_i + (-alpha) s_i */
+            const double qi = UNF ? a : a + nalpha * bb;
+            if (!UNF) qout[i] = qi;
             const double yi = t + sig * qi;
             out1[i] = yi;
             part0 += qi * qi;
@@ -236,8 +238,8 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             bb = rhat[i];
             c = switched ? rprev[i] : 0.0;
         } else if (MODE == SP_SEED2) {
-            a = rprev[i];
-            bb = svec[i];
+            a = xa[i];
+            bb = UNF ? 0.0 : svec[i];
         } else {
             a = xa[i];
         }
@@ -399,7 +401,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
 
             constexpr int U = HB_NNZ_CAP / HB_BLOCK;
             static_assert(U * HB_BLOCK == HB_NNZ_CAP, "tile capacity");
-            if (MODE == SP_SEED2 && T == 1 && narrow && N > 0) {
+            if (MODE == SP_SEED2 && !UNF && T == 1 && narrow && N > 0) {
                 /* One row per thread.  Values and 16-bit column offsets are
                  * staged in LDS, then each thread walks its own row: the lanes of
                  * a wave gather the same stencil slot of consecutive rows, i.e.
@@ -439,7 +441,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                 }
                 return;
             }
-            if (MODE == SP_SEED2) {
+            if (MODE == SP_SEED2 && !UNF) {
                 /* two gathers per entry (r and s): entries in rounds of four */
                 if (narrow) {
     #pragma unroll 4
@@ -532,7 +534,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
     if (tid == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
 }
 
-template <int MODE, bool EXACT, bool VI = false>
+template <int MODE, bool EXACT, bool VI = false, bool UNF = false>
 DEV void spmv_tile(const hb_dev &d, unsigned b)
 {
     if (MODE == SP_SEED1)
@@ -540,7 +542,7 @@ DEV void spmv_tile(const hb_dev &d, unsigned b)
                                    d.rhat, d.s, nullptr, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
                                    d.doff, d.didx);
     else
-        spmv_body<MODE, EXACT, VI>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
+        spmv_body<MODE, EXACT, VI, UNF>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
                                    d.y, d.qring, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile, d.doff,
                                    d.didx);
 }
@@ -556,10 +558,28 @@ DEV unsigned xcd_tile(unsigned b, unsigned nb)
     return x * q + min(x, r) + b / 8;
 }
 
-template <int MODE, bool EXACT, bool VI>
+template <int MODE, bool EXACT, bool VI, bool UNF = false>
 __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
 {
-    spmv_tile<MODE, EXACT, VI>(d, xcd_tile(blockIdx.x, gridDim.x));
+    spmv_tile<MODE, EXACT, VI, UNF>(d, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+/* q = r_{k-1} - alpha~ s for every row (matrices whose SpMV 2 would gather r
+ * and s at scattered columns: it then gathers q alone).  The same expression
+ * as the fused form, so q and everything after it are bitwise the same.
+ * src/solver_dynamic.c:171-176. */
+__global__ __launch_bounds__(HB_BLOCK) void k_qform(hb_dev d)
+{
+    const hb_state *st = d.st;
+    const int done = st->done, k = st->k;
+    if (done | (k < 1)) return;
+    const int n = d.n;
+    const double nalpha = -d.aa[k];
+    const double *__restrict__ r = d.rring + off((k - 1) % (d.W + 1), n);
+    const double *__restrict__ s = d.s;
+    double *__restrict__ q = d.qring + off(k % d.W, n);
+    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < n; i += gridDim.x * HB_BLOCK)
+        q[i] = r[i] + nalpha * s[i];
 }
 
 /* p[0 .. count) = v (a fill kernel: usable where a memset cannot express v) */
@@ -1789,7 +1809,11 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
         else hipLaunchKernelGGL((k_fin_spmv<SP_SEED1, false>), one, bl, 0, s, *d);
         break;
     case HB_STAGE_SPMV2:
-        if (d->exact) {
+        if (d->q_unfused && !d->exact) {
+            hipLaunchKernelGGL(k_qform, gv, bl, 0, s, *d);
+            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true, true>), gs, bl, 0, s, *d);
+            else hipLaunchKernelGGL((k_spmv<SP_SEED2, false, false, true>), gs, bl, 0, s, *d);
+        } else if (d->exact) {
             if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, true, true>), gs, bl, 0, s, *d);
             else hipLaunchKernelGGL((k_spmv<SP_SEED2, true, false>), gs, bl, 0, s, *d);
         } else {

@@ -81,6 +81,9 @@ typedef struct hb_state {
 typedef struct hb_dev {
     int n, M, W, cap, nblk;    /* rows, shifts, window, archive length, SpMV workgroups */
     int exact;                 /* 1: reference summation order everywhere  */
+    int q_unfused;             /* 1: SpMV 2 gathers q, formed beforehand by k_qform
+                                  (matrices with scattered columns); 0: it forms
+                                  q = r - alpha s at every gathered column       */
     int vec_grid;              /* workgroups of the elementwise kernels    */
     int shift_gx, shift_gy;    /* grid of the shift kernel                 */
     int shift_group;           /* shifts per shift-kernel workgroup row    */

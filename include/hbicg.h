@@ -57,6 +57,11 @@ typedef struct hbicg_params {
                             byte per row and diagonal (no columns, no row pointers); the
                             products and their order are those of the CSR read, so results
                             are bitwise unchanged.  0: 8-byte CSR values                   */
+    int    q_mode;       /* SpMV 2 input q = r - alpha~ s (default mode only): 0 (default)
+                            auto: formed at every gathered column when most entries lie
+                            within 16-bit column offsets of their tile (stencils), else
+                            formed for every row first and gathered; 1: always at the
+                            gathered columns; 2: always beforehand.  Same results.        */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -73,6 +78,9 @@ typedef struct hbicg_report {
     int    window;       /* shift-update window the solve ran with             */
     int    value_table;  /* distinct values the SpMV read through its table (0: none) */
     int    diag_tiles;   /* SpMV tiles stored by diagonals (value table only)         */
+    int    q_unfused;    /* 1: most entries' columns lie beyond 16-bit offsets of their
+                            tile, so SpMV 2 gathered q = r - alpha s formed beforehand
+                            (one gather per entry instead of two); results unchanged */
 } hbicg_report;
 
 void hbicg_default_params(hbicg_params *p);

@@ -49,11 +49,14 @@ def test_params_roundtrip():
     L = _lib.load()
     p = _lib.default_params()
     assert p.tol == 1e-12 and p.max_iter == 2000 and p.exit_on_error == 1
-    q = make_params(tol=1e-10, max_iter=77, window=4, exact_order=True)
+    assert p.value_index == 1 and p.q_mode == 0
+    q = make_params(tol=1e-10, max_iter=77, window=4, exact_order=True, q_mode=2)
     assert L.hbicg_set_params(C.byref(q)) == 0
     r = _lib.Params()
     assert L.hbicg_get_params(C.byref(r)) == 0
-    assert (r.tol, r.max_iter, r.window, r.exact_order) == (1e-10, 77, 4, 1)
+    assert (r.tol, r.max_iter, r.window, r.exact_order, r.q_mode) == (1e-10, 77, 4, 1, 2)
+    bad = make_params(q_mode=3)
+    assert L.hbicg_set_params(C.byref(bad)) == -1
     bad = make_params(window=1025)
     assert L.hbicg_set_params(C.byref(bad)) == -1
     bad = make_params(tol=-1.0)

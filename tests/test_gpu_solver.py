@@ -324,6 +324,38 @@ def test_value_table_256_values_csr_tiles():
     assert a.k == c.k and np.array_equal(a.x, c.x)
 
 
+@pytest.mark.parametrize("which", ["stencil", "scattered"])
+def test_q_formed_before_or_at_gather_bitwise(c0, which):
+    """SpMV 2's input q = r - alpha s formed at every gathered column (q_mode
+    1) or for every row first and gathered (q_mode 2): the same expression,
+    so every result is bitwise equal.  Auto (q_mode 0) forms it at the
+    gathered columns for the stencil (16-bit column offsets) and beforehand
+    for a power-law matrix with columns spread uniformly (n = 100000, rows of
+    1 to 2000 entries, some longer than one SpMV tile; most entries lie
+    beyond 16-bit offsets), which here is capped at 40 iterations (it
+    converges slowly) and also compared with the oracle at 12 iterations."""
+    if which == "stencil":
+        A, b, sigma, seed, cap, auto = c0.A, c0.b, c0.sigma, 8, 2000, 0
+    else:
+        pr = P.config2(n=100000, m=8)
+        A, b, sigma, seed, cap, auto = pr.A, pr.b, pr.sigma, pr.seed, 40, 1
+    print(f"case q_mode {which}", flush=True)
+    mk = lambda qm, it=cap: make_params(tol=1e-10, max_iter=it, q_mode=qm)
+    r0 = dropin_solve(A, b, sigma, seed, "dynamic", mk(0))
+    r1 = dropin_solve(A, b, sigma, seed, "dynamic", mk(1))
+    r2 = dropin_solve(A, b, sigma, seed, "dynamic", mk(2))
+    assert (r0.report["q_unfused"], r1.report["q_unfused"], r2.report["q_unfused"]) == (auto, 0, 1)
+    for r in (r1, r2):
+        assert r.k == r0.k and r.switches == r0.switches
+        assert np.array_equal(r.x, r0.x) and np.array_equal(r.r, r0.r)
+    if which == "scattered":
+        ref = O.solve(A, b, sigma, seed, tol=1e-10, max_iter=12)
+        got = dropin_solve(A, b, sigma, seed, "dynamic", mk(0, 12))
+        assert got.k == ref.k
+        rel = np.linalg.norm(got.x - ref.x, axis=1) / np.linalg.norm(ref.x, axis=1)
+        assert rel.max() <= 1e-9, rel.max()
+
+
 def test_value_table_not_used_for_many_values():
     """More than 256 distinct values: the SpMV reads the 8-byte values."""
     rng = np.random.default_rng(3)
