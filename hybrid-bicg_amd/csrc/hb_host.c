@@ -1235,7 +1235,7 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     size_t nsp = 0, nvec = 0, ncoef = 0;
     const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WBATCH;
     for (int q = 0; q < nprob; ++q) {
-        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax, 0);
+        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax, p->value_index);
         if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
         bt->hdevs[q] = bt->plans[q]->dev;
         nsp += (size_t)bt->plans[q]->nblk;
@@ -1277,6 +1277,11 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     HIPCK(hipEventCreate(&bt->t1));
     bt->B.nprob = nprob;
     bt->B.exact = bt->exact;
+    /* the value-table SpMV when every problem has a table (one kernel per
+     * launch serves all problems) */
+    bt->B.vi = 1;
+    for (int q = 0; q < nprob; ++q)
+        if (!bt->plans[q]->dev.vidx) bt->B.vi = 0;
     bt->B.devs = bt->ddevs;
     bt->B.t_spmv = bt->t_spmv;
     bt->B.t_vec = bt->t_vec;

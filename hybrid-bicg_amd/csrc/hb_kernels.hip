@@ -1664,7 +1664,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_resid_final(const double *part, in
  * Launch tile t of a phase works on problem tab[2t] tile tab[2t+1]; finalize
  * launches use workgroup p for problem p.  Problems that are done return at
  * once inside the bodies. */
-template <int MODE, bool EXACT>
+template <int MODE, bool EXACT, bool VI = false>
 __global__ __launch_bounds__(HB_BLOCK) void kb_spmv(const hb_dev *__restrict__ devs,
                                                     const int *__restrict__ tab)
 {
@@ -1673,7 +1673,7 @@ __global__ __launch_bounds__(HB_BLOCK) void kb_spmv(const hb_dev *__restrict__ d
      * order measured 14% slower on the 256-problem batch) */
     const unsigned t = blockIdx.x;
     const hb_dev &d = devs[tab[2 * t]];
-    spmv_tile<MODE, EXACT>(d, tab[2 * t + 1]);
+    spmv_tile<MODE, EXACT, VI>(d, tab[2 * t + 1]);
 }
 
 template <int MODE, bool EXACT>
@@ -1927,20 +1927,28 @@ int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s)
     const dim3 bl(HB_BLOCK), gp(B->nprob);
     switch (stage) {
     case HB_STAGE_SPMV1:
-        if (B->exact)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED1, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        if (B->exact && B->vi)
+            hipLaunchKernelGGL((kb_spmv<SP_SEED1, true, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        else if (B->exact)
+            hipLaunchKernelGGL((kb_spmv<SP_SEED1, true, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        else if (B->vi)
+            hipLaunchKernelGGL((kb_spmv<SP_SEED1, false, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
         else
-            hipLaunchKernelGGL((kb_spmv<SP_SEED1, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+            hipLaunchKernelGGL((kb_spmv<SP_SEED1, false, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
         break;
     case HB_STAGE_FIN1:
         if (B->exact) hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, true>), gp, bl, 0, s, B->devs);
         else hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, false>), gp, bl, 0, s, B->devs);
         break;
     case HB_STAGE_SPMV2:
-        if (B->exact)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED2, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        if (B->exact && B->vi)
+            hipLaunchKernelGGL((kb_spmv<SP_SEED2, true, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        else if (B->exact)
+            hipLaunchKernelGGL((kb_spmv<SP_SEED2, true, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        else if (B->vi)
+            hipLaunchKernelGGL((kb_spmv<SP_SEED2, false, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
         else
-            hipLaunchKernelGGL((kb_spmv<SP_SEED2, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+            hipLaunchKernelGGL((kb_spmv<SP_SEED2, false, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
         break;
     case HB_STAGE_FIN2:
         if (B->exact) hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, true>), gp, bl, 0, s, B->devs);

@@ -142,6 +142,7 @@ typedef struct hb_dev {
  * launch covers the tiles of every problem. */
 typedef struct hb_batch {
     int nprob, exact, W;
+    int vi;                    /* 1: every problem's SpMV reads its value table  */
     int n_spmv_tiles, n_vec_tiles, n_shift_tiles, n_coef_tiles;
     const hb_dev *devs;        /* [nprob] device array                     */
     const int *t_spmv;         /* [2*n_spmv_tiles]  (problem, SpMV tile)   */
