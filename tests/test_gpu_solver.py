@@ -356,6 +356,29 @@ def test_q_formed_before_or_at_gather_bitwise(c0, which):
         assert rel.max() <= 1e-9, rel.max()
 
 
+def test_value_table_with_q_formed_first_bitwise():
+    """A scattered-column matrix (power-law rows, n = 100000) whose values
+    are rounded to 1/8, the diagonal one constant (18 distinct values): its
+    SpMV reads the value table in CSR tiles, and SpMV 2 gathers q formed
+    beforehand.  Bitwise equal to the 8-byte read and to forming q at the
+    gathered columns."""
+    pr = P.config2(n=100000, m=8)
+    A = pr.A
+    ptr = np.asarray(A.ptr, np.int64)
+    rows = np.repeat(np.arange(A.n), np.diff(ptr))
+    val = np.round(np.asarray(A.val) * 8.0) / 8.0
+    diag = np.asarray(A.col) == rows
+    val[diag] = 2400.0   # above every row's off-diagonal sum (rows of at most 2000 entries)
+    A = P.CSR(ptr=A.ptr, col=A.col, val=val, n=A.n)
+    print(f"case value table + q formed first, {len(np.unique(val))} values", flush=True)
+    mk = lambda vi, qm: make_params(tol=1e-10, max_iter=40, value_index=vi, q_mode=qm)
+    a = dropin_solve(A, pr.b, pr.sigma, pr.seed, "dynamic", mk(True, 0))
+    b = dropin_solve(A, pr.b, pr.sigma, pr.seed, "dynamic", mk(False, 1))
+    assert a.report["value_table"] > 0 and a.report["q_unfused"] == 1
+    assert b.report["value_table"] == 0 and b.report["q_unfused"] == 0
+    assert a.k == b.k and np.array_equal(a.x, b.x) and np.array_equal(a.r, b.r)
+
+
 def test_value_table_not_used_for_many_values():
     """More than 256 distinct values: the SpMV reads the 8-byte values."""
     rng = np.random.default_rng(3)
