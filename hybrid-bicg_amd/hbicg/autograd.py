@@ -105,8 +105,11 @@ _transposes = {}
 
 
 def _plan_for(val, col, ptr, M, tol, max_iter):
+    """A cached plan that reads val in place (value_index=0: a value table
+    would hold the values of the plan's creation, and val changes in place
+    between calls, e.g. under an optimiser; so does the transposed copy)."""
     key = (val.data_ptr(), col.data_ptr(), ptr.data_ptr(), int(val.numel()), M, tol, max_iter)
-    params = make_params(tol=tol, max_iter=max_iter, graph=1)
+    params = make_params(tol=tol, max_iter=max_iter, graph=1, value_index=False)
     return _plans.get(key, lambda: Plan(None, M, params, device=val.device, tensors=(val, col, ptr)))
 
 
@@ -159,7 +162,7 @@ class ShiftedSolve(torch.autograd.Function):
         M = x.shape[0]
         gx = gx.contiguous()
         tval, tcol, tptr = _transpose_into(vd, col, ptr)
-        params = make_params(tol=ctx.adj_tol, max_iter=ctx.max_iter, graph=1)
+        params = make_params(tol=ctx.adj_tol, max_iter=ctx.max_iter, graph=1, value_index=False)
         split = collinear_split(gx)
         if split is not None:
             w, c = split

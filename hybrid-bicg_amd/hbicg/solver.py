@@ -105,7 +105,10 @@ class Plan:
     def __init__(self, A, M, params=None, device="cuda", stream=None, tensors=None):
         """A: problems.CSR on the host, or None with tensors=(val, col, ptr)
         already on the device (float64, int32, int32; col/ptr hold the
-        unsigned 32-bit indices)."""
+        unsigned 32-bit indices).  With params.value_index (the default) a
+        qualifying matrix is read through a value table built here: values
+        changed in place later need a new plan, or value_index=False, which
+        reads val at every solve."""
         import torch
         self._torch = torch
         L = _lib.load()

@@ -116,6 +116,10 @@ typedef struct hbicg_plan hbicg_plan;
 
 /* Uploads A (a_on_device = 0) or references caller device memory (1),
  * bins the rows for the SpMV and allocates the workspace for M shifts.
+ * With params.value_index = 1 and a matrix that qualifies, the solver's
+ * SpMV reads the value table built here, so values the caller changes in
+ * its device memory afterwards need a new plan (or value_index = 0, which
+ * reads A in place at every solve).
  * stream: a hipStream_t, or NULL for a stream of the plan's own.
  * Returns NULL on failure (hbicg_errno() tells why). */
 hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,

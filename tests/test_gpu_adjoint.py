@@ -137,3 +137,30 @@ def test_adjoint_solutions_match_oracle_transposed_solves(prob):
         ref += w0[k] * r.x[0]
     got = b.grad.cpu().numpy()
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-8
+
+
+def test_values_changed_in_place_between_calls(prob):
+    """The differentiable solve keeps its plans across calls and reads A's
+    values in place: after an in-place update of val (as an optimiser
+    makes), the forward solution and the gradient w.r.t. b (one adjoint
+    solve on A^T) equal those of a fresh copy of the updated values."""
+    A, b, sigma, w, c = prob
+    val, col, ptr = dev_csr(A)
+    bt = torch.as_tensor(b).cuda()
+    st = torch.as_tensor(sigma)
+    wt, ct = torch.as_tensor(w).cuda(), torch.as_tensor(c).cuda()
+
+    def run(v):
+        bb = bt.clone().requires_grad_(True)
+        L, x = loss_collinear(v, col, ptr, bb, st, wt, ct)
+        L.backward()
+        return x.detach().cpu().numpy(), bb.grad.cpu().numpy()
+
+    run(val)                                  # plans cached for val's memory
+    with torch.no_grad():
+        val.mul_(1.25)                        # same memory, new values
+    x1, g1 = run(val)
+    fresh = val.clone()                       # other memory: new plans
+    x2, g2 = run(fresh)
+    assert np.array_equal(x1, x2)
+    assert np.array_equal(g1, g2)
