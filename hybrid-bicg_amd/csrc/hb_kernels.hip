@@ -1507,14 +1507,35 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
     const hb_state *st = d.st;
     if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     const int n = d.n, W = d.W;
-    const int i0 = bx * HB_BLOCK + threadIdx.x;
-    if (i0 < n) {
-        double *ps = pset + off(st->prev_seed, n) + i0;
-        const double pb = st->pb, pbw = st->pbw;
-        const double r = rring[off(st->kprev % (W + 1), n) + i0];
-        double pe = pb * *ps;
-        pe = pe + r;
-        *ps = pe + pbw * svec[i0];
+    const double pb = st->pb, pbw = st->pbw;
+    if (!(n & 1)) {
+        /* even n (every vector 16-byte aligned): workgroup bx updates rows
+         * 512 bx .. 512 bx + 511, two per thread with 16-byte accesses; the
+         * flush below never reads this p (the seed's window-start p is the
+         * snapshot) */
+        const int i2 = 2 * (bx * HB_BLOCK + (int)threadIdx.x);
+        if (i2 < n) {
+            dbl2 *ps = (dbl2 *)(pset + off(st->prev_seed, n) + i2);
+            const dbl2 r = *(const dbl2 *)(rring + off(st->kprev % (W + 1), n) + i2);
+            const dbl2 sv = *(const dbl2 *)(svec + i2);
+            dbl2 p = *ps;
+            double pe = pb * p.x;
+            pe = pe + r.x;
+            p.x = pe + pbw * sv.x;
+            pe = pb * p.y;
+            pe = pe + r.y;
+            p.y = pe + pbw * sv.y;
+            *ps = p;
+        }
+    } else {
+        const int i0 = bx * HB_BLOCK + threadIdx.x;
+        if (i0 < n) {
+            double *ps = pset + off(st->prev_seed, n) + i0;
+            const double r = rring[off(st->kprev % (W + 1), n) + i0];
+            double pe = pb * *ps;
+            pe = pe + r;
+            *ps = pe + pbw * svec[i0];
+        }
     }
     if (!st->flush) return;
     if (st->flush_xonly) flush_apply_x2<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
