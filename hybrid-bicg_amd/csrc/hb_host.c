@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <limits.h>
 #include <string.h>
 #include <time.h>
 
@@ -347,7 +348,10 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     for (int t = 0; t < nb; ++t) {
         const unsigned r0 = hblk[t], r1 = hblk[t + 1];
         const size_t R = r1 - r0, N = hptr[r1] - hptr[r0];
-        int k = nd < HB_VDICT ? tile_diagonals(hptr, hcol, r0, r1, hoff + (size_t)HB_DIAG_K * t) : 0;
+        /* row + diagonal of a missing entry stays within HB_ROWS_CAP of
+         * [0, n): the kernel forms it in int */
+        const int dia_ok = nd < HB_VDICT && (long long)pl->n + HB_ROWS_CAP < (long long)INT_MAX;
+        int k = dia_ok ? tile_diagonals(hptr, hcol, r0, r1, hoff + (size_t)HB_DIAG_K * t) : 0;
         const size_t KP = k <= 8 ? 8 : 16;
         if (k > 0 && KP * R > 3 * N) k = 0;
         htp[t] = (unsigned)P;
