@@ -29,6 +29,9 @@ Printed: one JSON line with the contract fields plus
                 the 256-problem batch against per-problem plain solves, the
                 forward + adjoint solve of the 64^3 / 32-shift loss, and GPU
                 COO -> CSR of the 128^3 operator's shuffled entries;
+  attainable_copy
+                a 1 GiB device copy timed on the same box: the HBM bandwidth a
+                plain stream reaches here, beside the 8 TB/s peak;
   dropin_host_buffers
                 the headline workload through shifted_lopbicg_dynamic on host
                 arrays: set-up, PCIe transfers and solve (never the value);
@@ -464,6 +467,24 @@ def main():
         del src, dst
         log("copy probe: 3 copies of 1 GiB done")
 
+    # attainable HBM bandwidth on this box: a 1 GiB device-to-device copy
+    # (read + write), best of 5, HIP events on the current stream
+    csrc = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    cdst = torch.empty_like(csrc)
+    cdst.copy_(csrc)
+    best_copy = None
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        cdst.copy_(csrc)
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1)
+        best_copy = t if best_copy is None else min(best_copy, t)
+    copy_gbps = 2.0 * csrc.numel() * 8 / (best_copy * 1e-3) / 1e9
+    del csrc, cdst
+    log(f"attainable copy: {copy_gbps:.0f} GB/s")
+
     t_gen = time.perf_counter()
     prob = P.config1(grid=args.grid, m=args.shifts)
     t_gen = time.perf_counter() - t_gen
@@ -647,6 +668,10 @@ def main():
                     "of every active shift read and written each iteration) must move for this "
                     "solve, divided by the whole solve time measured here"},
         "roofline": roofline,
+        "attainable_copy": {"GBps": round(copy_gbps, 1), "frac_of_peak": round(copy_gbps / HBM_PEAK_GBPS, 4),
+                            "roofline_achieved_over_copy": None if roofline is None else
+                            round(roofline["achieved"] / copy_gbps, 4),
+                            "note": "1 GiB device copy (torch copy_, read + write bytes), best of 5"},
         "window_flush_roofline": flush_roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
