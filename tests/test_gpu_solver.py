@@ -303,6 +303,29 @@ def test_value_table_diagonal_tiles(exact):
         assert a.k == ref.k and np.array_equal(a.x, ref.x) and np.array_equal(a.r, ref.r)
 
 
+def test_value_table_diagonal_tiles_with_empty_rows(c0):
+    """Rows with no entries at all inside diagonal tiles (every index byte
+    marks a missing diagonal): bitwise equal to the 8-byte read, and in
+    exact order to the oracle."""
+    A = c0.A
+    ptr = np.asarray(A.ptr, np.int64)
+    rows = [list(zip(A.col[ptr[i]:ptr[i + 1]].tolist(), A.val[ptr[i]:ptr[i + 1]].tolist()))
+            for i in range(A.n)]
+    for i in (5, 300, 301, 1000, 2047, 4095):
+        rows[i] = []
+    A = _rows_csr(rows, A.n)
+    print("case diagonal tiles with empty rows", flush=True)
+    for exact in (False, True):
+        mk = lambda vi: make_params(tol=c0.tol, exact_order=exact, value_index=vi)
+        a = dropin_solve(A, c0.b, c0.sigma, 0, "dynamic", mk(True))
+        b = dropin_solve(A, c0.b, c0.sigma, 0, "dynamic", mk(False))
+        assert a.report["diag_tiles"] > 0
+        assert a.k == b.k and np.array_equal(a.x, b.x) and np.array_equal(a.r, b.r)
+        if exact:
+            ref = O.solve(A, c0.b, c0.sigma, 0, tol=c0.tol)
+            assert a.k == ref.k and np.array_equal(a.x, ref.x)
+
+
 def test_value_table_256_values_csr_tiles():
     """256 distinct values fill the table; no index byte is left to mark a
     missing diagonal, so every tile stays in CSR form, bitwise equal to
