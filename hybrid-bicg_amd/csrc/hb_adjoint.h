@@ -12,7 +12,8 @@ extern "C" {
 #endif
 
 /* A^T of an n x n CSR on the device; rows of A^T list their entries in
- * ascending column order.  work: 2*n + ceil(n/256) unsigned. */
+ * ascending column order, duplicate entries in their order in A (a stable
+ * transpose).  work: 2*n + ceil(n/256) + nnz unsigned. */
 int hb_transpose(int n, unsigned nnz, const unsigned *ptr, const unsigned *col, const double *val,
                  unsigned *tptr, unsigned *tcol, double *tval, unsigned *work, hipStream_t s);
 

@@ -91,13 +91,15 @@ __global__ __launch_bounds__(TB) void k_scan_final(const unsigned *__restrict__ 
 }
 
 /* one thread per row of A; entries land in their column's segment of A^T
- * in arbitrary order (sorted afterwards) */
+ * in arbitrary order, each with its source entry index e (sorted
+ * afterwards) */
 __global__ __launch_bounds__(TB) void k_scatter(const unsigned *__restrict__ ptr,
                                                 const unsigned *__restrict__ col,
                                                 const double *__restrict__ val, int n,
                                                 unsigned *__restrict__ fill,
                                                 unsigned *__restrict__ tcol,
-                                                double *__restrict__ tval)
+                                                double *__restrict__ tval,
+                                                unsigned *__restrict__ tsrc)
 {
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
@@ -105,29 +107,36 @@ __global__ __launch_bounds__(TB) void k_scatter(const unsigned *__restrict__ ptr
         const unsigned pos = atomicAdd(&fill[col[e]], 1u);
         tcol[pos] = (unsigned)i;
         tval[pos] = val[e];
+        tsrc[pos] = e;
     }
 }
 
-/* insertion sort of each row of A^T by column: the rows come out in
- * ascending source-row order, the order a stable transpose produces */
+/* insertion sort of each row of A^T by source entry index e: entries of A
+ * are numbered row by row, so the row of A^T lists its entries by ascending
+ * source row and, for duplicate entries of one row, in their CSR order --
+ * the order a stable transpose produces, whatever order the scatter's
+ * atomics placed them in */
 __global__ __launch_bounds__(TB) void k_sort_rows(const unsigned *__restrict__ tptr, int n,
                                                   unsigned *__restrict__ tcol,
-                                                  double *__restrict__ tval)
+                                                  double *__restrict__ tval,
+                                                  unsigned *__restrict__ tsrc)
 {
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     const unsigned lo = tptr[i], hi = tptr[i + 1];
     for (unsigned a = lo + 1; a < hi; ++a) {
-        const unsigned c = tcol[a];
+        const unsigned c = tcol[a], e = tsrc[a];
         const double v = tval[a];
         unsigned b = a;
-        while (b > lo && tcol[b - 1] > c) {
+        while (b > lo && tsrc[b - 1] > e) {
             tcol[b] = tcol[b - 1];
             tval[b] = tval[b - 1];
+            tsrc[b] = tsrc[b - 1];
             --b;
         }
         tcol[b] = c;
         tval[b] = v;
+        tsrc[b] = e;
     }
 }
 
@@ -213,17 +222,18 @@ extern "C" {
 int hb_transpose(int n, unsigned nnz, const unsigned *ptr, const unsigned *col, const double *val,
                  unsigned *tptr, unsigned *tcol, double *tval, unsigned *work, hipStream_t s)
 {
-    /* work: cnt[n] | fill[n] | bsum[ceil(n/TB)] */
+    /* work: cnt[n] | fill[n] | bsum[ceil(n/TB)] | tsrc[nnz] */
     const int nb = (n + TB - 1) / TB;
     unsigned *cnt = work, *fill = work + n, *bsum = work + 2 * (size_t)n;
+    unsigned *tsrc = bsum + nb;
     hipLaunchKernelGGL(k_zero_u, dim3(nb < 4096 ? nb : 4096), dim3(TB), 0, s, cnt, n);
     const int g = nnz ? (int)((nnz + TB - 1) / TB < 65536 ? (nnz + TB - 1) / TB : 65536) : 1;
     hipLaunchKernelGGL(k_count_cols, dim3(g), dim3(TB), 0, s, col, nnz, cnt);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(TB), 0, s, cnt, n, bsum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TB), 0, s, bsum, nb);
     hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(TB), 0, s, cnt, n, bsum, tptr, fill);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TB), 0, s, ptr, col, val, n, fill, tcol, tval);
-    hipLaunchKernelGGL(k_sort_rows, dim3(nb), dim3(TB), 0, s, tptr, n, tcol, tval);
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TB), 0, s, ptr, col, val, n, fill, tcol, tval, tsrc);
+    hipLaunchKernelGGL(k_sort_rows, dim3(nb), dim3(TB), 0, s, tptr, n, tcol, tval, tsrc);
     return (int)hipGetLastError();
 }
 

@@ -995,13 +995,14 @@ int hbicg_plan_true_residuals(hbicg_plan *pl, const double *b, const double *sig
         db = pl->b_buf;
     }
     HIPCK(hipMemcpyAsync(pl->sigma, sigma, (size_t)pl->M * 8, hipMemcpyHostToDevice, pl->stream));
-    const int nblk = (pl->n + HB_BLOCK - 1) / HB_BLOCK;
-    part = (double *)dalloc((size_t)nblk * pl->M * 2 * 8, &rc);
+    const int R = hb_k_resid_blocks(pl->n, pl->M);
+    part = (double *)dalloc((size_t)R * pl->M * 2 * 8, &rc);
     dout = (double *)dalloc((size_t)pl->M * 8, &rc);
     if (rc) goto fail;
-    hb_dev d = pl->dev;
-    d.x_set = (double *)dx;
-    if (hb_k_true_residual(&d, db, part, dout, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    if (hb_k_true_residual(&pl->dev, db, pl->sigma, dx, part, dout, pl->stream) != 0) {
+        rc = HBICG_E_HIP;
+        goto fail;
+    }
     HIPCK(hipMemcpyAsync(out, dout, (size_t)pl->M * 8, hipMemcpyDeviceToHost, pl->stream));
     HIPCK(hipStreamSynchronize(pl->stream));
 fail:
@@ -1486,7 +1487,7 @@ int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, doub
         return HBICG_E_ARG;
     const hipStream_t s = (hipStream_t)stream;
     const size_t n = A->n;
-    work = (unsigned *)dalloc((2 * n + (n + HB_BLOCK - 1) / HB_BLOCK + 1) * sizeof(unsigned), &rc);
+    work = (unsigned *)dalloc((2 * n + (n + HB_BLOCK - 1) / HB_BLOCK + (size_t)A->nz + 1) * sizeof(unsigned), &rc);
     if (rc) return rc;
     if (hb_transpose((int)n, A->nz, A->ptr, A->col, A->val, tptr, tcol, tval, work, s) != 0) {
         rc = HBICG_E_HIP;

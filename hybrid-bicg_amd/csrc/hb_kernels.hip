@@ -682,7 +682,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     __shared__ double sh_zp[HB_BLOCK];
     __shared__ int sh_j[HB_BLOCK];
     __shared__ int sh_nc[HB_BLOCK];
-    __shared__ int sh_switch, sh_spike, sh_flush;
+    __shared__ int sh_switch, sh_spike, sh_flush, sh_bad;
     __shared__ double sh_rprev;
     hb_state *st = d.st;
     const int tid = threadIdx.x;
@@ -697,6 +697,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         const double beta = (ak / wk) * (rTr / rTr_old);
         sh_rprev = st->dot_r;
         sh_spike = 0;
+        sh_bad = 0;
         st->dot_r = dot_r;
         st->rTr = rTr;
         d.ba[k] = beta;
@@ -776,6 +777,10 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             const double pik = eta + pim1;
             const double aj = (pim1 / pik) * ak;
             const double den = 1.0 - wk * dj;
+            /* a shift whose 1 - omega~ (sigma_s - sigma_j) or pi_j[k] is zero
+             * or not finite divides by it below (src/solver_dynamic.c:245-253);
+             * the arithmetic goes on as in the reference and the status says so */
+            if (!(den != 0.0 && isfinite(den) && pik != 0.0 && isfinite(pik))) sh_bad = 1;
             const double wj = wk / den;
             const double z = d.zeta[j];
             const double az = aj * z;
@@ -844,6 +849,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     }
     if (tid == 0) {
         st->stop_count += sh_nc[0];
+        if (sh_bad) st->status |= HB_ST_BREAKDOWN_SHIFT;
         if (sh_j[0] >= 0) st->max_sigma = sh_j[0];
         int sw = 0;
         if (d.stop[s] && st->stop_count < M) {
@@ -1652,20 +1658,30 @@ __global__ void k_publish(const hb_state *st, int *flag)
     }
 }
 
-/* ---- true residual of every shift ---------------------------------------- */
-__global__ __launch_bounds__(HB_BLOCK) void k_resid_part(hb_dev d, const double *b, double *part)
+/* ---- true residual of every shift ----------------------------------------
+ * out[j] = ||b - (A + sigma_j I) x_j|| / ||b|| in FP64: grid (R, M), the R
+ * workgroups of shift j striding over the rows; row i's product summed in
+ * CSR order, then + sigma_j x_i and - b_i as the reference forms it
+ * (src/solver_dynamic.c:405-421, src/matrix.c:498-516); the squared entries
+ * go to one partial pair per workgroup, combined in a fixed order per shift.
+ * R depends only on (n, M), so every caller gets the same roundings. */
+__global__ __launch_bounds__(HB_BLOCK) void k_resid_part(hb_dev d, const double *__restrict__ b,
+                                                         const double *__restrict__ sigma,
+                                                         const double *__restrict__ x_set,
+                                                         double *__restrict__ part)
 {
     const int j = blockIdx.y;
-    const int i = blockIdx.x * HB_BLOCK + threadIdx.x;
-    const double *x = d.x_set + off(j, d.n);
+    const double *__restrict__ x = x_set + off(j, d.n);
+    const double sj = sigma[j];
     double e2 = 0.0, b2 = 0.0;
-    if (i < d.n) {
+    for (int i = blockIdx.x * HB_BLOCK + threadIdx.x; i < d.n; i += gridDim.x * HB_BLOCK) {
         double t = 0.0;
         for (unsigned e = d.ptr[i]; e < d.ptr[i + 1]; ++e) t += d.val[e] * x[d.col[e]];
-        t = t + d.sigma[j] * x[i];
-        const double e1 = t - b[i];
-        e2 = e1 * e1;
-        b2 = b[i] * b[i];
+        t = t + sj * x[i];
+        const double bi = b[i];
+        const double e1 = t - bi;
+        e2 += e1 * e1;
+        b2 += bi * bi;
     }
     block_sum2(e2, b2);
     if (threadIdx.x == 0) {
@@ -1906,11 +1922,19 @@ int hb_k_finalize(const hb_dev *d, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *out, hipStream_t s)
+int hb_k_resid_blocks(int n, int M)
 {
-    const int nb = (d->n + HB_BLOCK - 1) / HB_BLOCK;
-    hipLaunchKernelGGL(k_resid_part, dim3(nb, d->M), dim3(HB_BLOCK), 0, s, *d, b, part);
-    hipLaunchKernelGGL(k_resid_final, dim3(d->M), dim3(HB_BLOCK), 0, s, part, nb, out);
+    const int nb = (n + HB_BLOCK - 1) / HB_BLOCK;
+    const int cap = M > 0 && M <= HB_RESID_PAIRS ? HB_RESID_PAIRS / M : 1;
+    return nb < cap ? nb : cap;
+}
+
+int hb_k_true_residual(const hb_dev *d, const double *b, const double *sigma, const double *x,
+                       double *part, double *out, hipStream_t s)
+{
+    const int R = hb_k_resid_blocks(d->n, d->M);
+    hipLaunchKernelGGL(k_resid_part, dim3(R, d->M), dim3(HB_BLOCK), 0, s, *d, b, sigma, x, part);
+    hipLaunchKernelGGL(k_resid_final, dim3(d->M), dim3(HB_BLOCK), 0, s, part, R, out);
     return (int)hipGetLastError();
 }
 

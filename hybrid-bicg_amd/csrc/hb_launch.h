@@ -43,9 +43,12 @@ int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s);
 /* Applies a seed-switch rescale of r left pending by the last iteration. */
 int hb_k_finalize(const hb_dev *d, hipStream_t s);
 
-/* out[j] = ||b - (A + sigma_j I) x_j|| / ||b|| for every shift j.
- * part must hold 2*M*ceil(n/HB_BLOCK) doubles. */
-int hb_k_true_residual(const hb_dev *d, const double *b, double *part, double *out, hipStream_t s);
+/* out[j] = ||b - (A + sigma_j I) x_j|| / ||b|| for every shift j (sigma, x,
+ * b, out device pointers).  part must hold 2*M*hb_k_resid_blocks(n, M)
+ * doubles; the grid, and so the roundings, depend on (n, M) only. */
+int hb_k_resid_blocks(int n, int M);
+int hb_k_true_residual(const hb_dev *d, const double *b, const double *sigma, const double *x,
+                       double *part, double *out, hipStream_t s);
 
 /* p[0 .. count) = v. */
 int hb_k_fill(double *p, size_t count, double v, hipStream_t s);

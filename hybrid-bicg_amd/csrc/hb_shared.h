@@ -22,11 +22,13 @@ extern "C" {
 #define HB_DIAG_K       16    /* most diagonals of a diagonal tile             */
 #define HB_DIAG_NONE    255   /* index byte of a diagonal a row has no entry on */
 #define HB_CC_ROWS(W)   (2 * (W) + 5) /* rows of one chunk's coefficient block  */
+#define HB_RESID_PAIRS  65536 /* most partial pairs of one true-residual evaluation */
 
 /* Error bits in hb_state.status */
 #define HB_ST_BREAKDOWN_RTS   1   /* (r^,s) == 0 or non-finite alpha          */
 #define HB_ST_BREAKDOWN_QTY   2   /* (q,y) == 0 or non-finite omega           */
 #define HB_ST_NO_SWITCH_TARGET 4  /* seed stopped but no shift qualified      */
+#define HB_ST_BREAKDOWN_SHIFT  8  /* a shift's 1 - omega~ d or pi_k is 0 or non-finite */
 
 /*
  * Device-resident state of one shifted solve.  Every scalar the reference

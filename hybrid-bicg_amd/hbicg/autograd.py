@@ -13,8 +13,15 @@ nonzeros in CSR order; ptr/col are int32 tensors holding the unsigned
 A^T is built on the device (hbicg_csr_transpose).  When the g_k are
 collinear, g_k = w_k c (a loss sum_k w_k c^T x_k), one shifted solve on A^T
 with right-hand side c gives every mu_k and lam_k = w_k mu_k.  Otherwise
-each shift is its own single-shift problem on A^T, all solved together by
-the batched solver.  Gradients are assembled by hbicg_adjoint_gradients.
+each shift with a nonzero g_k is its own single-shift problem on A^T, all
+solved together by the batched solver.  Gradients are assembled by
+hbicg_adjoint_gradients.
+
+A zero right-hand side is never handed to the solver: BiCG's first step
+divides (r^, s) = 0 by itself, and the solver, like the reference, carries
+the NaN to the iteration cap.  Its solution is 0, so a zero b gives x = 0
+in the forward pass, a zero g_k gives lam_k = 0, and an all-zero gradient
+gives zero gradients.
 """
 from __future__ import annotations
 
@@ -65,13 +72,14 @@ def gradients(val, col, ptr, mu, w, x, need_b=True, need_sigma=True, need_val=Tr
 
 
 def collinear_split(g, rtol=1e-13):
-    """(w, c) with g_k = w_k c when the rows of g are collinear, else None."""
+    """(w, c) with g_k = w_k c when the rows of g are collinear, else None.
+    g must have a nonzero row (the caller handles an all-zero gradient)."""
     norms = torch.linalg.vector_norm(g, dim=1)
     k0 = int(torch.argmax(norms))
     c = g[k0]
     cc = torch.dot(c, c)
     if float(cc) == 0.0:
-        return torch.zeros(g.shape[0], dtype=g.dtype, device=g.device), c
+        raise ValueError("collinear_split: every row of g is zero")
     w = (g @ c) / cc
     resid = torch.linalg.vector_norm(g - w[:, None] * c)
     if float(resid) <= rtol * float(torch.linalg.vector_norm(g)):
@@ -146,8 +154,12 @@ class ShiftedSolve(torch.autograd.Function):
         vd = val.detach()
         if not vd.is_contiguous():
             vd = vd.contiguous()
-        plan = _plan_for(vd, col, ptr, M, tol, max_iter)
-        k, x, rep = plan.solve(b.detach().contiguous(), sig, int(seed))
+        bd = b.detach().contiguous()
+        if bool(torch.any(bd != 0)):
+            plan = _plan_for(vd, col, ptr, M, tol, max_iter)
+            k, x, rep = plan.solve(bd, sig, int(seed))
+        else:
+            k, x = 1, torch.zeros((M, bd.numel()), dtype=torch.float64, device=bd.device)
         ctx.save_for_backward(vd, b, x)
         ctx.ptr, ctx.col, ctx.sig, ctx.seed = ptr, col, sig, int(seed)
         ctx.adj_tol, ctx.max_iter = adj_tol, max_iter
@@ -161,6 +173,12 @@ class ShiftedSolve(torch.autograd.Function):
         ptr, col, sig = ctx.ptr, ctx.col, ctx.sig
         M = x.shape[0]
         gx = gx.contiguous()
+        live = torch.any(gx != 0, dim=1).cpu().numpy()
+        if not live.any():
+            zero = lambda t, need: torch.zeros_like(t) if need else None
+            gs = zero(torch.empty(M, dtype=ctx.sigma_dtype, device=ctx.sigma_device), ctx.needs_input_grad[2])
+            return (zero(vd, ctx.needs_input_grad[0]), zero(b, ctx.needs_input_grad[1]), gs,
+                    None, None, None, None, None, None)
         tval, tcol, tptr = _transpose_into(vd, col, ptr)
         params = make_params(tol=ctx.adj_tol, max_iter=ctx.max_iter, graph=1, value_index=False)
         split = collinear_split(gx)
@@ -169,13 +187,18 @@ class ShiftedSolve(torch.autograd.Function):
             plan = _plan_for(tval, tcol, tptr, M, ctx.adj_tol, ctx.max_iter)
             _, mu, _ = plan.solve(c, sig, ctx.seed)
         else:
+            # one single-shift problem per shift whose g_k is nonzero;
+            # lam_k = 0 for the others
             w = None
+            rows = [k for k in range(M) if live[k]]
             spec = [{"val": tval, "col": tcol, "ptr": tptr, "b": gx[k], "sigma": sig[k:k + 1],
-                     "seed": 0} for k in range(M)]
+                     "seed": 0} for k in rows]
             bt = Batch(None, params, device=x.device, device_spec=spec)
             _, lams, _ = bt.solve()
             bt.close()
-            mu = torch.cat([l_ for l_ in lams], dim=0).contiguous()
+            mu = torch.zeros_like(x)
+            for k, l_ in zip(rows, lams):
+                mu[k] = l_[0]
         gb, gs, gv = gradients(vd, col, ptr, mu, w, x, ctx.needs_input_grad[1],
                                ctx.needs_input_grad[2], ctx.needs_input_grad[0])
         if gs is not None:

@@ -23,10 +23,21 @@ extern "C" {
 #define HBICG_E_NOMEM      (-5)  /* host or device allocation failed                    */
 #define HBICG_E_NODEVICE   (-6)  /* no HIP device                                       */
 
-/* status bits of hbicg_report.status (device-detected conditions) */
+/* status bits of hbicg_report.status (device-detected conditions).  The
+ * reference does not trap breakdowns (src/solver_dynamic.c:167,189,245-253):
+ * the arithmetic goes on with the resulting NaN or infinity, and so does
+ * this solver, to the stopping rule or the iteration cap; these bits say
+ * that it happened. */
 #define HBICG_S_BREAKDOWN_RTS    1  /* (r^, s) = 0 or non-finite alpha~ */
 #define HBICG_S_BREAKDOWN_QTY    2  /* (q, y) = 0 or non-finite omega~  */
-#define HBICG_S_NO_SWITCH_TARGET 4  /* seed stopped, no shift to switch to; solve ended */
+#define HBICG_S_NO_SWITCH_TARGET 4  /* the seed stopped while shifts still ran, none of which
+                                       ever had a residual estimate above the seed's (only
+                                       NaN estimates do that): the solve ends after that
+                                       iteration (the reference switches to an
+                                       uninitialised index, src/solver_dynamic.c:25,318) */
+#define HBICG_S_BREAKDOWN_SHIFT  8  /* a running shift's 1 - omega~ (sigma_s - sigma_j) or
+                                       pi_j[k] was 0 or not finite (a NaN shift, or a shift
+                                       that makes the recurrence divide by zero) */
 
 typedef struct hbicg_params {
     double tol;          /* stopping rule, default EPS (1e-12)                           */
@@ -204,8 +215,10 @@ int hbicg_batch_true_residuals(hbicg_batch *bt, int p, const double *b, const do
 
 /* ---- adjoint support (device pointers) ------------------------------------- */
 /* A^T of an n x n device CSR into tptr[n+1], tcol[nz], tval[nz] (device),
- * rows listing their entries in ascending column order.  stream: hipStream_t
- * or NULL.  Synchronises the stream. */
+ * rows listing their entries in ascending column order and duplicate entries
+ * in their order in A (a stable transpose: the result does not depend on the
+ * order the device's atomics place entries in).  stream: hipStream_t or
+ * NULL.  Synchronises the stream. */
 int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, double *tval, void *stream);
 
 /* Gradients of a loss through x_k = (A + sigma_k I)^{-1} b from the adjoint

@@ -16,11 +16,17 @@
  * non-square A_info prints the reference's message and calls exit(1), like
  * the reference, unless hbicg_params.exit_on_error is 0.
  *
- * The three variants compute bit-identical results (as the reference's do):
+ * The three variants differ only in when the per-shift vector updates are
+ * applied (the reference's differ only in MPI placement and OpenMP schedule):
  *   normal  - shift vectors updated every iteration (window 1)
  *   static  - shift vectors updated in windows of hbicg_params.window
  *             iterations, on the solver stream
  *   dynamic - windowed shift updates, iteration replayed as hipGraph chunks
+ * With hbicg_params.exact_order = 1 all three are bit-identical to each
+ * other and to the reference's arithmetic.  In the default mode static and
+ * dynamic are bit-identical to each other; normal (window 1) agrees with
+ * them to rounding: x within 1e-10 relative without a seed switch, and
+ * after a switch k, stop and switch iterations within 2 and x within 1e-9.
  * Stopping rule and iteration cap come from hbicg_set_params (defaults
  * EPS = 1e-12 and MAX_ITER = 2000, src/solver.h:13-14).
  */

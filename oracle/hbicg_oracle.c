@@ -212,6 +212,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
     if (ws_alloc(&w, n, M, cap) != 0) { ws_free(&w); return -1; }
     if (tr) {
         tr->n_switch = 0;
+        tr->status = 0;
         tr->t_seed = tr->t_shift = tr->t_check = 0.0;
         if (tr->stop_iter) memset(tr->stop_iter, 0, (size_t)M * sizeof(int));
         if (tr->res_hist) for (size_t t = 0; t < (size_t)cap * M; ++t) tr->res_hist[t] = -1.0;
@@ -296,8 +297,24 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
             }
         }
 
-        /* seed switch (src/solver_dynamic.c:312-341) */
-        if (w.stop[seed] && stop_count < M && max_sigma >= 0) {
+        /* seed switch (src/solver_dynamic.c:312-341).  The reference never
+         * initialises max_sigma (src/solver_dynamic.c:25), so a seed that
+         * stops before any running shift ever had an estimate above 1 --
+         * only possible when those shifts' estimates are NaN -- switches to
+         * an undefined index there.  Here that case ends the solve after
+         * this iteration with ORC_NO_SWITCH_TARGET in the trace status. */
+        if (w.stop[seed] && stop_count < M && max_sigma < 0) {
+            if (tr) tr->status |= ORC_NO_SWITCH_TARGET;
+            if (tr) {
+                const double t3 = wtime();
+                tr->t_seed += t1 - t0;
+                tr->t_shift += t2 - t1;
+                tr->t_check += t3 - t2;
+            }
+            ++k;
+            break;
+        }
+        if (w.stop[seed] && stop_count < M) {
             const int m = max_sigma;
             const double ds = sigma[seed] - sigma[m];
             for (int i = 1; i <= k; ++i) {

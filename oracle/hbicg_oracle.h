@@ -46,7 +46,15 @@ typedef struct {
     double  t_seed;       /* out: seconds in seed steps (src/solver_dynamic.c:212-221 buckets) */
     double  t_shift;      /* out: seconds in the shift loop                                  */
     double  t_check;      /* out: seconds in convergence check and seed switches             */
+    int     status;       /* out: ORC_NO_SWITCH_TARGET when the solve ended for that reason   */
 } orc_trace;
+
+/* The seed stopped while other shifts still ran, none of which ever had a
+ * residual estimate above the seed's (their estimates are NaN): there is no
+ * shift to switch to.  The reference leaves this undefined (it switches to
+ * an uninitialised index); the restatement ends the solve after that
+ * iteration. */
+#define ORC_NO_SWITCH_TARGET 4
 
 /* One solve of (A + sigma_j I) x_j = b, j = 0..M-1.
  *   x_set : [M][n] shift-major, accumulated in place (zero on entry for x0 = 0)

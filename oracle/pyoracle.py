@@ -21,6 +21,8 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _up = C.POINTER(C.c_uint)
 
+NO_SWITCH_TARGET = 4   # OracleResult.status bit (hbicg_oracle.h ORC_NO_SWITCH_TARGET)
+
 
 class _Csr(C.Structure):
     _fields_ = [("val", _dp), ("col", _up), ("ptr", _up), ("n", C.c_uint), ("nz", C.c_uint)]
@@ -30,7 +32,8 @@ class _Trace(C.Structure):
     _fields_ = [("stop_iter", _ip), ("switch_k", _ip), ("switch_to", _ip), ("n_switch", C.c_int),
                 ("alpha", _dp), ("beta", _dp), ("omega", _dp), ("res_hist", _dp),
                 ("seed_hist", _ip), ("zeta", _dp), ("pi_k", _dp), ("dots", _dp),
-                ("t_seed", C.c_double), ("t_shift", C.c_double), ("t_check", C.c_double)]
+                ("t_seed", C.c_double), ("t_shift", C.c_double), ("t_check", C.c_double),
+                ("status", C.c_int)]
 
 
 _lib = None
@@ -97,6 +100,7 @@ class OracleResult:
     t_seed: float = 0.0      # seconds in seed steps
     t_shift: float = 0.0     # seconds in the shift loop
     t_check: float = 0.0     # seconds in check + switch
+    status: int = 0          # NO_SWITCH_TARGET (4) when the solve ended for that reason
 
 
 def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
@@ -119,7 +123,7 @@ def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
     dots = np.zeros(cap * 5)
     tr = _Trace(_p(stop_iter, _ip), _p(sw_k, _ip), _p(sw_to, _ip), 0, _p(alpha), _p(beta),
                 _p(omega), _p(res_hist), _p(seed_hist, _ip), _p(zeta), _p(pi_k), _p(dots),
-                0.0, 0.0, 0.0)
+                0.0, 0.0, 0.0, 0)
     k = L.orc_shifted_lopbicg(C.byref(h.s), _p(x), _p(r), _p(sigma), M, int(seed), float(tol),
                               int(max_iter), C.byref(tr))
     if k < 0:
@@ -127,7 +131,7 @@ def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
     sw = [(int(sw_k[i]), int(sw_to[i])) for i in range(tr.n_switch)]
     return OracleResult(k, x, r, stop_iter, sw, alpha, beta, omega, res_hist.reshape(cap, M),
                         seed_hist, zeta, pi_k, dots.reshape(cap, 5), tr.t_seed, tr.t_shift,
-                        tr.t_check)
+                        tr.t_check, tr.status)
 
 
 def mult(A, x) -> np.ndarray:

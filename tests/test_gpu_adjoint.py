@@ -164,3 +164,87 @@ def test_values_changed_in_place_between_calls(prob):
     x2, g2 = run(fresh)
     assert np.array_equal(x1, x2)
     assert np.array_equal(g1, g2)
+
+
+def test_loss_on_two_shifts_only(prob):
+    """A loss that reads only x_1 and x_4 leaves the other rows of dL/dx
+    zero.  Those adjoint solves are skipped (a zero right-hand side would
+    make BiCG divide 0 by 0), so every gradient is finite and matches
+    central differences."""
+    A, b0, sig0, _, _ = prob
+    print("case adjoint loss on 2 of 6 shifts", flush=True)
+    val0, col, ptr = dev_csr(A)
+    t1 = torch.as_tensor(P.uniform(7, 1, A.n, -1.0, 1.0)).cuda()
+    t4 = torch.as_tensor(P.uniform(7, 2, A.n, -1.0, 1.0)).cuda()
+    val = val0.clone().requires_grad_(True)
+    b = torch.as_tensor(b0).cuda().requires_grad_(True)
+    sigma = torch.as_tensor(sig0).clone().requires_grad_(True)
+
+    def L(v, bb, ss):
+        x = shifted_solve(v, ptr, col, bb, ss, seed=0, tol=TOL)
+        return ((x[1] - t1) ** 2).sum() + ((x[4] - t4) ** 2).sum()
+
+    L(val, b, sigma).backward()
+    for g in (val.grad, b.grad, sigma.grad):
+        assert torch.isfinite(g).all()
+    assert float(sigma.grad[0]) == 0.0 and float(sigma.grad[5]) == 0.0
+    rng = np.random.default_rng(13)
+    d = torch.as_tensor(rng.uniform(-1, 1, A.n)).cuda()
+    with torch.no_grad():
+        fd = (float(L(val0, b.detach() + H * d, sigma.detach())) -
+              float(L(val0, b.detach() - H * d, sigma.detach()))) / (2 * H)
+    an = float(b.grad @ d)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    for k in (1, 4):
+        e = torch.zeros_like(sigma)
+        e[k] = H
+        with torch.no_grad():
+            fd = (float(L(val0, b.detach(), sigma.detach() + e)) -
+                  float(L(val0, b.detach(), sigma.detach() - e))) / (2 * H)
+        assert abs(fd - float(sigma.grad[k])) <= 1e-6 * abs(float(sigma.grad[k])), (k, fd)
+    D = torch.as_tensor(rng.uniform(-1, 1, A.nnz)).cuda()
+    with torch.no_grad():
+        fd = (float(L(val0 + H * D, b.detach(), sigma.detach())) -
+              float(L(val0 - H * D, b.detach(), sigma.detach()))) / (2 * H)
+    an = float(val.grad @ D)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_zero_rhs_and_zero_gradient(prob):
+    """b = 0: x = 0 without a solve (the solver would carry 0/0 to its cap).
+    A loss with an all-zero gradient gives zero gradients."""
+    A, _, sig0, _, _ = prob
+    val0, col, ptr = dev_csr(A)
+    val = val0.clone().requires_grad_(True)
+    b = torch.zeros(A.n, dtype=torch.float64, device="cuda", requires_grad=True)
+    sigma = torch.as_tensor(sig0).clone().requires_grad_(True)
+    x = shifted_solve(val, ptr, col, b, sigma, seed=0, tol=1e-10)
+    assert x.shape == (len(sig0), A.n) and not torch.any(x != 0)
+    (x.sum() * 0.0).backward()
+    for g in (val.grad, b.grad, sigma.grad):
+        assert g is not None and not torch.any(g != 0)
+
+
+def test_transpose_with_duplicate_entries_is_stable():
+    """Duplicate (i, j) entries (Matrix Market files may hold them; coo2csr
+    keeps them) land in A^T in their order in A, whatever order the
+    scatter's atomics place them in: two transposes agree bit for bit with
+    each other and with a stable host transpose."""
+    rng = np.random.default_rng(21)
+    n = 2000
+    lens = rng.integers(1, 40, n)
+    rows = np.repeat(np.arange(n), lens)
+    cols = rng.integers(0, 64, rows.size)             # few columns: many duplicates
+    order = np.lexsort((cols, rows))                  # row-major, columns ascending
+    rows, cols = rows[order], cols[order]
+    ptr = np.zeros(n + 1, dtype=np.uint32)
+    ptr[1:] = np.cumsum(lens)
+    A = P.CSR(ptr, cols.astype(np.uint32), rng.uniform(-1, 1, rows.size), n)
+    assert np.any((np.diff(cols) == 0) & (np.diff(rows) == 0))
+    val, col, ptr_d = dev_csr(A)
+    T = A.transpose()
+    outs = [transpose(val, col, ptr_d) for _ in range(3)]
+    for tval, tcol, tptr in outs:
+        assert np.array_equal(tptr.cpu().numpy().view(np.uint32), T.ptr)
+        assert np.array_equal(tcol.cpu().numpy().view(np.uint32), T.col)
+        assert np.array_equal(tval.cpu().numpy(), T.val)

@@ -1,0 +1,136 @@
+"""The batched and differentiable workloads at their full sizes.
+
+Batched: 256 independent problems, n log-uniform in [1k, 200k] (2-D 5-point
+and 3-D 7-point convection-diffusion, convection uniform in [0, 0.4]), 8 to
+64 shifts each log-spaced in [1e-3, 1], tolerance 1e-10, in one batch.
+Every problem converges, every shift's true residual (the checker that the
+full-size tests hold against the oracle's) is within 10x the stopping rule,
+and every problem equals a plain plan of its own run with the batch's
+window bit for bit: k, stop iterations, switch log and x.
+
+Differentiable: 3-D 7-point convection-diffusion on 64^3 (n = 262,144),
+32 shifts log-spaced in [1e-2, 1], L = sum_k w_k c^T x_k with w, c uniform
+in [0, 1] from seed 4.  The gradients w.r.t. b (along a random direction),
+every sigma_k and the nonzeros of A (along a random direction on the
+pattern) match central differences with h = 1e-6 to 1e-6 relative; the
+adjoint solutions mu_k = (A + sigma_k I)^{-T} c match per-shift solves of the
+oracle on A^T to 1e-8 relative for four shifts.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as O
+from hbicg import Batch, Plan, make_params
+from hbicg import problems as P
+from hbicg.autograd import shifted_solve, transpose
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_256_problems():
+    probs = P.config3(count=256)
+    print(f"case batched 256 problems (sum n = {sum(p.n for p in probs)}, "
+          f"sum M = {sum(p.M for p in probs)})", flush=True)
+    bt = Batch(probs, make_params(tol=1e-10, graph=1))
+    k, xs, ms = bt.solve()
+    W = None
+    worst = 0.0
+    for i, p in enumerate(probs):
+        rep, stop, sw = bt.report(i)
+        assert rep["stop_count"] == p.M, (i, p.name)
+        tr = bt.true_residuals(i, xs[i])
+        worst = max(worst, float(tr.max()))
+        assert tr.max() <= 10 * p.tol, (i, p.name, tr.max())
+        W = rep["window"] if W is None else W
+        assert rep["window"] == W
+    print(f"  batch {ms:.1f} ms, window {W}, max true residual {worst:.2e}", flush=True)
+    for i, p in enumerate(probs):
+        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1, window=W))
+        kp, xp, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, p.seed)
+        _, stop, sw = bt.report(i)
+        _, pstop, psw = pl.report()
+        assert k[i] == kp, (i, p.name)
+        assert np.array_equal(stop, pstop) and sw == psw, (i, p.name)
+        assert torch.equal(xs[i], xp), (i, p.name)
+        pl.close()
+        del xp
+    bt.close()
+
+
+TOL = 1e-15
+H = 1e-6
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return P.config4()
+
+
+def _dev_csr(A):
+    return (torch.as_tensor(A.val).cuda(), torch.as_tensor(A.col.view(np.int32)).cuda(),
+            torch.as_tensor(A.ptr.view(np.int32)).cuda())
+
+
+def test_adjoint_64cubed_finite_differences(c4):
+    print("case adjoint 64^3, 32 shifts: gradients", flush=True)
+    A = c4.A
+    val0, col, ptr = _dev_csr(A)
+    w = torch.as_tensor(c4.extra["w"]).cuda()
+    c = torch.as_tensor(c4.extra["c"]).cuda()
+
+    def loss(v, bb, ss):
+        x = shifted_solve(v, ptr, col, bb, ss, seed=c4.seed, tol=TOL)
+        return (w * (x @ c)).sum()
+
+    val = val0.clone().requires_grad_(True)
+    b = torch.as_tensor(c4.b).cuda().requires_grad_(True)
+    sigma = torch.as_tensor(c4.sigma).clone().requires_grad_(True)
+    loss(val, b, sigma).backward()
+    b0, s0 = b.detach(), sigma.detach()
+
+    def Lval(v, bb, ss):
+        with torch.no_grad():
+            return float(loss(v, bb, ss))
+
+    rng = np.random.default_rng(44)
+    d = torch.as_tensor(rng.uniform(-1, 1, A.n)).cuda()
+    fd = (Lval(val0, b0 + H * d, s0) - Lval(val0, b0 - H * d, s0)) / (2 * H)
+    an = float(b.grad @ d)
+    print(f"  b: fd {fd:.10e} analytic {an:.10e}", flush=True)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    worst = 0.0
+    for k in range(c4.M):
+        e = torch.zeros_like(s0)
+        e[k] = H
+        fd = (Lval(val0, b0, s0 + e) - Lval(val0, b0, s0 - e)) / (2 * H)
+        an = float(sigma.grad[k])
+        worst = max(worst, abs(fd - an) / abs(an))
+        assert abs(fd - an) <= 1e-6 * abs(an), (k, fd, an)
+    print(f"  sigma: max relative FD gap over 32 shifts {worst:.2e}", flush=True)
+    D = torch.as_tensor(rng.uniform(-1, 1, A.nnz)).cuda()
+    fd = (Lval(val0 + H * D, b0, s0) - Lval(val0 - H * D, b0, s0)) / (2 * H)
+    an = float(val.grad @ D)
+    print(f"  A: fd {fd:.10e} analytic {an:.10e}", flush=True)
+    assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_adjoint_64cubed_solutions_match_oracle(c4):
+    """The adjoint solve the backward pass runs for this loss -- one shifted
+    solve on the device-built A^T with right-hand side c -- against the
+    oracle's single-shift solves on the host-built A^T."""
+    print("case adjoint 64^3: mu_k against oracle transposed solves", flush=True)
+    A = c4.A
+    val, col, ptr = _dev_csr(A)
+    tval, tcol, tptr = transpose(val, col, ptr)
+    pl = Plan(None, c4.M, make_params(tol=1e-14, graph=1, value_index=False),
+              tensors=(tval, tcol, tptr))
+    k, mu, _ = pl.solve(torch.as_tensor(c4.extra["c"]).cuda(), c4.sigma, c4.seed)
+    pl.close()
+    mu = mu.cpu().numpy()
+    T = A.transpose()
+    for j in (0, 10, 21, 31):
+        ref = O.solve(T, c4.extra["c"], c4.sigma[j:j + 1], 0, tol=1e-14).x[0]
+        rel = np.linalg.norm(mu[j] - ref) / np.linalg.norm(ref)
+        print(f"  shift {j}: rel {rel:.2e}", flush=True)
+        assert rel <= 1e-8, (j, rel)

@@ -1,28 +1,46 @@
 """Parity at the workloads' full sizes.
 
-* 128^3 7-point convection-diffusion, 64 shifts (n = 2,097,152) and the
-  irregular power-law matrix, 128 shifts (n = 1,000,000, 31M nonzeros):
-  the first iterations run by the oracle on the host and, with the same
-  iteration cap, by the GPU in exact-order mode must agree bit for bit (x of
-  every shift, r); in the default mode within 1e-9 relative.
-* complete solves to the 1e-10 stopping rule on the GPU: every shift that
-  stops has a true residual ||b - (A + sigma I) x|| / ||b|| (recomputed in
-  FP64) within 10x the rule; the static and dynamic schedules agree bit for
-  bit.  On the stencil every shift stops.  On the power-law matrix the
-  smallest shifts (sigma < 0.025, the seed among them) do not reach 1e-10
-  within the 2000-iteration cap -- the method converges too slowly there --
-  and the solve returns the cap + 1 like the reference; every shift with
-  sigma >= 0.025 stops.
+128^3 7-point convection-diffusion, 64 shifts (n = 2,097,152), against the
+oracle's complete solves committed in tests/golden/c1_oracle.npz (written by
+tests/golden/make_c1_fixture.py), for first seed 0 (no switch) and first
+seed 63 (the largest shift converges first; the seed switches at iteration
+31 to shift 0, src/solver_dynamic.c:309-359):
+  * exact order (the reference's summation order) equals the oracle bit for
+    bit: k, every stop iteration, the switch log, the sampled rows and a
+    bit-pattern checksum of every x_j and of the returned r;
+  * the default order -- a device plan, and shifted_lopbicg_static and
+    shifted_lopbicg_dynamic on host buffers -- stays within the parity bar:
+    k and every stop and switch iteration within 2, the same switch targets,
+    x_j within 1e-9 relative on the sampled rows and in norm, and every true
+    residual within 10x the stopping rule; static and dynamic agree bit for
+    bit with each other.
+The GPU true-residual checker is held against the oracle's on the same x.
+
+The irregular power-law matrix (n = 1e6, 31M nonzeros, 128 shifts): the
+first iterations run by the oracle on the host and by the GPU with the same
+cap agree bit for bit in exact order and within 1e-9 in the default order;
+the complete solve meets the stopping rule for every shift with sigma >=
+0.025 (the smallest shifts, the seed among them, do not reach 1e-10 within
+the 2000-iteration cap, so the solve returns cap + 1 like the reference).
 """
+import os
+
 import numpy as np
 import pytest
 import torch
 
 import pyoracle as O
-from hbicg import Plan, make_params
+from hbicg import Plan, dropin_solve, make_params
 from hbicg import problems as P
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_oracle.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return dict(np.load(GOLD))
 
 
 @pytest.fixture(scope="module")
@@ -36,6 +54,106 @@ def c2():
     return P.config2()
 
 
+def _bit_sums(a):
+    """Per row, the sum modulo 2^64 of the entries' bit patterns (as the
+    fixture script computes it): independent of summation order, so equal
+    on any host exactly when the rows are -- up to a collision of two
+    different vectors, which random data does not produce."""
+    return np.ascontiguousarray(a).view(np.uint64).sum(axis=1, dtype=np.uint64)
+
+
+def _rel_rows(a, b):
+    return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+
+
+def _within_bar(k, stop, sw, x, gold, s, what):
+    S = int(gold["stride"])
+    assert abs(k - int(gold[f"k_{s}"])) <= 2, (what, k, int(gold[f"k_{s}"]))
+    assert np.all(np.abs(stop - gold[f"stop_{s}"]) <= 2), (what, stop - gold[f"stop_{s}"])
+    gsw = [tuple(int(v) for v in row) for row in gold[f"switch_{s}"]]
+    assert len(sw) == len(gsw), (what, sw, gsw)
+    for (kg, sg), (kr, sr) in zip(sw, gsw):
+        assert abs(kg - kr) <= 2 and sg == sr, (what, sw, gsw)
+    rel = _rel_rows(x[:, ::S], gold[f"xs_{s}"])
+    assert rel.max() <= 1e-9, (what, rel.max())
+    nrel = np.abs(np.linalg.norm(x, axis=1) - gold[f"xnorm_{s}"]) / gold[f"xnorm_{s}"]
+    assert nrel.max() <= 1e-9, (what, nrel.max())
+    print(f"  {what}: k={k} (oracle {int(gold[f'k_{s}'])}), max rel dx on samples {rel.max():.2e}, "
+          f"in norm {nrel.max():.2e}", flush=True)
+
+
+@pytest.mark.parametrize("seed", [0, 63])
+def test_c1_exact_order_equals_oracle(c1, gold, seed):
+    print(f"case c1 exact order, seed {seed}", flush=True)
+    pl = Plan(c1.A, c1.M, make_params(tol=c1.tol, max_iter=c1.max_iter, exact_order=True, graph=1))
+    b = torch.as_tensor(c1.b).cuda()
+    r = torch.empty_like(b)
+    k, x, _ = pl.solve(b, c1.sigma, seed, r_out=r)
+    _, stop, sw = pl.report()
+    tr = pl.true_residuals(b, c1.sigma, x)
+    pl.close()
+    S = int(gold["stride"])
+    xh = x.cpu().numpy()
+    rh = r.cpu().numpy()
+    del x
+    assert k == int(gold[f"k_{seed}"])
+    assert np.array_equal(stop, gold[f"stop_{seed}"])
+    assert sw == [tuple(int(v) for v in row) for row in gold[f"switch_{seed}"]]
+    if seed == 63:
+        assert len(sw) >= 1
+    assert np.array_equal(xh[:, ::S], gold[f"xs_{seed}"])
+    assert np.array_equal(_bit_sums(xh), gold[f"xbits_{seed}"])
+    assert np.array_equal(rh[::S], gold[f"rs_{seed}"])
+    assert np.array_equal(_bit_sums(rh[None, :]), gold[f"rbits_{seed}"])
+    # the same x: the GPU checker against the oracle's (they differ only in
+    # the order the squared residual entries are summed)
+    np.testing.assert_allclose(tr, gold[f"tres_{seed}"], rtol=1e-12, atol=0)
+    assert tr.max() <= 10 * c1.tol
+
+
+@pytest.mark.parametrize("seed", [0, 63])
+def test_c1_default_order_plan_and_dropins(c1, gold, seed):
+    print(f"case c1 default order, seed {seed}", flush=True)
+    b = torch.as_tensor(c1.b).cuda()
+    pl = Plan(c1.A, c1.M, make_params(tol=c1.tol, max_iter=c1.max_iter, graph=1))
+    k, x, rep = pl.solve(b, c1.sigma, seed)
+    _, stop, sw = pl.report()
+    tr = pl.true_residuals(b, c1.sigma, x)
+    assert rep["stop_count"] == c1.M and tr.max() <= 10 * c1.tol, tr.max()
+    _within_bar(k, stop, sw, x.cpu().numpy(), gold, seed, "device plan")
+    del x
+    pl.close()
+    xs = {}
+    for variant in ("static", "dynamic"):
+        got = dropin_solve(c1.A, c1.b, c1.sigma, seed, variant, make_params(tol=c1.tol))
+        assert got.report["stop_count"] == c1.M
+        _within_bar(got.k, got.stop_iter, got.switches, got.x, gold, seed, f"shifted_lopbicg_{variant}")
+        xs[variant] = got
+    assert xs["static"].k == xs["dynamic"].k
+    assert np.array_equal(xs["static"].x, xs["dynamic"].x)
+    assert np.array_equal(xs["static"].r, xs["dynamic"].r)
+
+
+def test_true_residual_checker_against_oracle(c1):
+    """hbicg_plan_true_residuals against the oracle's checker on x far from
+    converged (every residual O(1)): the 64x64 workload's 16 shifts and 4
+    shifts of the 128^3 workload."""
+    rng = np.random.default_rng(41)
+    c0 = P.config0()
+    cases = [(c0.A, c0.b, c0.sigma), (c1.A, c1.b, c1.sigma[[0, 21, 42, 63]])]
+    for A, bh, sigma in cases:
+        print(f"case true residual checker n={A.n} M={len(sigma)}", flush=True)
+        M = len(sigma)
+        xh = rng.uniform(-1, 1, (M, A.n))
+        pl = Plan(A, M, make_params(tol=1e-10))
+        b = torch.as_tensor(bh).cuda()
+        got = pl.true_residuals(b, sigma, torch.as_tensor(xh).cuda())
+        pl.close()
+        want = O.true_residuals(A, xh, bh, sigma)
+        assert want.min() > 0.1
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+
+
 def _capped(prob, cap, exact, window=0):
     pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=cap, exact_order=exact,
                                           window=window, graph=1))
@@ -47,42 +165,36 @@ def _capped(prob, cap, exact, window=0):
     return out
 
 
-@pytest.mark.parametrize("name,cap", [("c1", 12), ("c2", 8)])
-def test_capped_iterations_match_oracle(name, cap, request):
-    prob = request.getfixturevalue(name)
-    print(f"case {name} capped at {cap} iterations: oracle", flush=True)
-    ref = O.solve(prob.A, prob.b, prob.sigma, prob.seed, tol=prob.tol, max_iter=cap)
-    print(f"case {name}: GPU exact order", flush=True)
-    k, x, r = _capped(prob, cap, exact=True)
+def test_c2_capped_iterations_match_oracle(c2):
+    cap = 8
+    print(f"case c2 capped at {cap} iterations: oracle", flush=True)
+    ref = O.solve(c2.A, c2.b, c2.sigma, c2.seed, tol=c2.tol, max_iter=cap)
+    print("case c2: GPU exact order", flush=True)
+    k, x, r = _capped(c2, cap, exact=True)
     assert k == ref.k
     assert np.array_equal(r, ref.r)
     assert np.array_equal(x, ref.x)
-    print(f"case {name}: GPU default order", flush=True)
-    k, x, r = _capped(prob, cap, exact=False)
+    print("case c2: GPU default order", flush=True)
+    k, x, r = _capped(c2, cap, exact=False)
     assert k == ref.k
     rel = np.linalg.norm(x - ref.x, axis=1) / np.maximum(np.linalg.norm(ref.x, axis=1), 1e-300)
     assert rel.max() <= 1e-9, rel.max()
 
 
-@pytest.mark.parametrize("name", ["c1", "c2"])
-def test_full_solve_true_residuals(name, request):
-    prob = request.getfixturevalue(name)
-    print(f"case {name}: full solve, static and dynamic", flush=True)
-    b = torch.as_tensor(prob.b).cuda()
+def test_c2_full_solve_true_residuals(c2):
+    print("case c2: full solve, graph and direct launches", flush=True)
+    b = torch.as_tensor(c2.b).cuda()
     xs = []
     for graph in (1, 0):
-        pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, graph=graph))
-        k, x, rep = pl.solve(b, prob.sigma, prob.seed)
+        pl = Plan(c2.A, c2.M, make_params(tol=c2.tol, graph=graph))
+        k, x, rep = pl.solve(b, c2.sigma, c2.seed)
         _, stop, sw = pl.report()
-        tr = pl.true_residuals(b, prob.sigma, x)
+        tr = pl.true_residuals(b, c2.sigma, x)
         stopped = stop > 0
         assert rep["stop_count"] == int(stopped.sum())
-        if name == "c1":
-            assert np.all(stopped) and k < prob.max_iter + 1
-        else:
-            assert np.all(stopped[prob.sigma >= 0.025])
-            assert k == prob.max_iter + 1 if not np.all(stopped) else k < prob.max_iter + 1
-        assert tr[stopped].max() <= 10 * prob.tol, tr[stopped].max()
+        assert np.all(stopped[c2.sigma >= 0.025])
+        assert k == c2.max_iter + 1 if not np.all(stopped) else k < c2.max_iter + 1
+        assert tr[stopped].max() <= 10 * c2.tol, tr[stopped].max()
         xs.append(x)
         pl.close()
     assert torch.equal(xs[0], xs[1])

@@ -289,3 +289,73 @@ def test_coo2csr_numpy_stable_argsort():
 def test_coo2csr_rejects_row_out_of_range():
     with pytest.raises(ValueError):
         O.coo2csr(3, [0, 3], [0, 0], [1.0, 2.0])
+
+
+# ---- true residual, both directions -------------------------------------------
+def test_true_residuals_match_scipy_on_unconverged_x(c0):
+    """||b - (A + sigma_j I) x_j|| / ||b|| (src/solver_dynamic.c:405-421) on
+    random, far-from-converged x_j, where every residual is O(1), against
+    scipy's CSR product and numpy norms to 1e-13: a dropped sigma term, a
+    sign error, a square where a root belongs or the norm of b taken
+    squared all move the value by O(1).  A 1-D Laplacian with a closed-form
+    residual checks the same function without scipy."""
+    rng = np.random.default_rng(31)
+    x = rng.uniform(-1, 1, (c0.M, c0.n))
+    got = O.true_residuals(c0.A, x, c0.b, c0.sigma)
+    S = c0.A.to_scipy()
+    want = np.array([np.linalg.norm(c0.b - (S @ x[j] + c0.sigma[j] * x[j])) / np.linalg.norm(c0.b)
+                     for j in range(c0.M)])
+    assert want.min() > 0.5
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=0)
+    # tridiag(-1, 2, -1), x = 1: (A x)_i = 1 at both ends, 0 inside; with
+    # b = e_0 the residual of shift s is b - A x - s x, whose norm squared is
+    # (1 - 1 - s)^2 + (n - 2) s^2 + (1 + s)^2
+    n = 50
+    rows = np.arange(n)
+    ptr = np.zeros(n + 1, dtype=np.uint32)
+    cols, vals = [], []
+    for i in rows:
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                cols.append(j)
+                vals.append(v)
+        ptr[i + 1] = len(cols)
+    T = P.CSR(ptr, np.array(cols, dtype=np.uint32), np.array(vals), n)
+    sig = np.array([0.0, 0.25, 2.0])
+    b = np.zeros(n)
+    b[0] = 1.0
+    got = O.true_residuals(T, np.ones((3, n)), b, sig)
+    want = np.sqrt(sig ** 2 + (n - 2) * sig ** 2 + (1 + sig) ** 2)
+    np.testing.assert_allclose(got, want, rtol=1e-15)
+
+
+# ---- breakdown semantics -----------------------------------------------------
+def test_breakdowns_run_to_the_cap(c0):
+    """The reference does not trap a breakdown: b = 0 gives (r^, s) = 0 and
+    alpha~ = 0/0 at the first step (src/solver_dynamic.c:167); with A = 2I
+    and sigma_s = 0, alpha~ = (b,b)/(b,2b) = 1/2 exactly, q = 0 and
+    omega~ = 0/0 (:189).  NaN then meets no stopping rule and the loop runs
+    to max_iter + 1."""
+    r = O.solve(c0.A, np.zeros(c0.n), c0.sigma, 0, tol=c0.tol, max_iter=25)
+    assert r.k == 26 and np.all(np.isnan(r.x)) and r.status == 0
+    n = 300
+    A = P.CSR(np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32), np.full(n, 2.0), n)
+    r = O.solve(A, P.uniform(9, 1, n, -1.0, 1.0), np.array([0.0, 1.0]), 0, tol=1e-10, max_iter=12)
+    assert r.k == 13 and r.alpha[1] == 0.5 and np.isnan(r.omega[1])
+    assert np.all(np.isnan(r.x))
+
+
+def test_nan_shift_ends_without_switch_target(c0):
+    """A NaN shift's estimate is never above the seed's, so when the seed
+    stops there is no shift to switch to (the reference's max_sigma is then
+    uninitialised): the solve ends after that iteration with the
+    NO_SWITCH_TARGET status, the finite shifts solved as without the NaN
+    one."""
+    sigma = np.array([c0.sigma[0], np.nan, c0.sigma[5]])
+    r = O.solve(c0.A, c0.b, sigma, 0, tol=c0.tol)
+    clean = O.solve(c0.A, c0.b, sigma[[0, 2]], 0, tol=c0.tol)
+    assert r.status == O.NO_SWITCH_TARGET
+    assert r.k == r.stop_iter[0] + 1 and r.stop_iter[1] == 0
+    assert r.stop_iter[0] == clean.stop_iter[0] and r.stop_iter[2] == clean.stop_iter[1]
+    assert np.array_equal(r.x[[0, 2]], clean.x[:, :])
+    assert np.all(np.isnan(r.x[1]))
