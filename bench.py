@@ -62,6 +62,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Physical cores of this host from lscpu (sockets x cores per socket),
+    the CPUs this process may run on, and the model name."""
+    import subprocess
+    info = {"cpus_usable": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["physical_cores"] = int(kv["Socket(s)"]) * int(kv["Core(s) per socket"])
+        info["model"] = kv.get("Model name")
+    except Exception:   # lscpu missing or unparsable: report what is known
+        pass
+    return info
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed
     rocprofv3 counter summary (profiles/pmc_traffic.json), or None."""
@@ -87,7 +106,12 @@ def parse():
     ap.add_argument("--value-index", type=int, default=1, choices=[0, 1],
                     help="1 (library default): SpMV reads the matrix through its value table "
                          "(and diagonal tiles) where it qualifies; 0: 8-byte CSR values")
-    ap.add_argument("--cpu-iters", type=int, default=40, help="oracle iterations timed (0: skip)")
+    ap.add_argument("--fused", type=int, default=1, choices=[0, 1],
+                    help="1 (library default): five launches per iteration (finalizes and combined "
+                         "coefficients run inside the neighbouring launches); 0: eight")
+    ap.add_argument("--cpu-iters", type=int, default=2000,
+                    help="iteration cap of the timed oracle solve (the workload's own cap, 2000, "
+                         "times the complete solve; 0: skip)")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-problem batched leg")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -99,10 +123,13 @@ def parse():
                     help="skip the host-buffer (PCIe-inclusive) drop-in call of the headline workload")
     ap.add_argument("--no-powerlaw", action="store_true",
                     help="skip the 1e6-row power-law matrix leg (128 shifts)")
-    ap.add_argument("--mode", default="headline", choices=["headline", "sweep", "repeat"],
+    ap.add_argument("--mode", default="headline", choices=["headline", "sweep", "repeat", "powerlaw"],
                     help="headline: the 128^3 / 64-shift solve and its legs; sweep: shift counts "
                          "1..--sweep-max x {normal, dynamic} (src/main_seed_diff.c); repeat: one "
-                         "solve of --repeat-shifts shifts timed --repeats times (src/main_repeat.c)")
+                         "solve of --repeat-shifts shifts timed --repeats times (src/main_repeat.c); "
+                         "powerlaw: the power-law leg alone (for counter runs)")
+    ap.add_argument("--pl-iters", type=int, default=0,
+                    help="powerlaw mode: iteration cap of its solves (0: the workload's 2000)")
     ap.add_argument("--sweep-grid", type=int, default=64, help="3-D grid side of the sweep's operator")
     ap.add_argument("--sweep-max", type=int, default=1024)
     ap.add_argument("--repeat-shifts", type=int, default=1024)
@@ -257,6 +284,8 @@ def powerlaw_leg(args, log):
     from hbicg import problems as P
     t = time.perf_counter()
     prob = P.config2()
+    if args.pl_iters > 0:
+        prob.max_iter = args.pl_iters
     log(f"power-law matrix generated in {time.perf_counter() - t:.1f} s")
     dev = torch.device("cuda:0")
     b = torch.as_tensor(prob.b).to(dev)
@@ -287,7 +316,8 @@ def powerlaw_leg(args, log):
     log(f"power-law: {ms:.1f} ms for {k - 1} iterations, {rep['stop_count']}/{prob.M} shifts stopped")
     return {"workload": f"power-law rows on [1,2000] (n={prob.A.n}, nnz={prob.A.nnz}), {prob.M} shifts "
                         f"uniform in [0,0.5], relres {prob.tol:g}, cap {prob.max_iter}",
-            "ms_per_solve": round(ms, 2), "iterations": k - 1, "ms_per_iteration": round(ms / (k - 1), 4),
+            "shifts": prob.M, "ms_per_solve": round(ms, 2), "iterations": k - 1,
+            "ms_per_iteration": round(ms / (k - 1), 4),
             "shifts_stopped": rep["stop_count"], "value_table": rep["value_table"], "spmv": spmv}
 
 
@@ -452,6 +482,15 @@ def main():
         return sweep_mode(args)
     if args.mode == "repeat":
         return repeat_mode(args)
+    if args.mode == "powerlaw":
+        out = powerlaw_leg(args, log)
+        print(json.dumps({
+            "metric": METRIC, "value": round(out["shifts"] / (out["ms_per_solve"] * 1e-3), 3),
+            "unit": "shift-solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": out["ms_per_solve"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": out["workload"], "mode": "powerlaw"}, "powerlaw": out}))
+        return
     import numpy as np
     import torch
 
@@ -494,7 +533,7 @@ def main():
     graph = 1 if args.variant == "dynamic" else 0
     window = args.window if args.window > 0 else (1 if args.variant == "normal" else 0)
     params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph,
-                         value_index=bool(args.value_index))
+                         value_index=bool(args.value_index), fused=bool(args.fused))
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)
     x = torch.zeros((M, n), dtype=torch.float64, device=dev)
@@ -528,7 +567,8 @@ def main():
     general = None
     if value_table or not args.value_index:
         gplan = Plan(prob.A, M, make_params(tol=prob.tol, max_iter=prob.max_iter, window=window,
-                                            graph=graph, value_index=not value_table), device=dev)
+                                            graph=graph, value_index=not value_table,
+                                            fused=bool(args.fused)), device=dev)
         xg = torch.zeros_like(x)
         gplan.solve(b, prob.sigma, prob.seed, xg)
         torch.cuda.synchronize()
@@ -564,7 +604,7 @@ def main():
     kernels, roofline, flush_roofline = {}, None, None
     if not args.no_profile:
         pparams = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=0,
-                              profile=True, value_index=bool(args.value_index))
+                              profile=True, value_index=bool(args.value_index), fused=bool(args.fused))
         pplan = Plan(prob.A, M, pparams, device=dev)
         xp = torch.zeros_like(x)
         pplan.solve(b, prob.sigma, prob.seed, xp)
@@ -607,20 +647,18 @@ def main():
     cpu = None
     if args.cpu_iters > 0:
         import pyoracle as O
-        S = args.cpu_iters
-        log(f"cpu baseline: {S} oracle iterations on {O.num_threads()} threads")
-        ores = O.solve(prob.A, prob.b, prob.sigma, prob.seed, tol=prob.tol, max_iter=S)
-        done = ores.k - 1
-        seed_per_iter = ores.t_seed / done
-        shift_per_update = ores.t_shift / (done * (M - 1))
-        check_per_iter = ores.t_check / done
-        t_full = (seed_per_iter + check_per_iter) * iters + shift_per_update * updates
+        hc = host_cores()
+        log(f"cpu baseline: the oracle's solve on {O.num_threads()} threads (host {hc})")
+        t0c = time.perf_counter()
+        ores = O.solve(prob.A, prob.b, prob.sigma, prob.seed, tol=prob.tol, max_iter=args.cpu_iters)
+        t_full = time.perf_counter() - t0c
+        complete = ores.k < args.cpu_iters + 1 or bool(np.all(ores.stop_iter > 0))
         cpu = {"value": round(M / t_full, 4), "unit": "shift-solves/s", "cores": O.num_threads(),
-               "kind": "port",
-               "sample": (f"oracle (C/OpenMP restatement of the reference) on this workload for "
-                          f"{done} iterations ({ores.t_seed + ores.t_shift + ores.t_check:.1f} s); "
-                          f"seed-step and per-shift-update timers extrapolated to the GPU run's "
-                          f"{iters} iterations and {updates} shift updates -> {t_full:.1f} s per solve")}
+               "kind": "port", "host": hc,
+               "sample": (f"one {'complete ' if complete else ''}solve of this workload by the oracle "
+                          f"(C/OpenMP restatement of the reference, one rank, {O.num_threads()} OpenMP "
+                          f"threads): {ores.k - 1} iterations, {t_full:.1f} s wall "
+                          f"(seed steps {ores.t_seed:.1f} s, shift updates {ores.t_shift:.1f} s)")}
 
     batched = None
     if not args.no_batched:
@@ -654,6 +692,7 @@ def main():
         "config": {"workload": f"conv-diff 3-D 7-point {args.grid}^3 (n={n}, nnz={prob.A.nnz}), "
                                f"{M} shifts log-spaced [1e-3,1], relres {prob.tol:g}, seed=smallest shift",
                    "variant": args.variant, "window": window, "hipgraph": bool(graph),
+                   "fused_launches": bool(args.fused),
                    "spmv_value_table": value_table, "spmv_diag_tiles": diag_tiles},
         "other_spmv_format": general,
         "iterations": iters,

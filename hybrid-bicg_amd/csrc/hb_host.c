@@ -45,6 +45,7 @@ void hbicg_default_params(hbicg_params *p)
     p->trace = 0;
     p->value_index = 1;
     p->q_mode = 0;
+    p->fused = 1;
 }
 
 static const hbicg_params *params(void)
@@ -452,7 +453,7 @@ out:
  * SpMV may read the matrix through a value table (not in batches). */
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
                                  const hbicg_params *p, void *stream, int variant, int w_alloc,
-                                 int w_cap, int allow_vi)
+                                 int w_cap, int allow_vi, int batched)
 {
     int rc = 0;
     unsigned *hptr = NULL, *hblk = NULL;
@@ -562,7 +563,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
         pl->flush_host = (int *)calloc(cap, sizeof(int));
         if (!pl->flush_host) rc = HBICG_E_NOMEM;
     }
-    pl->part = (double *)dalloc((size_t)maxparts * 2 * 8, &rc);
+    pl->part = (double *)dalloc((size_t)maxparts * 6 * 8, &rc);   /* three phases' partials */
     pl->p_snap = (double *)dalloc(nb, &rc);
     pl->stop = (int *)dalloc((size_t)M * 4, &rc);
     pl->stop_iter = (int *)dalloc((size_t)M * 4, &rc);
@@ -604,6 +605,10 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->stop_iter = pl->stop_iter;
     d->act_len = pl->act_len; d->flist = pl->flist; d->fpos = pl->fpos; d->switch_k = pl->switch_k;
     d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st; d->p_snap = pl->p_snap;
+    d->part2 = pl->part + (size_t)maxparts * 2;
+    d->part3 = pl->part + (size_t)maxparts * 4;
+    /* batches launch each phase over all their problems' tiles unfused */
+    d->fused = !pl->exact && p->fused && !batched;
 
     /* 16-bit column offsets for the tiles whose entries stay within 32767
      * columns of the tile's first row */
@@ -655,7 +660,7 @@ hbicg_plan *hbicg_plan_create(const hbicg_csr *A, int a_on_device, int M,
                               const hbicg_params *p, void *stream)
 {
     const hbicg_params *pp = p ? p : params();
-    return plan_create_v(A, a_on_device, M, pp, stream, 1, 0, HB_WMAX, pp->value_index);
+    return plan_create_v(A, a_on_device, M, pp, stream, 1, 0, HB_WMAX, pp->value_index, 0);
 }
 
 /* Enqueues one chunk of iterations followed by the done-flag publication. */
@@ -1066,7 +1071,7 @@ static int dropin(CSR_Matrix *A_diag, CSR_Matrix *A_offd, INFO_Matrix *A_info, d
     if (!A_diag->ptr || (A_diag->nz > 0 && (!A_diag->val || !A_diag->col))) return HBICG_E_ARG;
     const double t0 = now_ms();
     hbicg_csr A = {A_diag->val, A_diag->col, A_diag->ptr, A_diag->rows, A_diag->nz};
-    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0, HB_WMAX, p->value_index);
+    hbicg_plan *pl = plan_create_v(&A, 0, M, p, NULL, variant, 0, HB_WMAX, p->value_index, 0);
     if (!pl) return g_errno ? g_errno : HBICG_E_HIP;
     const double t1 = now_ms();
     const int k = hbicg_plan_solve(pl, r, sigma, seed, x_set, r, 0, NULL);
@@ -1240,7 +1245,7 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     size_t nsp = 0, nvec = 0, ncoef = 0;
     const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WBATCH;
     for (int q = 0; q < nprob; ++q) {
-        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax, p->value_index);
+        bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax, p->value_index, 1);
         if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
         bt->hdevs[q] = bt->plans[q]->dev;
         nsp += (size_t)bt->plans[q]->nblk;

@@ -48,6 +48,25 @@
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
+/* Constant address space: the compiler takes what is read through it as
+ * unchanged for the rest of the launch, and a wave-uniform address there
+ * becomes a scalar load. */
+#define HB_K __attribute__((address_space(4)))
+
+/* p, from here to the end of the launch read-only, as constant memory.  The
+ * empty asm makes the returned pointer unknown to the compiler until this
+ * point, so no read through it moves above what comes before (a wait for
+ * the data in the same launch).  Without it a kernel with a barrier or an
+ * atomic anywhere before the reads loses their scalar loads: the compiler
+ * cannot show that nothing in between stored to them, and makes every
+ * uniform read a vector load. */
+template <typename T>
+DEV const HB_K T *read_only(const T *p)
+{
+    asm volatile("" : "+s"(p));
+    return (const HB_K T *)p;
+}
+
 /* ---- reductions ----------------------------------------------------------- */
 DEV double wave_sum(double v)
 {
@@ -120,6 +139,69 @@ DEV double serial_dot(const double *x, const double *y, int n, double *lds)
         __syncthreads();
     }
     return acc;
+}
+
+/* ---- values handed between the workgroups of one launch ---------------------
+ * A workgroup dispatched first computes a value the launch's other
+ * workgroups need.  It stores it with an agent-scope relaxed atomic store (a
+ * write-through store, sc1, on gfx950) into a mailbox that holds
+ * HB_MAIL_EMPTY until then; the others poll it with agent-scope relaxed
+ * atomic loads (sc1), which read past any stale cache line.  Every word
+ * that crosses workgroups inside a launch moves this way, so no L2
+ * write-back or invalidate (what an agent-scope release or acquire costs)
+ * is needed.  The first workgroups of a grid are dispatched before the
+ * later ones (in order on each XCD), so the publisher is always running
+ * when a waiter spins; the spin is bounded all the same (about 0.5 s), and
+ * a waiter that gives up sets HB_ST_SYNC_TIMEOUT and goes on instead of
+ * hanging; every later waiter sees the bit and does not wait at all. */
+#define HB_SPIN_LIMIT (1 << 17)
+
+/* a waiter polls the status word too: once one has timed out, none waits */
+DEV bool sync_timed_out(hb_state *st)
+{
+    return (__hip_atomic_load(&st->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & HB_ST_SYNC_TIMEOUT) != 0;
+}
+
+DEV void mail_put(unsigned long long *slot, unsigned long long v)
+{
+    __hip_atomic_store(slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+DEV void mail_put_d(unsigned long long *slot, double v)
+{
+    mail_put(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+/* the published double; thread 0 of the workgroup waits, every thread gets it */
+DEV double mail_wait(unsigned long long *slot, hb_state *st)
+{
+    __shared__ double sh_v;
+    if (threadIdx.x == 0) {
+        unsigned long long v;
+        for (int spins = 0;; ++spins) {
+            v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v != HB_MAIL_EMPTY) break;
+            if (spins == HB_SPIN_LIMIT || ((spins & 255) == 255 && sync_timed_out(st))) {
+                atomicOr(&st->status, HB_ST_SYNC_TIMEOUT);
+                break;
+            }
+            /* a few hundred cycles between polls: thousands of resident
+             * workgroups poll one word, and the publisher needs the memory
+             * system more than they do */
+            __builtin_amdgcn_s_sleep(8);
+        }
+        sh_v = __longlong_as_double((long long)v);
+    }
+    __syncthreads();
+    return sh_v;
+}
+
+/* a double the kernel's other workgroups read before the launch ends */
+template <bool PUB>
+DEV void store_pub(double *p, double v)
+{
+    if (PUB) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
 }
 
 DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
@@ -542,9 +624,9 @@ DEV void spmv_tile(const hb_dev &d, unsigned b)
                                    d.rhat, d.s, nullptr, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
                                    d.doff, d.didx);
     else
-        spmv_body<MODE, EXACT, VI, UNF>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring, d.s, d.rhat,
-                                   d.y, d.qring, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile, d.doff,
-                                   d.didx);
+        spmv_body<MODE, EXACT, VI, UNF>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring,
+                                              d.s, d.rhat, d.y, d.qring, d.part2, 0.0, d.vidx, d.col16p, d.tpad,
+                                              d.vdict, d.dtile, d.doff, d.didx);
 }
 
 /* Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups round
@@ -625,12 +707,18 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv_plain(hb_dev d, const double 
  * omega~[k] (after SpMV 2, :183-190).  One workgroup per problem: on gfx950
  * an agent-scope release is an L2 write-back, far too costly to pay once per
  * SpMV workgroup for a last-to-arrive tail. */
-template <int MODE, bool EXACT>
+template <int MODE, bool EXACT, bool PUB = false>
 DEV void fin_spmv_body(const hb_dev &d)
 {
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     hb_state *st = d.st;
     if (st->done) return;
+    /* PUB (omega~ only): this workgroup runs first in the seed update's
+     * launch and hands omega~ to that launch's other workgroups.  The
+     * mailbox is empty already (the scalar stage empties it); emptying it
+     * again here keeps a back-to-back repeat of the launch waiting too. */
+    static_assert(!PUB || MODE == SP_SEED2, "only omega~ is published");
+    if (PUB && threadIdx.x == 0) mail_put(&st->mail_w, HB_MAIL_EMPTY);
     const int k = st->k, tid = threadIdx.x;
     const int switched = (MODE == SP_SEED1) ? st->switched : 0;
     const double rTr = st->rTr;
@@ -646,7 +734,7 @@ DEV void fin_spmv_body(const hb_dev &d)
             a1 = serial_dot(q, d.y, d.n, lds);
         }
     } else {
-        reduce_partials(d.part, d.nblk, a0, a1);
+        reduce_partials(MODE == SP_SEED1 ? d.part : d.part2, d.nblk, a0, a1);
     }
     if (tid == 0) {
         if (MODE == SP_SEED1) {
@@ -663,6 +751,7 @@ DEV void fin_spmv_body(const hb_dev &d)
             d.wa[k] = omega;
             st->rTr_old = rTr;
             if (a1 == 0.0 || !isfinite(omega)) st->status |= HB_ST_BREAKDOWN_QTY;
+            if (PUB) mail_put_d(&st->mail_w, omega);
         }
     }
 }
@@ -682,7 +771,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     __shared__ double sh_zp[HB_BLOCK];
     __shared__ int sh_j[HB_BLOCK];
     __shared__ int sh_nc[HB_BLOCK];
-    __shared__ int sh_switch, sh_spike, sh_flush, sh_bad;
+    __shared__ int sh_switch, sh_spike, sh_flush, sh_bad, sh_upd;
     __shared__ double sh_rprev;
     hb_state *st = d.st;
     const int tid = threadIdx.x;
@@ -719,8 +808,9 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
          * The set only shrinks inside a window (a seed switch closes it), so
          * list positions stay valid until the flush; coefficients and
          * update counts are stored by position, contiguous for the shift
-         * kernel's shifts in flight. */
-        int base = 0;
+         * kernel's shifts in flight.  The default mode lists the seed first
+         * (below), the shifts after it. */
+        int base = d.exact ? 0 : 1;
         for (int c0 = 0; c0 < M; c0 += HB_BLOCK) {
             const int j = c0 + tid;
             const int f = (j < M && !d.stop[j] && j != s) ? 1 : 0;
@@ -741,15 +831,18 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         }
         /* default mode: the seed's own x update (x_s += alpha~ p_s +
          * omega~ q, src/solver_dynamic.c:194-195) is deferred too, the seed
-         * listed last with the coefficients of its recurrence */
-        seedpos = d.exact ? -1 : base;
+         * listed first with the coefficients of its recurrence.  It is
+         * updated in every window step, like the slowest shifts (the
+         * smallest shifts, first in the list): grouped with them, the flush
+         * chunks of shifts that stopped early end their loops early. */
+        seedpos = d.exact ? -1 : 0;
         if (tid == 0) {
             if (seedpos >= 0) {
                 d.flist[seedpos] = s;
                 d.fpos[s] = seedpos;
             }
             st->win_seedpos = seedpos;
-            st->flush_cnt = base + (seedpos >= 0 ? 1 : 0);
+            st->flush_cnt = base;
         }
     }
 
@@ -891,6 +984,21 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     }
 
     const int len = slot + 1;
+    if (d.flush_log) {
+        /* profiled solves: window steps the listed shifts were updated in,
+         * for the flop count of a flush (the flush skips each chunk's steps
+         * after its last update) */
+        const int Lc = len - sh_spike, cnt = st->flush_cnt;
+        int u = 0;
+        for (int pos = tid; pos < cnt; pos += HB_BLOCK) u += min(d.act_len[pos], Lc);
+        sh_nc[tid] = u;
+        __syncthreads();
+        for (int w = HB_BLOCK / 2; w > 0; w >>= 1) {
+            if (tid < w) sh_nc[tid] += sh_nc[tid + w];
+            __syncthreads();
+        }
+        if (tid == 0) sh_upd = sh_nc[0];
+    }
     if (tid == 0) {
         const int done_next = (st->stop_count >= M) || (k + 1 >= d.cap) ||
                               (st->status & HB_ST_NO_SWITCH_TARGET);
@@ -925,11 +1033,11 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         if (fl) bytes += 8.0 * (2 * len + 1) + (xonly ? 16.0 : 32.0) * st->flush_cnt;
         st->shift_bytes += bytes * (double)d.n;
         /* multiply-adds per row and listed shift: x_j takes p_j, r_{k0-1}
-         * and two window vectors per iteration; p_j as many again; a tail
-         * iteration applied on its own six more */
-        if (fl)
-            st->flush_fma += (double)st->flush_cnt *
-                             ((2.0 * (len - spike) + 2.0) * (xonly ? 1.0 : 2.0) + (spike ? 6.0 : 0.0));
+         * and two window vectors per iteration it was updated in; p_j as
+         * many again; a tail iteration applied on its own six more */
+        if (fl && d.flush_log)
+            st->flush_fma += (2.0 * (double)sh_upd + 2.0 * st->flush_cnt) * (xonly ? 1.0 : 2.0) +
+                             (spike ? 6.0 * st->flush_cnt : 0.0);
         if (d.flush_log && k < d.cap) d.flush_log[k] = fl;
     }
     if (d.growth_limit > 0.0) {
@@ -942,15 +1050,17 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
 }
 
 /* ---- seed update: x_s, r, (r,r), (r^,r) (src/solver_dynamic.c:194-204) --- */
+template <bool MAIL = false>
 DEV void seed_update_body(const hb_dev &d, int b, const double *__restrict__ qring,
                           const double *__restrict__ y, const double *__restrict__ pset,
                           double *__restrict__ xset, const double *__restrict__ rhat,
                           double *__restrict__ rring, double *__restrict__ part)
 {
-    const hb_state *st = d.st;
+    hb_state *st = d.st;
     if (st->done) return;
     const int k = st->k, s = st->seed, n = d.n, nb = d.vec_grid;
-    const double a = d.aa[k], w = d.wa[k], nw = -d.wa[k];
+    /* MAIL: omega~ is finalized by the launch's first workgroup */
+    const double a = d.aa[k], w = MAIL ? mail_wait(&st->mail_w, st) : d.wa[k], nw = -w;
     const double *q = qring + off(k % d.W, n);
     double *rn = rring + off(k % (d.W + 1), n);
     const double *ps = pset + off(s, n);
@@ -977,14 +1087,27 @@ DEV void seed_update_body(const hb_dev &d, int b, const double *__restrict__ qri
     if (threadIdx.x == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
 }
 
+template <bool MAIL = false>
 DEV void seed_update_tile(const hb_dev &d, int b)
 {
-    seed_update_body(d, b, d.qring, d.y, d.p_set, d.x_set, d.rhat, d.rring, d.part);
+    seed_update_body<MAIL>(d, b, d.qring, d.y, d.p_set, d.x_set, d.rhat, d.rring, d.part3);
 }
 
 __global__ __launch_bounds__(HB_BLOCK) void k_seed_update(hb_dev d)
 {
     seed_update_tile(d, blockIdx.x);
+}
+
+/* Fused seed update (default mode): workgroup 0 finalizes omega~ from SpMV
+ * 2's partials and publishes it; the others run the update once it is
+ * there. */
+__global__ __launch_bounds__(HB_BLOCK) void k_seed_update_fin(hb_dev d)
+{
+    if (blockIdx.x == 0) {
+        fin_spmv_body<SP_SEED2, false, true>(d);
+        return;
+    }
+    seed_update_tile<true>(d, blockIdx.x - 1);
 }
 
 /* (r,r), (r^,r) from the seed-update partials, then the scalar stage. */
@@ -993,7 +1116,7 @@ DEV void fin_seed_body(const hb_dev &d)
 {
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     __shared__ double sh[2];
-    const hb_state *st = d.st;
+    hb_state *st = d.st;
     if (st->done) return;
     double dr, rr;
     if (EXACT) {
@@ -1001,9 +1124,16 @@ DEV void fin_seed_body(const hb_dev &d)
         dr = serial_dot(rn, rn, d.n, lds);
         rr = serial_dot(d.rhat, rn, d.n, lds);
     } else {
-        reduce_partials(d.part, d.vec_grid, dr, rr);
+        reduce_partials(d.part3, d.vec_grid, dr, rr);
     }
-    if (threadIdx.x == 0) { sh[0] = dr; sh[1] = rr; }
+    if (threadIdx.x == 0) {
+        sh[0] = dr;
+        sh[1] = rr;
+        /* the fused launches' readers of omega~ and of the coefficient
+         * count finished with the last launches */
+        st->mail_w = HB_MAIL_EMPTY;
+        st->coef_ready = 0;
+    }
     __syncthreads();
     scalar_stage(d, sh[0], sh[1]);
 }
@@ -1262,10 +1392,11 @@ DEV dd dd_mul(dd a, double c)
     const double e = __builtin_fma(a.hi, c, -p);
     return quick_two_sum(p, __builtin_fma(a.lo, c, e));
 }
+template <bool PUB = false>
 DEV void flush_coef_body(const hb_dev &d, int pos, const double *__restrict__ coef,
                          double *__restrict__ ccoef)
 {
-    const hb_state *st = d.st;
+    hb_state *st = d.st;
     if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     if (!st->flush || pos >= st->flush_cnt) return;
     const int L = st->flush_len, M = d.M, W = d.W;
@@ -1281,7 +1412,7 @@ DEV void flush_coef_body(const hb_dev &d, int pos, const double *__restrict__ co
 #pragma unroll
         for (int e = 0; e < 6; ++e) {
             const double id = e == 4 ? 1.0 : 0.0;
-            blk[base + (e / 2) * (2 * HB_UG) + (e % 2) * HB_UG] = on ? c[e] : id;
+            store_pub<PUB>(&blk[base + (e / 2) * (2 * HB_UG) + (e % 2) * HB_UG], on ? c[e] : id);
         }
     }
     const int nb = min(lv, Lc);
@@ -1304,9 +1435,17 @@ DEV void flush_coef_body(const hb_dev &d, int pos, const double *__restrict__ co
             pk = pn;
         }
         const int row = pch ? 0 : 1 + kc;
-        blk[(size_t)row * (2 * HB_UG) + pos % HB_UG] = xk.hi + xk.lo;
-        blk[(size_t)row * (2 * HB_UG) + HB_UG + pos % HB_UG] = pk.hi + pk.lo;
+        store_pub<PUB>(&blk[(size_t)row * (2 * HB_UG) + pos % HB_UG], xk.hi + xk.lo);
+        store_pub<PUB>(&blk[(size_t)row * (2 * HB_UG) + HB_UG + pos % HB_UG], pk.hi + pk.lo);
         if (pch) break;
+    }
+    if (PUB) {
+        /* every store of this workgroup is done before the count moves (the
+         * flush workgroups of this launch read the block once the count
+         * reaches the list length) */
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(&st->coef_ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1336,18 +1475,31 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
+    const HB_K double *cc = read_only(ccoef);
+    const HB_K int *fl = read_only(flist), *al = read_only((const int *)d.act_len);
     const size_t cstride = (size_t)HB_CC_ROWS(W) * (2 * UG);
     for (int sub = 0; sub < HB_BLOCK / 64; ++sub) {
         const int i = bx * HB_BLOCK + sub * 64 + lane;
         if (i >= n) break;
         for (int c = wave; c < nch; c += HB_BLOCK / 64) {
-            const double *__restrict__ cb = ccoef + (size_t)c * cstride;
+            const HB_K double *cb = cc + (size_t)c * cstride;
             const int nu = min(UG, cnt - c * UG);
+            /* a shift's coefficients of window steps after the last one it
+             * was updated in are exactly zero: the chunk's loop stops at its
+             * latest such step (shifts stopping at similar iterations sit in
+             * the same chunk, since the list is ascending and the stop
+             * iteration follows the shift) */
+            int nbc = 0;
+            for (int u = 0; u < nu; ++u) nbc = max(nbc, min(al[c * UG + u], Lc));
+            /* uniform across the wavefront: the loop index, and with it the
+             * coefficient addresses, stay scalar */
+            nbc = __builtin_amdgcn_readfirstlane(nbc);
+            if (tail) nbc = Lc;   /* the tail starts from the ring slots the loop ends on */
             double x[UG], p[UG];
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
-                    const int j = flist[c * UG + u];
+                    const int j = fl[c * UG + u];
                     x[u] = __builtin_nontemporal_load(xset + off(j, n) + i);
                     /* the seed's window-start p is the snapshot (its p_s
                      * has moved on every iteration) */
@@ -1365,19 +1517,19 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
             int rs = (k0 - 1) % (W + 1), qs = k0 % W;
             double gr = rring[off(rs, n) + i];
             {
-                const double *__restrict__ ck = cb + 2 * UG;
+                const HB_K double *ck = cb + 2 * UG;
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[u], gr, x[u]);
                     p[u] = __builtin_fma(ck[UG + u], gr, p[u]);
                 }
             }
-            for (int b = 0; b < Lc; ++b) {
+            for (int b = 0; b < nbc; ++b) {
                 rs = rs + 1 == W + 1 ? 0 : rs + 1;
                 const double gq = qring[off(qs, n) + i];
                 gr = rring[off(rs, n) + i];
                 qs = qs + 1 == W ? 0 : qs + 1;
-                const double *__restrict__ ck = cb + (size_t)(2 * b + 2) * (2 * UG);
+                const HB_K double *ck = cb + (size_t)(2 * b + 2) * (2 * UG);
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x[u] = __builtin_fma(ck[u], gq, x[u]);
@@ -1394,7 +1546,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
                 rs = rs + 1 == W + 1 ? 0 : rs + 1;
                 const double gq = qring[off(qs, n) + i];
                 const double gn = rring[off(rs, n) + i];
-                const double *__restrict__ ct = cb + (size_t)(2 * W + 2) * (2 * UG);
+                const HB_K double *ct = cb + (size_t)(2 * W + 2) * (2 * UG);
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     double xe = __builtin_fma(ct[u], gq, x[u]);
@@ -1409,7 +1561,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
-                    const int j = flist[c * UG + u];
+                    const int j = fl[c * UG + u];
                     __builtin_nontemporal_store(x[u], xset + off(j, n) + i);
                     /* the seed keeps its own p_s (the regrouped one is
                      * only its x's companion) */
@@ -1437,6 +1589,8 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
+    const HB_K double *cc = read_only(ccoef);
+    const HB_K int *fl = read_only(flist), *al = read_only((const int *)d.act_len);
     const size_t cstride = (size_t)HB_CC_ROWS(W) * (2 * UG);
     for (int sub = 0; sub < HB_BLOCK / 64; sub += 2) {
         const int i0 = bx * HB_BLOCK + sub * 64 + lane;
@@ -1444,13 +1598,23 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
         const bool has1 = i0 + 64 < n;
         const int i1 = has1 ? i0 + 64 : i0;   /* a clamped duplicate when past the end */
         for (int c = wave; c < nch; c += HB_BLOCK / 64) {
-            const double *__restrict__ cb = ccoef + (size_t)c * cstride;
+            const HB_K double *cb = cc + (size_t)c * cstride;
             const int nu = min(UG, cnt - c * UG);
+            /* a shift's coefficients of window steps after the last one it
+             * was updated in are exactly zero: the chunk's loop stops at its
+             * latest such step (shifts stopping at similar iterations sit in
+             * the same chunk, since the list is ascending and the stop
+             * iteration follows the shift) */
+            int nbc = 0;
+            for (int u = 0; u < nu; ++u) nbc = max(nbc, min(al[c * UG + u], Lc));
+            /* uniform across the wavefront: the loop index, and with it the
+             * coefficient addresses, stay scalar */
+            nbc = __builtin_amdgcn_readfirstlane(nbc);
             double x0[UG], x1[UG];
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
-                    const int j = flist[c * UG + u];
+                    const int j = fl[c * UG + u];
                     const double *pj = c * UG + u == sp ? d.p_snap : pset + off(j, n);
                     x0[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i0),
                                           __builtin_nontemporal_load(xset + off(j, n) + i0));
@@ -1465,19 +1629,19 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
             {
                 const double *gp = rring + off(rs, n);
                 const double g0 = gp[i0], g1 = gp[i1];
-                const double *__restrict__ ck = cb + 2 * UG;
+                const HB_K double *ck = cb + 2 * UG;
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x0[u] = __builtin_fma(ck[u], g0, x0[u]);
                     x1[u] = __builtin_fma(ck[u], g1, x1[u]);
                 }
             }
-            for (int b = 0; b < Lc; ++b) {
+            for (int b = 0; b < nbc; ++b) {
                 rs = rs + 1 == W + 1 ? 0 : rs + 1;
                 const double *qp = qring + off(qs, n), *rp = rring + off(rs, n);
                 const double q0 = qp[i0], q1 = qp[i1], r0 = rp[i0], r1 = rp[i1];
                 qs = qs + 1 == W ? 0 : qs + 1;
-                const double *__restrict__ ck = cb + (size_t)(2 * b + 2) * (2 * UG);
+                const HB_K double *ck = cb + (size_t)(2 * b + 2) * (2 * UG);
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     x0[u] = __builtin_fma(ck[u], q0, x0[u]);
@@ -1492,7 +1656,7 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
-                    const int j = flist[c * UG + u];
+                    const int j = fl[c * UG + u];
                     __builtin_nontemporal_store(x0[u], xset + off(j, n) + i0);
                     if (has1) __builtin_nontemporal_store(x1[u], xset + off(j, n) + i1);
                 }
@@ -1504,13 +1668,13 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
 /* Tile bx: every thread first applies the seed p update of the finished
  * iteration to its element (src/solver_dynamic.c:208-210), then the tile
  * takes part in the window flush when one is due. */
-template <int UG>
+template <int UG, bool FUSED = false>
 DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
                        double *__restrict__ pset, const double *__restrict__ rring,
                        const double *__restrict__ qring, const double *__restrict__ svec,
                        const double *__restrict__ ccoef, const int *__restrict__ flist)
 {
-    const hb_state *st = d.st;
+    hb_state *st = d.st;
     if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
     const int n = d.n, W = d.W;
     const double pb = st->pb, pbw = st->pbw;
@@ -1544,18 +1708,54 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
         }
     }
     if (!st->flush) return;
+    if (FUSED) {
+        /* the combined coefficients come from the first workgroups of this
+         * launch: wait until every listed position has stored its block.
+         * This workgroup reads no coefficient before, so no cache holds a
+         * stale copy of one */
+        if (threadIdx.x == 0) {
+            const int want = st->flush_cnt;
+            for (int spins = 0;; ++spins) {
+                if (__hip_atomic_load(&st->coef_ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) break;
+                if (spins == HB_SPIN_LIMIT / 16 || ((spins & 15) == 15 && sync_timed_out(st))) {
+                    atomicOr(&st->status, HB_ST_SYNC_TIMEOUT);
+                    break;
+                }
+                /* the coefficients take tens to hundreds of microseconds (a
+                 * long window's last flush): every resident tile waits here,
+                 * and polls that come every few microseconds leave the memory
+                 * system to the workgroups forming them */
+                __builtin_amdgcn_s_sleep(127);
+            }
+        }
+        __syncthreads();
+    }
     if (st->flush_xonly) flush_apply_x2<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
     else flush_apply<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
 }
 
+template <bool FUSED = false>
 DEV void shift_cc_tile(const hb_dev &d, int bx)
 {
-    shift_cc_body<HB_UG>(d, bx, d.x_set, d.p_set, d.rring, d.qring, d.s, d.ccoef, d.flist);
+    shift_cc_body<HB_UG, FUSED>(d, bx, d.x_set, d.p_set, d.rring, d.qring, d.s, d.ccoef, d.flist);
 }
 
 __global__ __launch_bounds__(HB_BLOCK) void k_shift_cc(hb_dev d)
 {
     shift_cc_tile(d, blockIdx.x);
+}
+
+/* Fused shift kernel (default mode): workgroups 0..M-1 form the combined
+ * coefficients of list positions 0..M-1 when a flush is due (at once
+ * otherwise); the rest are the 256-element tiles of the seed p update and
+ * the flush. */
+__global__ __launch_bounds__(HB_BLOCK) void k_shift_fused(hb_dev d)
+{
+    if ((int)blockIdx.x < d.M) {
+        flush_coef_body<true>(d, blockIdx.x, d.coef, d.ccoef);
+        return;
+    }
+    shift_cc_tile<true>(d, blockIdx.x - d.M);
 }
 
 /* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
@@ -1624,6 +1824,8 @@ DEV void init_fin_body(const hb_dev &d)
         st->zscale = 1.0;
         st->shift_bytes = 0.0;
         st->flush_fma = 0.0;
+        st->coef_ready = 0;
+        st->mail_w = HB_MAIL_EMPTY;
     }
 }
 
@@ -1828,9 +2030,38 @@ int hb_k_init(const hb_dev *d, hipStream_t s)
     return (int)hipGetLastError();
 }
 
+/* The fused default-mode iteration: six launches (SpMV 1, its finalize,
+ * SpMV 2, the seed update with the omega~ finalize as its first workgroup,
+ * the scalar stage, the shift kernel with the combined coefficients in its
+ * first workgroups).  SpMV 2 keeps its finalize as a launch of its own:
+ * reading SpMV 1's 8192 partials takes a workgroup about 4 us, in which a
+ * fused launch's first round of tiles would have nothing to do.  The
+ * stages folded into another launch enqueue nothing. */
+static int stage_fused(const hb_dev *d, int stage, hipStream_t s)
+{
+    const dim3 bl(HB_BLOCK);
+    switch (stage) {
+    case HB_STAGE_SEED:
+        hipLaunchKernelGGL(k_seed_update_fin, dim3(d->vec_grid + 1), bl, 0, s, *d);
+        break;
+    case HB_STAGE_SHIFT:
+        hipLaunchKernelGGL(k_shift_fused, dim3(d->M + d->shift_gx), bl, 0, s, *d);
+        break;
+    case HB_STAGE_FIN2:
+    case HB_STAGE_COEF:
+        break;
+    default:
+        return -1;
+    }
+    return (int)hipGetLastError();
+}
+
 int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
 {
     const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK), one(1);
+    if (d->fused && !d->exact &&
+        (stage == HB_STAGE_FIN2 || stage == HB_STAGE_SEED || stage == HB_STAGE_COEF || stage == HB_STAGE_SHIFT))
+        return stage_fused(d, stage, s);
     switch (stage) {
     case HB_STAGE_SPMV1:
         if (d->exact) {

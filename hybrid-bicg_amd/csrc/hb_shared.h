@@ -29,6 +29,13 @@ extern "C" {
 #define HB_ST_BREAKDOWN_QTY   2   /* (q,y) == 0 or non-finite omega           */
 #define HB_ST_NO_SWITCH_TARGET 4  /* seed stopped but no shift qualified      */
 #define HB_ST_BREAKDOWN_SHIFT  8  /* a shift's 1 - omega~ d or pi_k is 0 or non-finite */
+#define HB_ST_SYNC_TIMEOUT    16  /* a workgroup gave up waiting for a value another
+                                     workgroup of its launch publishes (never expected:
+                                     the publishing workgroups are dispatched first) */
+
+/* A mailbox word that holds no value yet: a signalling-NaN bit pattern, which
+ * no arithmetic produces (NaN results are quiet). */
+#define HB_MAIL_EMPTY 0x7ff4dead00c0ffeeull
 
 /*
  * Device-resident state of one shifted solve.  Every scalar the reference
@@ -66,6 +73,12 @@ typedef struct hb_state {
     int flush_xonly;   /* 1: last flush of the solve: x_j only, p_j not needed */
     int win_seedpos;   /* default mode: list position of the seed in the pending
                           window (its x is deferred like a shift's), else -1 */
+    int coef_ready;    /* fused shift kernel: list positions whose combined
+                          coefficients are stored (reset by the scalar stage) */
+    /* Mailbox of the fused seed update: omega~ of the running iteration,
+     * published by the launch's first workgroup to its other workgroups;
+     * HB_MAIL_EMPTY until then (the scalar stage empties it). */
+    unsigned long long mail_w;
 } hb_state;
 
 /* Every pointer below addresses global device memory.  Device code sees
@@ -83,6 +96,9 @@ typedef struct hb_state {
 typedef struct hb_dev {
     int n, M, W, cap, nblk;    /* rows, shifts, window, archive length, SpMV workgroups */
     int exact;                 /* 1: reference summation order everywhere  */
+    int fused;                 /* default mode: the omega~ finalize runs as the first
+                                  workgroup of the seed update's launch, the combined
+                                  coefficients as the first workgroups of the shift kernel */
     int q_unfused;             /* 1: SpMV 2 gathers q, formed beforehand by k_qform
                                   (matrices with scattered columns); 0: it forms
                                   q = r - alpha s at every gathered column       */
@@ -127,7 +143,10 @@ typedef struct hb_dev {
     HB_G int    *flist;        /* [M] the window's shift list, ascending   */
     HB_G int    *fpos;         /* [M] position of shift j in the list      */
     HB_G int    *switch_k, *switch_to; /* [M] switch log                   */
-    HB_G double *part;         /* workgroup partial sums [max(nblk,vec_grid)*2] */
+    HB_G double *part;         /* workgroup partial sums [max(nblk,vec_grid)*2] of the
+                                  initialisation and SpMV 1 */
+    HB_G double *part2;        /* the same for SpMV 2 */
+    HB_G double *part3;        /* the same for the seed update */
     HB_G double *p_snap;       /* [n] the seed's p at the pending window's start */
     HB_G int *flush_log;       /* [cap] params.profile: 1 where iteration k's shift
                                   kernel applied a window flush, else NULL       */

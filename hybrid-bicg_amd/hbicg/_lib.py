@@ -45,7 +45,7 @@ class Params(C.Structure):
                 ("exact_order", C.c_int), ("graph", C.c_int), ("chunk", C.c_int),
                 ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int),
                 ("profile", C.c_int), ("growth_limit", C.c_double), ("trace", C.c_int),
-                ("value_index", C.c_int), ("q_mode", C.c_int)]
+                ("value_index", C.c_int), ("q_mode", C.c_int), ("fused", C.c_int)]
 
 
 class KernelStat(C.Structure):

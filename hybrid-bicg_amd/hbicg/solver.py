@@ -19,7 +19,7 @@ VARIANTS = ("normal", "static", "dynamic")
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
                 device=0, exit_on_error=False, upload_x=True, profile=False,
-                growth_limit=0.0, trace=False, value_index=True, q_mode=0) -> _lib.Params:
+                growth_limit=0.0, trace=False, value_index=True, q_mode=0, fused=True) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -35,6 +35,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.trace = int(bool(trace))
     p.value_index = int(bool(value_index))
     p.q_mode = int(q_mode)
+    p.fused = int(bool(fused))
     return p
 
 

@@ -38,6 +38,9 @@ extern "C" {
 #define HBICG_S_BREAKDOWN_SHIFT  8  /* a running shift's 1 - omega~ (sigma_s - sigma_j) or
                                        pi_j[k] was 0 or not finite (a NaN shift, or a shift
                                        that makes the recurrence divide by zero) */
+#define HBICG_S_SYNC_TIMEOUT    16  /* a fused launch's workgroup gave up waiting for a value
+                                       its first workgroups publish; the results are not
+                                       valid (never expected: those are dispatched first) */
 
 typedef struct hbicg_params {
     double tol;          /* stopping rule, default EPS (1e-12)                           */
@@ -73,6 +76,13 @@ typedef struct hbicg_params {
                             within 16-bit column offsets of their tile (stencils), else
                             formed for every row first and gathered; 1: always at the
                             gathered columns; 2: always beforehand.  Same results.        */
+    int    fused;        /* default mode: 1 (default) six launches per iteration -- the
+                            omega~ finalize runs as the first workgroup of the seed
+                            update's launch, which hands omega~ to its other
+                            workgroups, and the combined window coefficients run in
+                            the shift kernel's first workgroups; 0: eight launches.
+                            Same results bit for bit; the batched solver always uses
+                            eight, exact order seven (no coefficient kernel)            */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -161,8 +171,10 @@ int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
  * applied a window flush.  launches counts the launches (executed
  * iterations), total_ms the summed HIP-event durations, bytes the
  * algorithmic HBM bytes those launches had to move, flops ([8] only) twice
- * the flush's multiply-adds.  Returns the number of entries filled (at most
- * 9). */
+ * the flush's multiply-adds.  With the six fused launches (params.fused, the
+ * default order) [3] and [6] launch nothing: the omega~ finalize is timed
+ * in [4], the coefficients in [7] and [8].  Returns the number of entries
+ * filled (at most 9). */
 typedef struct hbicg_kernel_stat {
     int    launches;
     double total_ms;

@@ -12,6 +12,7 @@
  *   src/openmp_matrix.c:38-84     row-sequential CSR SpMV
  *   src/openmp_vector.c:3-84      parallel BLAS-1, serial-master dot (ddot_v3)
  *   src/vector.c:3-27             serial BLAS-1
+ *   src/matrix.c:268-419          row blocks of a P-rank run (orc_shifted_lopbicg_ranks)
  * Parallelism mirrors the reference's single-rank OpenMP structure (rows in
  * parallel for SpMV and BLAS-1, dot products summed serially by one thread,
  * shifts distributed with schedule(dynamic,1)); none of it changes a rounding,
@@ -133,6 +134,120 @@ static void par_mult_dynamic(const orc_csr *A, const double *x, double *y)
     for (int i = 0; i < n; ++i) y[i] += row_dot(A, (unsigned)i, x);
 }
 
+/* ---- the reference on P MPI ranks -------------------------------------------
+ * With P ranks (src/matrix.c:296-306) rank p owns the contiguous rows
+ * [start(p), start(p+1)), the first m mod P ranks one row more; a row's
+ * entries whose columns fall in its own rank's range form the diagonal block,
+ * the others the off-diagonal block, both in the row's CSR order
+ * (src/matrix.c:378-392).  The SpMV sums each row's diagonal-block entries,
+ * then adds the sum of its off-diagonal-block entries
+ * (src/solver_dynamic.c:158-160, src/openmp_matrix.c:39-60); a dot is each
+ * rank's serial sum of its own rows (src/openmp_vector.c:60-76) combined by
+ * MPI_Allreduce, here recursive doubling, the pairwise tree of MPICH's
+ * allreduce for one double on a power-of-two communicator.  P = 1 is the
+ * one-rank run (its empty off-diagonal block adds 0.0 to a row sum that is
+ * never -0.0, so nothing changes). */
+typedef struct {
+    int P;
+    unsigned start[ORC_MAX_RANKS + 1];
+    double *tmp;   /* [n] products, as dot_temp_vec */
+} orc_ranks;
+
+static unsigned rank_start(unsigned m, int P, int p)
+{
+    const unsigned per = m / (unsigned)P, extra = m % (unsigned)P;
+    return (unsigned)p * per + ((unsigned)p < extra ? (unsigned)p : extra);
+}
+
+static void ranks_init(orc_ranks *R, unsigned n, int P, double *tmp)
+{
+    R->P = P;
+    for (int p = 0; p <= P; ++p) R->start[p] = p == P ? n : rank_start(n, P, p);
+    R->tmp = tmp;
+}
+
+static int rank_of(const orc_ranks *R, unsigned i)
+{
+    int p = 0;
+    while (i >= R->start[p + 1]) ++p;
+    return p;
+}
+
+/* y[i] = 0 + (diagonal-block sum) + (off-diagonal-block sum) */
+static void par_mult_ranks(const orc_csr *A, const orc_ranks *R, const double *x, double *y)
+{
+    const int n = (int)A->n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const int p = rank_of(R, (unsigned)i);
+        const unsigned lo = R->start[p], hi = R->start[p + 1];
+        double td = 0.0, to = 0.0;
+        for (unsigned e = A->ptr[i]; e < A->ptr[i + 1]; ++e) {
+            const unsigned c = A->col[e];
+            if (c >= lo && c < hi) td += A->val[e] * x[c];
+            else to += A->val[e] * x[c];
+        }
+        double t = y[i];
+        t += td;
+        t += to;
+        y[i] = t;
+    }
+}
+
+/* MPI_Allreduce(MPI_SUM) of one double per rank by recursive doubling */
+static double allreduce(double *v, int P)
+{
+    for (int d = 1; d < P; d *= 2)
+        for (int p = 0; p < P; p += 2 * d) v[p] = v[p] + v[p + d];
+    return v[0];
+}
+
+/* each rank's my_openmp_ddot_v3 (products, then a serial sum of its rows),
+ * then MPI_Allreduce */
+static double dot_ranks(const orc_ranks *R, int n, const double *x, const double *y)
+{
+    if (R->P == 1) return orc_ddot_v3(n, x, y, R->tmp);
+    double part[ORC_MAX_RANKS];
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) R->tmp[i] = x[i] * y[i];
+    for (int p = 0; p < R->P; ++p) {
+        double acc = 0.0;
+        for (unsigned i = R->start[p]; i < R->start[p + 1]; ++i) acc += R->tmp[i];
+        part[p] = acc;
+    }
+    return allreduce(part, R->P);
+}
+
+/* each rank's serial my_ddot, then MPI_Allreduce (src/solver_dynamic.c:
+ * 100-103 and 323-324) */
+static double ddot_ranks(const orc_ranks *R, const double *x, const double *y)
+{
+    double part[ORC_MAX_RANKS];
+    for (int p = 0; p < R->P; ++p) {
+        double acc = 0.0;
+        for (unsigned i = R->start[p]; i < R->start[p + 1]; ++i) acc += x[i] * y[i];
+        part[p] = acc;
+    }
+    return allreduce(part, R->P);
+}
+
+void orc_mult_ranks(const orc_csr *A, int ranks, const double *x, double *y)
+{
+    orc_ranks R;
+    ranks_init(&R, A->n, ranks, NULL);
+    par_mult_ranks(A, &R, x, y);
+}
+
+double orc_dot_ranks(int n, int ranks, const double *x, const double *y)
+{
+    double *tmp = malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    orc_ranks R;
+    ranks_init(&R, (unsigned)n, ranks, tmp);
+    const double v = dot_ranks(&R, n, x, y);
+    free(tmp);
+    return v;
+}
+
 /* ---- solver -------------------------------------------------------------- */
 typedef struct {
     int n, M, cap;              /* cap = max_iter + 1 archive entries */
@@ -204,12 +319,22 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
                         const double *sigma, int M, int seed,
                         double tol, int max_iter, orc_trace *tr)
 {
-    if (!A || !x_set || !r || !sigma || M <= 0 || seed < 0 || seed >= M || max_iter < 1)
+    return orc_shifted_lopbicg_ranks(A, x_set, r, sigma, M, seed, tol, max_iter, 1, tr);
+}
+
+int orc_shifted_lopbicg_ranks(const orc_csr *A, double *x_set, double *r,
+                              const double *sigma, int M, int seed,
+                              double tol, int max_iter, int ranks, orc_trace *tr)
+{
+    if (!A || !x_set || !r || !sigma || M <= 0 || seed < 0 || seed >= M || max_iter < 1 ||
+        ranks < 1 || ranks > ORC_MAX_RANKS || (ranks & (ranks - 1)) || (unsigned)ranks > A->n)
         return -1;
     const int n = (int)A->n;
     const int cap = max_iter + 1;
     orc_ws w;
     if (ws_alloc(&w, n, M, cap) != 0) { ws_free(&w); return -1; }
+    orc_ranks R;
+    ranks_init(&R, A->n, ranks, w.tmp);
     if (tr) {
         tr->n_switch = 0;
         tr->status = 0;
@@ -221,7 +346,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
     }
 
     /* initialisation (src/solver_dynamic.c:101-123) */
-    double rTr = orc_ddot(n, r, r);
+    double rTr = ranks == 1 ? orc_ddot(n, r, r) : ddot_ranks(&R, r, r);
     orc_dcopy(n, r, w.r_hat);
     for (int j = 0; j < M; ++j) {
         orc_dcopy(n, r, w.p_set + (size_t)j * n);
@@ -242,26 +367,28 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
         /* seed step (src/solver_dynamic.c:156-210) */
         par_copy(n, r, w.r_old);
         par_zero(n, w.s);
-        par_mult(A, p_s, w.s);
+        if (ranks == 1) par_mult(A, p_s, w.s);
+        else par_mult_ranks(A, &R, p_s, w.s);
         par_axpy(n, sigma[seed], p_s, w.s);
-        const double rTs = orc_ddot_v3(n, w.r_hat, w.s, w.tmp);
+        const double rTs = dot_ranks(&R, n, w.r_hat, w.s);
         w.aa[k] = rTr / rTs;
 
         par_axpy(n, -w.aa[k], w.s, r);          /* r now holds q */
         par_copy(n, r, w.q);
         par_zero(n, w.y);
-        par_mult_dynamic(A, r, w.y);
+        if (ranks == 1) par_mult_dynamic(A, r, w.y);
+        else par_mult_ranks(A, &R, r, w.y);
         par_axpy(n, sigma[seed], r, w.y);
-        const double qTq = orc_ddot_v3(n, r, r, w.tmp);
-        const double qTy = orc_ddot_v3(n, r, w.y, w.tmp);
+        const double qTq = dot_ranks(&R, n, r, r);
+        const double qTy = dot_ranks(&R, n, r, w.y);
         w.wa[k] = qTq / qTy;
         const double rTr_old = rTr;
 
         par_axpy(n, w.aa[k], p_s, x_s);
         par_axpy(n, w.wa[k], r, x_s);
         par_axpy(n, -w.wa[k], w.y, r);          /* r now holds the new residual */
-        dot_r = orc_ddot_v3(n, r, r, w.tmp);
-        rTr = orc_ddot_v3(n, w.r_hat, r, w.tmp);
+        dot_r = dot_ranks(&R, n, r, r);
+        rTr = dot_ranks(&R, n, w.r_hat, r);
         w.ba[k] = (w.aa[k] / w.wa[k]) * (rTr / rTr_old);
 
         par_scal(n, w.ba[k], p_s);
@@ -324,7 +451,7 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
                 w.wa[i] = w.wa[i] / (1.0 - w.wa[i] * ds);
             }
             orc_dscal(n, 1.0 / (w.zeta[m] * PI(&w, m, k)), r);
-            rTr = orc_ddot(n, w.r_hat, r);
+            rTr = ranks == 1 ? orc_ddot(n, w.r_hat, r) : ddot_ranks(&R, w.r_hat, r);
             for (int j = 0; j < M; ++j) { w.eta[j] = 0.0; w.zeta[j] = 1.0; }
             for (int i = 1; i <= k; ++i) {
                 for (int j = 0; j < M; ++j) {

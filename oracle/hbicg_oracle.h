@@ -66,6 +66,24 @@ int orc_shifted_lopbicg(const orc_csr *A, double *x_set, double *r,
                         const double *sigma, int M, int seed,
                         double tol, int max_iter, orc_trace *tr);
 
+/* The same solve as the reference computes it on `ranks` MPI ranks (a power
+ * of two up to ORC_MAX_RANKS, at most n): contiguous row blocks as
+ * src/matrix.c:296-306 assigns them, each row's SpMV sum split into its
+ * diagonal-block and off-diagonal-block parts (src/openmp_matrix.c:39-60,
+ * src/solver_dynamic.c:158-160, 179-181), dots as per-rank serial sums
+ * combined by MPI_Allreduce (recursive doubling).  Only these roundings
+ * change with the rank count; ranks = 1 is orc_shifted_lopbicg. */
+#define ORC_MAX_RANKS 64
+int orc_shifted_lopbicg_ranks(const orc_csr *A, double *x_set, double *r,
+                              const double *sigma, int M, int seed,
+                              double tol, int max_iter, int ranks, orc_trace *tr);
+
+/* The P-rank SpMV (y[i] += diagonal-block sum, then += off-diagonal-block
+ * sum) and dot (per-rank serial sums, recursive doubling) of
+ * orc_shifted_lopbicg_ranks, exported for the tests. */
+void   orc_mult_ranks(const orc_csr *A, int ranks, const double *x, double *y);
+double orc_dot_ranks(int n, int ranks, const double *x, const double *y);
+
 /* y[i] += sum_j val*x[col], rows summed sequentially from 0.0 (reference
  * src/matrix.c:498-516 mult). */
 void orc_mult(const orc_csr *A, const double *x, double *y);

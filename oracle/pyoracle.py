@@ -52,7 +52,13 @@ def lib():
         L.orc_shifted_lopbicg.restype = C.c_int
         L.orc_shifted_lopbicg.argtypes = [C.POINTER(_Csr), _dp, _dp, _dp, C.c_int, C.c_int,
                                           C.c_double, C.c_int, C.POINTER(_Trace)]
+        L.orc_shifted_lopbicg_ranks.restype = C.c_int
+        L.orc_shifted_lopbicg_ranks.argtypes = [C.POINTER(_Csr), _dp, _dp, _dp, C.c_int, C.c_int,
+                                                C.c_double, C.c_int, C.c_int, C.POINTER(_Trace)]
         L.orc_mult.argtypes = [C.POINTER(_Csr), _dp, _dp]
+        L.orc_mult_ranks.argtypes = [C.POINTER(_Csr), C.c_int, _dp, _dp]
+        L.orc_dot_ranks.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_dot_ranks.restype = C.c_double
         L.orc_true_residuals.argtypes = [C.POINTER(_Csr), _dp, _dp, _dp, C.c_int, _dp]
         L.orc_daxpy.argtypes = [C.c_int, C.c_double, _dp, _dp]
         L.orc_ddot.argtypes = [C.c_int, _dp, _dp]
@@ -103,8 +109,10 @@ class OracleResult:
     status: int = 0          # NO_SWITCH_TARGET (4) when the solve ended for that reason
 
 
-def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
-    """Run the restated reference solver. x0 defaults to zeros."""
+def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None, ranks=1) -> OracleResult:
+    """Run the restated reference solver. x0 defaults to zeros.  ranks > 1
+    (a power of two): the arithmetic of the reference's run on that many MPI
+    ranks (orc_shifted_lopbicg_ranks)."""
     L = lib()
     n = A.n
     sigma = np.ascontiguousarray(sigma, dtype=np.float64)
@@ -124,8 +132,8 @@ def solve(A, b, sigma, seed, tol=1e-12, max_iter=2000, x0=None) -> OracleResult:
     tr = _Trace(_p(stop_iter, _ip), _p(sw_k, _ip), _p(sw_to, _ip), 0, _p(alpha), _p(beta),
                 _p(omega), _p(res_hist), _p(seed_hist, _ip), _p(zeta), _p(pi_k), _p(dots),
                 0.0, 0.0, 0.0, 0)
-    k = L.orc_shifted_lopbicg(C.byref(h.s), _p(x), _p(r), _p(sigma), M, int(seed), float(tol),
-                              int(max_iter), C.byref(tr))
+    k = L.orc_shifted_lopbicg_ranks(C.byref(h.s), _p(x), _p(r), _p(sigma), M, int(seed), float(tol),
+                                    int(max_iter), int(ranks), C.byref(tr))
     if k < 0:
         raise ValueError("oracle rejected its arguments")
     sw = [(int(sw_k[i]), int(sw_to[i])) for i in range(tr.n_switch)]
@@ -141,6 +149,22 @@ def mult(A, x) -> np.ndarray:
     y = np.zeros(A.n)
     lib().orc_mult(C.byref(h.s), _p(x), _p(y))
     return y
+
+
+def mult_ranks(A, x, ranks) -> np.ndarray:
+    """y = A x as the reference computes it on `ranks` MPI ranks."""
+    h = _CsrHold(A)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(A.n)
+    lib().orc_mult_ranks(C.byref(h.s), int(ranks), _p(x), _p(y))
+    return y
+
+
+def dot_ranks(x, y, ranks) -> float:
+    """(x, y) as the reference computes it on `ranks` MPI ranks."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().orc_dot_ranks(x.shape[0], int(ranks), _p(x), _p(y))
 
 
 def true_residuals(A, x, b, sigma) -> np.ndarray:

@@ -17,8 +17,13 @@ Stored per first seed s:
   rs_s, rbits_s, rnorm_s  the returned r (the last seed's residual) at the
                same rows, its bit-pattern sum and its norm
   tres_s       [64] the oracle's true residual ||b - (A + sigma_j I) x_j|| / ||b||
+  k_s_rP, stop_s_rP, switch_s_rP, xs_s_rP
+               the same solve as the reference computes it on P = 2, 4, ..., 64 MPI
+               ranks (the oracle's orc_shifted_lopbicg_ranks: split row sums,
+               per-rank dots): the reference's own spread across its
+               decompositions, which the default-order tests are held to
 
-The oracle takes about two minutes per solve on 8 threads, too long for the
+The oracle takes one to two minutes per solve on 8 threads, too long for the
 CPU test suite, so the GPU tests compare against these numbers instead of
 running it.  Run from the repository root:
     python tests/golden/make_c1_fixture.py
@@ -36,6 +41,7 @@ import pyoracle as O  # noqa: E402
 from hbicg import problems as P  # noqa: E402
 
 SEEDS = (0, 63)
+RANKS = (2, 4, 8, 16, 32, 64)
 STRIDE = 16411   # prime: 128 rows per shift, spread over the whole grid
 
 
@@ -62,6 +68,15 @@ def main():
         out[f"rnorm_{s}"] = np.array(np.linalg.norm(r.r))
         out[f"tres_{s}"] = O.true_residuals(p.A, r.x, p.b, p.sigma)
         del r
+        for P_ in RANKS:
+            t = time.time()
+            r = O.solve(p.A, p.b, p.sigma, s, tol=p.tol, max_iter=p.max_iter, ranks=P_)
+            print(f"seed {s}, {P_} ranks: k={r.k}, switches {r.switches}, {time.time() - t:.0f} s", flush=True)
+            out[f"k_{s}_r{P_}"] = np.array(r.k)
+            out[f"stop_{s}_r{P_}"] = r.stop_iter.astype(np.int32)
+            out[f"switch_{s}_r{P_}"] = np.array(r.switches, dtype=np.int32).reshape(-1, 2)
+            out[f"xs_{s}_r{P_}"] = r.x[:, ::STRIDE].copy()
+            del r
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "c1_oracle.npz"), **out)
 
 

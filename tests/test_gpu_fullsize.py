@@ -10,10 +10,21 @@ seed 63 (the largest shift converges first; the seed switches at iteration
     bit-pattern checksum of every x_j and of the returned r;
   * the default order -- a device plan, and shifted_lopbicg_static and
     shifted_lopbicg_dynamic on host buffers -- stays within the parity bar:
-    k and every stop and switch iteration within 2, the same switch targets,
-    x_j within 1e-9 relative on the sampled rows and in norm, and every true
-    residual within 10x the stopping rule; static and dynamic agree bit for
-    bit with each other.
+    k, stop and switch iterations against the reference's own results (see
+    below), the same switch targets, x_j within 1e-9 relative of the
+    one-rank run on the sampled rows and in norm, and every true residual
+    within 10x the stopping rule; static and dynamic agree bit for bit with
+    each other.
+"The reference's own results" are its runs on 1, 2, 4, ..., 64 MPI ranks
+(stored in the fixture): on this operator BiCG amplifies a rounding
+difference about 1e5-fold every ten iterations, so the split row sums and
+per-rank dot sums of another decomposition already move the reference's k
+by several iterations and single shifts' stop iterations by 10 to 20,
+while x stays within a few 1e-10.  The tree-summed dots of the default
+order are one more such rounding: k must lie within 2 of the range the
+reference's decompositions span, and no shift's stop iteration may move
+further from the one-rank run than the reference's own decompositions
+move one, plus 2.
 The GPU true-residual checker is held against the oracle's on the same x.
 
 The irregular power-law matrix (n = 1e6, 31M nonzeros, 128 shifts): the
@@ -66,19 +77,35 @@ def _rel_rows(a, b):
     return np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
 
 
+RANKS = (2, 4, 8, 16, 32, 64)
+
+
 def _within_bar(k, stop, sw, x, gold, s, what):
+    """The parity bar of the default order against the reference's runs on
+    1, 2, 4, ..., 64 ranks: k within 2 of the range they span; the largest
+    change of a shift's stop iteration from the one-rank run no larger than
+    the largest the reference's own decompositions show, plus 2; the same
+    switch targets, at iterations within 2 of the range; x within 1e-9 of
+    the one-rank run."""
     S = int(gold["stride"])
-    assert abs(k - int(gold[f"k_{s}"])) <= 2, (what, k, int(gold[f"k_{s}"]))
-    assert np.all(np.abs(stop - gold[f"stop_{s}"]) <= 2), (what, stop - gold[f"stop_{s}"])
-    gsw = [tuple(int(v) for v in row) for row in gold[f"switch_{s}"]]
-    assert len(sw) == len(gsw), (what, sw, gsw)
-    for (kg, sg), (kr, sr) in zip(sw, gsw):
-        assert abs(kg - kr) <= 2 and sg == sr, (what, sw, gsw)
+    stop1 = gold[f"stop_{s}"].astype(np.int64)
+    ks = [int(gold[f"k_{s}"])] + [int(gold[f"k_{s}_r{P_}"]) for P_ in RANKS]
+    sws = [gold[f"switch_{s}"]] + [gold[f"switch_{s}_r{P_}"] for P_ in RANKS]
+    ref_dev = max(int(np.abs(gold[f"stop_{s}_r{P_}"] - stop1).max()) for P_ in RANKS)
+    dev = int(np.abs(stop.astype(np.int64) - stop1).max())
+    assert all(np.array_equal(w[:, 1], sws[0][:, 1]) for w in sws)   # the reference agrees on targets
+    assert min(ks) - 2 <= k <= max(ks) + 2, (what, k, ks)
+    assert dev <= ref_dev + 2, (what, dev, ref_dev, stop - stop1)
+    assert [t for _, t in sw] == [int(t) for t in sws[0][:, 1]], (what, sw)
+    for i, (kg, _) in enumerate(sw):
+        kr = [int(w[i, 0]) for w in sws]
+        assert min(kr) - 2 <= kg <= max(kr) + 2, (what, sw, kr)
     rel = _rel_rows(x[:, ::S], gold[f"xs_{s}"])
     assert rel.max() <= 1e-9, (what, rel.max())
     nrel = np.abs(np.linalg.norm(x, axis=1) - gold[f"xnorm_{s}"]) / gold[f"xnorm_{s}"]
     assert nrel.max() <= 1e-9, (what, nrel.max())
-    print(f"  {what}: k={k} (oracle {int(gold[f'k_{s}'])}), max rel dx on samples {rel.max():.2e}, "
+    print(f"  {what}: k={k} (reference on 1..64 ranks: {ks}), largest stop change {dev} "
+          f"(the reference's own: {ref_dev}), max rel dx on samples {rel.max():.2e}, "
           f"in norm {nrel.max():.2e}", flush=True)
 
 
@@ -105,9 +132,11 @@ def test_c1_exact_order_equals_oracle(c1, gold, seed):
     assert np.array_equal(_bit_sums(xh), gold[f"xbits_{seed}"])
     assert np.array_equal(rh[::S], gold[f"rs_{seed}"])
     assert np.array_equal(_bit_sums(rh[None, :]), gold[f"rbits_{seed}"])
-    # the same x: the GPU checker against the oracle's (they differ only in
-    # the order the squared residual entries are summed)
-    np.testing.assert_allclose(tr, gold[f"tres_{seed}"], rtol=1e-12, atol=0)
+    # the same x: the GPU checker against the oracle's.  Each residual entry
+    # is the same double in both; they differ in how the 2M squared entries
+    # are summed: the oracle's serial sum of n positive terms carries a
+    # relative error of up to about n u = 2.3e-10, the GPU's tree far less
+    np.testing.assert_allclose(tr, gold[f"tres_{seed}"], rtol=2.5e-10, atol=0)
     assert tr.max() <= 10 * c1.tol
 
 

@@ -437,3 +437,41 @@ def test_ragged_sizes(M, seed):
         assert rel.max() <= 1e-9, (variant, rel.max())
         tr = O.true_residuals(A, got.x, b, sigma)
         assert tr.max() <= 10 * 1e-10, (variant, tr.max())
+
+
+@pytest.mark.parametrize("value_index", [True, False])
+@pytest.mark.parametrize("seed", [0, 8])
+def test_fused_launches_bitwise(c0, seed, value_index):
+    """Default mode: the six-launch iteration (omega~ finalized by the first
+    workgroup of the seed update's launch and handed to its other
+    workgroups, combined coefficients formed in the shift kernel's first
+    workgroups) is the eight-launch iteration's arithmetic: k, stop
+    iterations, switches, x and r bit for bit, and no workgroup timed out
+    waiting."""
+    for variant in ("static", "dynamic"):
+        print(f"case fused vs unfused c0 seed={seed} {variant} value_index={value_index}", flush=True)
+        mk = lambda f: make_params(tol=c0.tol, fused=f, value_index=value_index)
+        a = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(True))
+        b = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(False))
+        assert a.k == b.k and a.switches == b.switches
+        np.testing.assert_array_equal(a.stop_iter, b.stop_iter)
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.r, b.r)
+        assert not (a.report["status"] & 16) and not (b.report["status"] & 16)
+
+
+def test_fused_launches_bitwise_gathered_q():
+    """A scattered-column matrix whose SpMV 2 gathers q formed beforehand
+    (an extra launch before SpMV 2): bit for bit the unfused result, window
+    flushes included."""
+    A = P.zipf_matrix(20000, 12.0, 300, seed=7)
+    b = P.uniform(7, 2, A.n, -1.0, 1.0)
+    sigma = P.uniform(7, 3, 24, 0.05, 0.5)
+    seed = int(np.argmin(sigma))
+    print("case fused vs unfused, gathered q", flush=True)
+    mk = lambda f: make_params(tol=1e-10, fused=f, window=16, q_mode=2)
+    a = dropin_solve(A, b, sigma, seed, "dynamic", mk(True))
+    c = dropin_solve(A, b, sigma, seed, "dynamic", mk(False))
+    assert a.report["q_unfused"] == 1
+    assert a.k == c.k and a.switches == c.switches and a.report["stop_count"] == len(sigma)
+    assert np.array_equal(a.x, c.x) and np.array_equal(a.r, c.r)
+    assert not (a.report["status"] & 16)

@@ -359,3 +359,71 @@ def test_nan_shift_ends_without_switch_target(c0):
     assert r.stop_iter[0] == clean.stop_iter[0] and r.stop_iter[2] == clean.stop_iter[1]
     assert np.array_equal(r.x[[0, 2]], clean.x[:, :])
     assert np.all(np.isnan(r.x[1]))
+
+
+# ---- the reference on P MPI ranks ---------------------------------------------
+def test_ranks_split_row_sums():
+    """src/solver_dynamic.c:158-160: a row's diagonal-block entries are summed,
+    then its off-diagonal-block sum is added.  Row 0 holds 1e16, 1, -1e16 at
+    columns 0, 2, 1: summed in CSR order the 1 is lost (1e16 + 1 rounds to
+    1e16), while on 2 ranks (rows and columns 0-1 | 2-3) the diagonal block
+    cancels exactly and the off-diagonal 1 survives.  Integer data sums
+    exactly in any order, so every rank count gives the dense product."""
+    ptr = np.array([0, 3, 4, 5, 6], dtype=np.uint32)
+    A = P.CSR(ptr, np.array([0, 2, 1, 1, 2, 3], dtype=np.uint32),
+              np.array([1e16, 1.0, -1e16, 2.0, 3.0, 4.0]), 4)
+    x = np.ones(4)
+    assert O.mult_ranks(A, x, 1)[0] == 0.0 and O.mult(A, x)[0] == 0.0
+    assert O.mult_ranks(A, x, 2)[0] == 1.0
+    rng = np.random.default_rng(51)
+    B = P.conv_diff_2d(9, 7, 4, 1)            # integer entries 4, -2, 0 ... exact products
+    xi = rng.integers(-50, 50, B.n).astype(np.float64)
+    dense = B.to_scipy() @ xi
+    for R in (1, 2, 4, 8, 16):
+        assert np.array_equal(O.mult_ranks(B, xi, R), dense)
+
+
+def test_ranks_dot_partition_and_allreduce():
+    """Rows m = 10 on 4 ranks: 3, 3, 2, 2 (src/matrix.c:296-299: the first
+    m mod P ranks take one more row); each rank sums its products serially,
+    MPI_Allreduce adds the four partials by recursive doubling,
+    (p0 + p1) + (p2 + p3).  Random data makes any other partition or
+    combination order a different double."""
+    rng = np.random.default_rng(52)
+    x = rng.standard_normal(10) * 10.0 ** rng.integers(-8, 8, 10)
+    y = rng.standard_normal(10)
+    t = x * y
+    part = [0.0, 0.0, 0.0, 0.0]
+    for p, (a, b) in enumerate(((0, 3), (3, 6), (6, 8), (8, 10))):
+        acc = 0.0
+        for i in range(a, b):
+            acc += t[i]
+        part[p] = acc
+    want = (part[0] + part[1]) + (part[2] + part[3])
+    assert O.dot_ranks(x, y, 4) == want
+    serial = 0.0
+    for v in t:
+        serial += v
+    assert O.dot_ranks(x, y, 1) == serial
+    # crafted: one element per rank, 1e16, 1, -1e16, 1 -- serially 1, by
+    # recursive doubling (1e16 + 1) + (-1e16 + 1) = 0
+    a = np.array([1e16, 1.0, -1e16, 1.0])
+    assert O.dot_ranks(a, np.ones(4), 1) == 1.0
+    assert O.dot_ranks(a, np.ones(4), 4) == (a[0] + a[1]) + (a[2] + a[3])
+
+
+def test_ranks_solves_agree(c0):
+    """The 64x64 workload on 1..16 ranks: the same k, stop iterations and
+    switches, solutions within 1e-10 of the one-rank run and converged; one
+    rank is orc_shifted_lopbicg itself."""
+    for seed in (0, 8):
+        base = O.solve(c0.A, c0.b, c0.sigma, seed, tol=c0.tol)
+        one = O.solve(c0.A, c0.b, c0.sigma, seed, tol=c0.tol, ranks=1)
+        assert one.k == base.k and np.array_equal(one.x, base.x)
+        for R in (2, 4, 16):
+            r = O.solve(c0.A, c0.b, c0.sigma, seed, tol=c0.tol, ranks=R)
+            assert r.k == base.k and np.array_equal(r.stop_iter, base.stop_iter)
+            assert r.switches == base.switches
+            rel = np.linalg.norm(r.x - base.x, axis=1) / np.linalg.norm(base.x, axis=1)
+            assert 0 < rel.max() <= 1e-10, rel.max()
+            assert O.true_residuals(c0.A, r.x, c0.b, c0.sigma).max() < 5 * c0.tol
