@@ -1525,3 +1525,9 @@ fail:
     hipFree(work);
     return rc;
 }
+
+
+/* ---- plan internals for the library's other translation units (hb_plan.h) ---- */
+const hb_dev *hb_plan_dev(const hbicg_plan *pl) { return &pl->dev; }
+hipStream_t hb_plan_stream(const hbicg_plan *pl) { return pl->stream; }
+int hb_plan_device(const hbicg_plan *pl) { return pl->device; }

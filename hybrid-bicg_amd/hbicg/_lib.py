@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "_build", "libhbicg.so")
 HBICG_E = {
     -1: "bad argument", -2: "matrix is not square", -3: "non-empty off-process block",
     -4: "HIP runtime error", -5: "allocation failed", -6: "no HIP device",
+    -7: "input larger than the no-allocation scratch",
 }
 
 _dp = C.POINTER(C.c_double)
@@ -84,7 +85,8 @@ EXPORTS = {
                  "mult"],
     "hbicg.h": ["hbicg_default_params", "hbicg_set_params", "hbicg_get_params",
                 "hbicg_last_report", "hbicg_last_residual_trace", "hbicg_plan_create", "hbicg_plan_destroy", "hbicg_errno",
-                "hbicg_plan_solve", "hbicg_plan_true_residuals", "hbicg_plan_spmv",
+                "hbicg_plan_solve", "hbicg_plan_true_residuals",
+                "hbicg_plan_true_residuals_noalloc", "hbicg_plan_spmv",
                 "hbicg_plan_report", "hbicg_plan_residual_trace", "hbicg_plan_kernel_stats",
                 "hbicg_batch_create",
                 "hbicg_batch_destroy", "hbicg_batch_solve", "hbicg_batch_report",
@@ -147,6 +149,9 @@ def load():
     L.hbicg_plan_solve.restype = C.c_int
     L.hbicg_plan_true_residuals.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p, C.c_int, _dp]
     L.hbicg_plan_true_residuals.restype = C.c_int
+    L.hbicg_plan_true_residuals_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p]
+    L.hbicg_plan_true_residuals_noalloc.restype = C.c_int
     L.hbicg_plan_spmv.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
     L.hbicg_plan_spmv.restype = C.c_int
     L.hbicg_plan_report.argtypes = [C.c_void_p, C.POINTER(Report), _ip, _ip, _ip, C.c_int]

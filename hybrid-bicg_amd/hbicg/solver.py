@@ -225,6 +225,23 @@ class Plan:
         _lib.check(rc, "hbicg_plan_true_residuals")
         return out
 
+    def true_residuals_noalloc(self, b, sigma, x, out):
+        """hbicg_plan_true_residuals_noalloc: sigma [M] and out [M] are float64
+        device tensors; nothing is allocated by the library.  Returns out
+        once the plan's stream has written it."""
+        torch = self._torch
+        for t in (b, sigma, x, out):
+            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+        assert sigma.numel() == self.M and out.numel() == self.M
+        self._enter()
+        rc = self.L.hbicg_plan_true_residuals_noalloc(self.h, C.c_void_p(b.data_ptr()),
+                                                      C.c_void_p(sigma.data_ptr()),
+                                                      C.c_void_p(x.data_ptr()),
+                                                      C.c_void_p(out.data_ptr()))
+        self._leave()
+        _lib.check(rc, "hbicg_plan_true_residuals_noalloc")
+        return out
+
     def spmv(self, x, shift=0.0):
         y = self._torch.empty_like(x)
         self._enter()

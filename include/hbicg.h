@@ -22,6 +22,7 @@ extern "C" {
 #define HBICG_E_HIP        (-4)  /* HIP runtime error                                   */
 #define HBICG_E_NOMEM      (-5)  /* host or device allocation failed                    */
 #define HBICG_E_NODEVICE   (-6)  /* no HIP device                                       */
+#define HBICG_E_SCRATCH    (-7)  /* input larger than a no-allocation entry point's scratch */
 
 /* status bits of hbicg_report.status (device-detected conditions).  The
  * reference does not trap breakdowns (src/solver_dynamic.c:167,189,245-253):
@@ -159,6 +160,19 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
  * GPU (pointers as in hbicg_plan_solve; out is a host array of M). */
 int hbicg_plan_true_residuals(hbicg_plan *pl, const double *b, const double *sigma,
                               const double *x_set, int on_device, double *out);
+
+/* hbicg_plan_true_residuals without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops).  b, sigma (M shifts), x_set
+ * ([M][n]) and out (M doubles) are device pointers; the kernels' partial sums
+ * live in a static 1 MiB scratch array of each GPU, so at most 65536 shifts
+ * (HBICG_E_SCRATCH beyond).  The work is enqueued on the plan's stream and
+ * the call returns without waiting: out holds the residuals once that stream
+ * gets there.  Results equal hbicg_plan_true_residuals' bit for bit.  Calls
+ * that overlap on one GPU (other streams, other threads) share the scratch
+ * and race. */
+int hbicg_plan_true_residuals_noalloc(hbicg_plan *pl, const double *b, const double *sigma,
+                                      const double *x_set, double *out);
 
 /* y = (A + shift I) x with the plan's SpMV (device pointers). */
 int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
