@@ -128,6 +128,8 @@ def parse():
                          "1..--sweep-max x {normal, dynamic} (src/main_seed_diff.c); repeat: one "
                          "solve of --repeat-shifts shifts timed --repeats times (src/main_repeat.c); "
                          "powerlaw: the power-law leg alone (for counter runs)")
+    ap.add_argument("--panel-cols", type=int, default=0,
+                    help="power-law leg: SpMV column-panel width (0 auto, -1 off)")
     ap.add_argument("--pl-iters", type=int, default=0,
                     help="powerlaw mode: iteration cap of its solves (0: the workload's 2000)")
     ap.add_argument("--sweep-grid", type=int, default=64, help="3-D grid side of the sweep's operator")
@@ -290,7 +292,8 @@ def powerlaw_leg(args, log):
     dev = torch.device("cuda:0")
     b = torch.as_tensor(prob.b).to(dev)
     x = torch.zeros((prob.M, prob.A.n), dtype=torch.float64, device=dev)
-    pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=prob.max_iter, graph=1), device=dev)
+    pl = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=prob.max_iter, graph=1,
+                                          panel_cols=args.panel_cols), device=dev)
     pl.solve(b, prob.sigma, prob.seed, x)
     torch.cuda.synchronize()
     x.zero_()
@@ -299,8 +302,8 @@ def powerlaw_leg(args, log):
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t) * 1e3
     pl.close()
-    pp = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=prob.max_iter, graph=0, profile=True),
-              device=dev)
+    pp = Plan(prob.A, prob.M, make_params(tol=prob.tol, max_iter=prob.max_iter, graph=0, profile=True,
+                                          panel_cols=args.panel_cols), device=dev)
     xp = torch.zeros_like(x)
     pp.solve(b, prob.sigma, prob.seed, xp)
     ks = pp.kernel_stats()
@@ -318,7 +321,8 @@ def powerlaw_leg(args, log):
                         f"uniform in [0,0.5], relres {prob.tol:g}, cap {prob.max_iter}",
             "shifts": prob.M, "ms_per_solve": round(ms, 2), "iterations": k - 1,
             "ms_per_iteration": round(ms / (k - 1), 4),
-            "shifts_stopped": rep["stop_count"], "value_table": rep["value_table"], "spmv": spmv}
+            "shifts_stopped": rep["stop_count"], "value_table": rep["value_table"],
+            "column_panels": rep["panels"], "spmv": spmv}
 
 
 def _driver_problem(n_side, M):

@@ -46,6 +46,7 @@ void hbicg_default_params(hbicg_params *p)
     p->value_index = 1;
     p->q_mode = 0;
     p->fused = 1;
+    p->panel_cols = 0;
 }
 
 static const hbicg_params *params(void)
@@ -117,7 +118,10 @@ struct hbicg_plan {
     int *doff;           /* index bytes                                               */
     unsigned char *didx;
     int n_diag_tiles;
+    double *pval;        /* column panels: the entries panel by panel, each      */
+    unsigned *pcol, *pptr, *pblk; /* panel's row starts and tiles                */
     int own_matrix, nblk;
+    int maxparts;        /* partial-sum pairs per phase the part buffer holds */
     /* workspace */
     double *rhat, *rring, *qring, *s, *y, *p_set, *x_host_buf, *b_buf, *sigma;
     double *aa, *ba, *wa, *pi, *eta, *zeta, *coef, *ccoef, *grow, *part, *trace, *p_snap;
@@ -188,7 +192,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->flush_log, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->pval, pl->pcol, pl->pptr, pl->pblk, pl->flush_log, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -451,6 +455,109 @@ out:
 /* w_alloc: ring storage for windows up to this (the batched solver's
  * autotuner); w_cap: the largest window the plan may choose; allow_vi: the
  * SpMV may read the matrix through a value table (not in batches). */
+/* Column panels of `width` columns (params.panel_cols): the entries of
+ * every row split by column into panels, stored panel by panel (each
+ * panel's rows in order, a row's entries in CSR order) with one row-pointer
+ * array per panel.  Needs every row's columns strictly ascending, so that
+ * the panels take a row's entries in CSR order; returns 1 without building
+ * anything when one is not, or when there would be a single panel. */
+static int build_panels(hbicg_plan *pl, const hbicg_csr *A, int a_on_device, const unsigned *hptr, int width)
+{
+    const int n = pl->n;
+    const size_t nz = pl->nz;
+    if ((long long)width * HB_MAXPANEL < (long long)n) width = (int)(((long long)n + HB_MAXPANEL - 1) / HB_MAXPANEL);
+    const int P = (int)(((long long)n + width - 1) / width);
+    if (P < 2 || nz == 0) return 1;
+    int rc = 0, grow = 0;
+    unsigned *qb[HB_MAXPANEL] = {0}, *hbt = NULL;
+    const unsigned *hc = (const unsigned *)A->col;
+    const double *hv = (const double *)A->val;
+    unsigned *tc = NULL, *hpp = NULL, *pc = NULL;
+    double *tv = NULL, *pv = NULL;
+    size_t *cur = NULL;
+    if (a_on_device) {
+        tc = (unsigned *)malloc(nz * 4);
+        tv = (double *)malloc(nz * 8);
+        if (!tc || !tv) { rc = HBICG_E_NOMEM; goto done; }
+        if (hipMemcpy(tc, A->col, nz * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(tv, A->val, nz * 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = HBICG_E_HIP; goto done; }
+        hc = tc;
+        hv = tv;
+    }
+    for (int i = 0; i < n; ++i)
+        for (unsigned e = hptr[i] + 1; e < hptr[i + 1]; ++e)
+            if (hc[e] <= hc[e - 1]) { rc = 1; goto done; }
+    hpp = (unsigned *)malloc((size_t)P * ((size_t)n + 1) * 4);
+    pc = (unsigned *)malloc(nz * 4);
+    pv = (double *)malloc(nz * 8);
+    cur = (size_t *)calloc((size_t)P + 1, sizeof(size_t));
+    if (!hpp || !pc || !pv || !cur) { rc = HBICG_E_NOMEM; goto done; }
+    for (size_t e = 0; e < nz; ++e) {
+        if (hc[e] >= (unsigned)n) { rc = HBICG_E_ARG; goto done; }
+        cur[hc[e] / (unsigned)width + 1]++;
+    }
+    for (int q = 0; q < P; ++q) cur[q + 1] += cur[q];   /* cur[q]: start of panel q */
+    for (int i = 0; i < n; ++i) {
+        for (int q = 0; q < P; ++q) hpp[(size_t)q * ((size_t)n + 1) + i] = (unsigned)cur[q];
+        for (unsigned e = hptr[i]; e < hptr[i + 1]; ++e) {
+            const size_t at = cur[hc[e] / (unsigned)width]++;
+            pc[at] = hc[e];
+            pv[at] = hv[e];
+        }
+    }
+    for (int q = 0; q < P; ++q) hpp[(size_t)q * ((size_t)n + 1) + n] = (unsigned)cur[q];
+    /* each panel's tiles, binned on its own entries */
+    {
+        size_t tot = 0;
+        for (int q = 0; q < P; ++q) {
+            if ((rc = build_blocks(hpp + (size_t)q * ((size_t)n + 1), n, &qb[q], &pl->dev.pnblk[q])) != 0)
+                goto done;
+            pl->dev.pboff[q] = (unsigned)tot;
+            tot += (size_t)pl->dev.pnblk[q] + 1;
+            if (pl->dev.pnblk[q] > pl->maxparts && pl->dev.pnblk[q] > grow) grow = pl->dev.pnblk[q];
+        }
+        hbt = (unsigned *)malloc(tot * 4);
+        if (!hbt) { rc = HBICG_E_NOMEM; goto done; }
+        for (int q = 0; q < P; ++q)
+            memcpy(hbt + pl->dev.pboff[q], qb[q], ((size_t)pl->dev.pnblk[q] + 1) * 4);
+        pl->pblk = (unsigned *)dalloc(tot * 4, &rc);
+        if (rc) goto done;
+        if (hipMemcpy(pl->pblk, hbt, tot * 4, hipMemcpyHostToDevice) != hipSuccess) { rc = HBICG_E_HIP; goto done; }
+    }
+    if (grow) {
+        /* the last pass's tiles write the partial sums: room for the most
+         * tiles any pass has, in each phase's slice */
+        double *np = (double *)dalloc((size_t)grow * 6 * 8, &rc);
+        if (rc) goto done;
+        hipFree(pl->part);
+        pl->part = np;
+        pl->maxparts = grow;
+        pl->dev.part = np;
+        pl->dev.part2 = np + (size_t)grow * 2;
+        pl->dev.part3 = np + (size_t)grow * 4;
+    }
+    pl->pval = (double *)dalloc(nz * 8, &rc);
+    pl->pcol = (unsigned *)dalloc(nz * 4, &rc);
+    pl->pptr = (unsigned *)dalloc((size_t)P * ((size_t)n + 1) * 4, &rc);
+    if (rc) goto done;
+    if (hipMemcpy(pl->pval, pv, nz * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->pcol, pc, nz * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(pl->pptr, hpp, (size_t)P * ((size_t)n + 1) * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = HBICG_E_HIP;
+        goto done;
+    }
+    pl->dev.npanel = P;
+    pl->dev.pval = pl->pval;
+    pl->dev.pcol = pl->pcol;
+    pl->dev.pptr = pl->pptr;
+    pl->dev.pblk = pl->pblk;
+    pl->dev.npart = pl->dev.pnblk[P - 1];
+done:
+    for (int q = 0; q < P && q < HB_MAXPANEL; ++q) free(qb[q]);
+    free(tc); free(tv); free(hpp); free(pc); free(pv); free(cur); free(hbt);
+    return rc;
+}
+
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
                                  const hbicg_params *p, void *stream, int variant, int w_alloc,
                                  int w_cap, int allow_vi, int batched)
@@ -607,9 +714,12 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->switch_to = pl->switch_to; d->part = pl->part; d->st = pl->st; d->p_snap = pl->p_snap;
     d->part2 = pl->part + (size_t)maxparts * 2;
     d->part3 = pl->part + (size_t)maxparts * 4;
+    d->npart = pl->nblk;
+    pl->maxparts = maxparts;
     /* batches launch each phase over all their problems' tiles unfused */
     d->fused = !pl->exact && p->fused && !batched;
 
+    int scattered = 0;   /* most entries beyond 16-bit column offsets of their tile */
     /* 16-bit column offsets for the tiles whose entries stay within 32767
      * columns of the tile's first row */
     pl->col16 = (short *)dalloc((size_t)pl->nz * 2, &rc);
@@ -638,12 +748,23 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
         free(hf);
         /* columns scattered beyond 16-bit offsets for most entries: SpMV 2
          * gathers q formed beforehand rather than r and s at every entry */
-        d->q_unfused = !pl->exact && (p->q_mode == 2 || (p->q_mode == 0 && 2.0 * narrow_nz < (double)pl->nz));
+        scattered = 2.0 * narrow_nz < (double)pl->nz;
+        d->q_unfused = !pl->exact && (p->q_mode == 2 || (p->q_mode == 0 && scattered));
     }
 
     pl->val_bytes = 8.0 * (double)pl->nz;
     pl->ptr_bytes = 4.0 * ((double)n + 1.0);
     if (allow_vi && pl->nz > 0 && (rc = build_value_layout(pl, A, a_on_device, hptr, hblk)) != 0) goto fail;
+    /* column panels: 8-byte CSR reads whose gathers would otherwise range
+     * over an x larger than one XCD's L2 (4 MiB) */
+    if (!batched && !d->vidx && p->panel_cols >= 0) {
+        const int width = p->panel_cols > 0 ? p->panel_cols : HB_PANEL_COLS;
+        if (p->panel_cols > 0 || (scattered && (size_t)n * 8 > (size_t)2 * HB_PANEL_COLS * 8)) {
+            rc = build_panels(pl, A, a_on_device, hptr, width);
+            if (rc < 0) goto fail;
+            rc = 0;
+        }
+    }
 
     if (a_on_device) free(hptr);
     free(hblk);
@@ -878,6 +999,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->value_table = pl->dev.nvdict;
     R->diag_tiles = pl->n_diag_tiles;
     R->q_unfused = pl->dev.q_unfused;
+    R->panels = pl->dev.npanel > 1 ? pl->dev.npanel : 1;
     R->dot_r = st.dot_r;
     R->dot_zero = st.dot_zero;
     R->rTr = st.rTr;

@@ -221,7 +221,18 @@ DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
  * gathers r_{k-1} (in rring) and s, PLAIN gathers x. */
 enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
 
-template <int MODE, bool EXACT, bool VI = false, bool UNF = false>
+/* Column panels (matrices whose columns are scattered over an x larger than
+ * one XCD's L2): the SpMV runs as one launch per panel of columns, each over
+ * the entries of every row whose columns lie in that panel, so the gathers
+ * of all the workgroups on an XCD hit the same few MB of x.  Each row's sum
+ * is carried from pass to pass in the output vector: the first pass stores
+ * it, the middle ones read and store it, the last reads it and runs the
+ * epilogue.  A row's entries are column-ascending, so the panels take them
+ * in CSR order, and a row summed by one lane (exact order) is summed in
+ * exactly the order of one pass. */
+enum { PASS_ONE = 0, PASS_FIRST = 1, PASS_MID = 2, PASS_LAST = 3 };
+
+template <int MODE, bool EXACT, bool VI = false, bool UNF = false, int PASS = PASS_ONE>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ col, const short *__restrict__ col16,
                    const unsigned *__restrict__ cflag, const unsigned *__restrict__ ptr,
@@ -280,7 +291,12 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         return xa[c];
     };
 
-    const unsigned r0 = blk[b], r1 = blk[b + 1], narrow = __builtin_amdgcn_readfirstlane(cflag[b]);
+    const unsigned r0 = blk[b], r1 = blk[b + 1];
+    const unsigned narrow = cflag ? __builtin_amdgcn_readfirstlane(cflag[b]) : 0u;
+    /* the row sums carried in from the previous panels, and whether this
+     * pass ends them (epilogue) or hands them on */
+    constexpr bool CARRY_IN = PASS == PASS_MID || PASS == PASS_LAST;
+    constexpr bool ENDS = PASS == PASS_ONE || PASS == PASS_LAST;
     const int R = (int)(r1 - r0);
     const unsigned e0 = ptr[r0];
     const int N = (int)(ptr[r1] - e0);
@@ -314,7 +330,12 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             out1[i] = t + sig * a;
         }
     };
+    auto finish = [&](unsigned i, double t, double a, double bb, double c) {
+        if (ENDS) epilogue(i, t, a, bb, c);
+        else out1[i] = t;
+    };
     auto operands = [&](unsigned i, double &a, double &bb, double &c) {
+        if (!ENDS) return;
         if (MODE == SP_SEED1) {
             a = xa[i];
             bb = rhat[i];
@@ -480,6 +501,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             const unsigned i_me = r0 + (unsigned)min(row, R - 1);
             double oa = 0.0, ob = 0.0, oc = 0.0;
             operands(i_me, oa, ob, oc);
+            const double cin = CARRY_IN ? out1[i_me] : 0.0;
 
             constexpr int U = HB_NNZ_CAP / HB_BLOCK;
             static_assert(U * HB_BLOCK == HB_NNZ_CAP, "tile capacity");
@@ -506,7 +528,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                 if (row < R) {
                     constexpr int CH = 4;
                     const int lo = (int)rp[row], hi = (int)rp[row + 1];
-                    double acc = 0.0;
+                    double acc = cin;
                     for (int base = lo; base < hi; base += CH) {
                         double vv[CH], xv[CH];
     #pragma unroll
@@ -519,7 +541,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                         for (int j = 0; j < CH; ++j)
                             if (base + j < hi) acc += vv[j] * xv[j];
                     }
-                    epilogue(i_me, acc, oa, ob, oc);
+                    finish(i_me, acc, oa, ob, oc);
                 }
                 return;
             }
@@ -568,30 +590,36 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             for (int t = tid; t <= R; t += HB_BLOCK) rp[t] = ptr[r0 + t] - e0;
             __syncthreads();
             if (row < R) {
-                double acc = 0.0;
+                /* lane 0 starts from the sum carried in: with one lane per row
+                 * the row is summed in CSR order across the panels */
+                double acc = lane == 0 ? cin : 0.0;
                 const int hi = (int)rp[row + 1];
                 for (int e = (int)rp[row] + lane; e < hi; e += T) acc += prod[e];
                 for (int o = T >> 1; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-                if (lane == 0) epilogue(i_me, acc, oa, ob, oc);
+                if (lane == 0) finish(i_me, acc, oa, ob, oc);
             }
         } else {
             /* one row longer than the LDS stage */
             double acc = 0.0;
             if (EXACT) {
-                if (tid == 0)
+                if (tid == 0) {
+                    if (CARRY_IN) acc = out1[r0];
                     for (int e = 0; e < N; ++e) acc += vat(e0 + e) * xval(col[e0 + e]);
+                }
             } else {
                 for (int e = tid; e < N; e += HB_BLOCK) acc += vat(e0 + e) * xval(col[e0 + e]);
                 double dummy = 0.0;
                 block_sum2(acc, dummy);
+                if (CARRY_IN && tid == 0) acc = out1[r0] + acc;
             }
             if (tid == 0) {
                 double oa = 0.0, ob = 0.0, oc = 0.0;
                 operands(r0, oa, ob, oc);
-                epilogue(r0, acc, oa, ob, oc);
+                finish(r0, acc, oa, ob, oc);
             }
         }
     };
+    static_assert(!VI || PASS == PASS_ONE, "column panels read the 8-byte values");
     if constexpr (VI) {
         const double vdv = tid < d.nvdict ? vdict[tid] : 0.0;
         unsigned dk = 0, dbase = 0;
@@ -610,7 +638,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         tile();
     }
 
-    if (MODE == SP_PLAIN) return;
+    if (MODE == SP_PLAIN || !ENDS) return;
 
     block_sum2(part0, part1);
     if (tid == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
@@ -644,6 +672,25 @@ template <int MODE, bool EXACT, bool VI, bool UNF = false>
 __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
 {
     spmv_tile<MODE, EXACT, VI, UNF>(d, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+/* Pass of column panel `panel` (of d.npanel): the panel's entries of every
+ * row, in the panel-major copy of the matrix (pval, pcol; row pointers
+ * pptr + panel (n + 1)), over the panel's own tiles (up to HB_ROWS_CAP rows
+ * and HB_NNZ_CAP of the panel's entries each).  The last pass's tiles write
+ * the partial sums (d.npart of them). */
+template <int MODE, bool EXACT, bool UNF, int PASS>
+__global__ __launch_bounds__(HB_BLOCK) void k_spmv_panel(hb_dev d, int panel)
+{
+    const unsigned b = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned *pp = d.pptr + (size_t)panel * ((size_t)d.n + 1);
+    const unsigned *pb = d.pblk + d.pboff[panel];
+    if (MODE == SP_SEED1)
+        spmv_body<MODE, EXACT, false, false, PASS>(d, b, d.pval, d.pcol, nullptr, nullptr, pp, pb, d.p_set,
+                                                   d.rring, nullptr, d.rhat, d.s, nullptr, d.part, 0.0);
+    else
+        spmv_body<MODE, EXACT, false, UNF, PASS>(d, b, d.pval, d.pcol, nullptr, nullptr, pp, pb, nullptr,
+                                                 d.rring, d.s, d.rhat, d.y, d.qring, d.part2, 0.0);
 }
 
 /* q = r_{k-1} - alpha~ s for every row (matrices whose SpMV 2 would gather r
@@ -734,7 +781,7 @@ DEV void fin_spmv_body(const hb_dev &d)
             a1 = serial_dot(q, d.y, d.n, lds);
         }
     } else {
-        reduce_partials(MODE == SP_SEED1 ? d.part : d.part2, d.nblk, a0, a1);
+        reduce_partials(MODE == SP_SEED1 ? d.part : d.part2, d.npart, a0, a1);
     }
     if (tid == 0) {
         if (MODE == SP_SEED1) {
@@ -2004,6 +2051,19 @@ __global__ __launch_bounds__(HB_BLOCK) void kb_publish(const hb_dev *__restrict_
 }
 
 /* ---- launch layer --------------------------------------------------------- */
+/* One SpMV as d->npanel column-panel passes (k_spmv_panel). */
+template <int MODE, bool EXACT, bool UNF>
+static void spmv_panels(const hb_dev *d, hipStream_t s)
+{
+    const dim3 bl(HB_BLOCK);
+    const int P = d->npanel;
+    hipLaunchKernelGGL((k_spmv_panel<MODE, EXACT, UNF, PASS_FIRST>), dim3(d->pnblk[0]), bl, 0, s, *d, 0);
+    for (int p = 1; p + 1 < P; ++p)
+        hipLaunchKernelGGL((k_spmv_panel<MODE, EXACT, UNF, PASS_MID>), dim3(d->pnblk[p]), bl, 0, s, *d, p);
+    hipLaunchKernelGGL((k_spmv_panel<MODE, EXACT, UNF, PASS_LAST>), dim3(d->pnblk[P - 1]), bl, 0, s, *d,
+                       P - 1);
+}
+
 template <int WM, int V, int U>
 static void launch_shift(const hb_dev *d, hipStream_t s)
 {
@@ -2062,6 +2122,20 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
     if (d->fused && !d->exact &&
         (stage == HB_STAGE_FIN2 || stage == HB_STAGE_SEED || stage == HB_STAGE_COEF || stage == HB_STAGE_SHIFT))
         return stage_fused(d, stage, s);
+    if (d->npanel > 1 && (stage == HB_STAGE_SPMV1 || stage == HB_STAGE_SPMV2)) {
+        if (stage == HB_STAGE_SPMV2 && d->q_unfused) hipLaunchKernelGGL(k_qform, gv, bl, 0, s, *d);
+        if (stage == HB_STAGE_SPMV1) {
+            if (d->exact) spmv_panels<SP_SEED1, true, false>(d, s);
+            else spmv_panels<SP_SEED1, false, false>(d, s);
+        } else if (d->exact) {
+            spmv_panels<SP_SEED2, true, false>(d, s);
+        } else if (d->q_unfused) {
+            spmv_panels<SP_SEED2, false, true>(d, s);
+        } else {
+            spmv_panels<SP_SEED2, false, false>(d, s);
+        }
+        return (int)hipGetLastError();
+    }
     switch (stage) {
     case HB_STAGE_SPMV1:
         if (d->exact) {

@@ -22,6 +22,9 @@ extern "C" {
 #define HB_DIAG_K       16    /* most diagonals of a diagonal tile             */
 #define HB_DIAG_NONE    255   /* index byte of a diagonal a row has no entry on */
 #define HB_CC_ROWS(W)   (2 * (W) + 5) /* rows of one chunk's coefficient block  */
+/* default column-panel width of the seed SpMVs: 2 MiB of x, half an XCD's L2 */
+#define HB_PANEL_COLS 262144
+#define HB_MAXPANEL   32      /* most panels; wider panels beyond               */
 #define HB_RESID_PAIRS  65536 /* most partial pairs of one true-residual evaluation */
 
 /* Error bits in hb_state.status */
@@ -124,6 +127,18 @@ typedef struct hb_dev {
     HB_G const unsigned char *didx;  /* index bytes of the diagonal tiles            */
     HB_G const double   *vdict;      /* [nvdict <= HB_VDICT] the matrix's distinct values */
     int nvdict;
+    int npanel;                      /* > 1: the seed SpMVs run as this many column-panel
+                                        passes over pval/pcol/pptr                    */
+    HB_G const double   *pval;       /* [nz] the entries panel by panel, each panel's  */
+    HB_G const unsigned *pcol;       /*      rows in order, a row's entries in CSR order */
+    HB_G const unsigned *pptr;       /* [npanel][n+1] row starts of each panel in pval */
+    HB_G const unsigned *pblk;       /* each panel's tiles (row starts), binned on that
+                                        panel's entries: panel p's [pnblk[p]+1] start
+                                        at pblk + pboff[p]                             */
+    int pnblk[HB_MAXPANEL];
+    unsigned pboff[HB_MAXPANEL];
+    int npart;                       /* partial-sum pairs the seed SpMVs write: nblk, or
+                                        the last panel's tile count                    */
     HB_G const double   *sigma;/* [M]                                      */
     HB_G const double   *rhat; /* [n]   shadow residual r^ = b             */
     HB_G double *rring;        /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */

@@ -46,7 +46,8 @@ class Params(C.Structure):
                 ("exact_order", C.c_int), ("graph", C.c_int), ("chunk", C.c_int),
                 ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int),
                 ("profile", C.c_int), ("growth_limit", C.c_double), ("trace", C.c_int),
-                ("value_index", C.c_int), ("q_mode", C.c_int), ("fused", C.c_int)]
+                ("value_index", C.c_int), ("q_mode", C.c_int), ("fused", C.c_int),
+                ("panel_cols", C.c_int)]
 
 
 class KernelStat(C.Structure):
@@ -62,7 +63,7 @@ class Report(C.Structure):
                 ("dot_r", C.c_double), ("dot_zero", C.c_double), ("rTr", C.c_double),
                 ("t_setup_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_total_ms", C.c_double),
                 ("window", C.c_int), ("value_table", C.c_int),
-                ("diag_tiles", C.c_int), ("q_unfused", C.c_int)]
+                ("diag_tiles", C.c_int), ("q_unfused", C.c_int), ("panels", C.c_int)]
 
 
 class Tuning(C.Structure):

@@ -19,7 +19,8 @@ VARIANTS = ("normal", "static", "dynamic")
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
                 device=0, exit_on_error=False, upload_x=True, profile=False,
-                growth_limit=0.0, trace=False, value_index=True, q_mode=0, fused=True) -> _lib.Params:
+                growth_limit=0.0, trace=False, value_index=True, q_mode=0, fused=True,
+                panel_cols=0) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -36,6 +37,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.value_index = int(bool(value_index))
     p.q_mode = int(q_mode)
     p.fused = int(bool(fused))
+    p.panel_cols = int(panel_cols)
     return p
 
 

@@ -84,6 +84,16 @@ typedef struct hbicg_params {
                             the shift kernel's first workgroups; 0: eight launches.
                             Same results bit for bit; the batched solver always uses
                             eight, exact order seven (no coefficient kernel)            */
+    int    panel_cols;   /* seed SpMVs in column panels of this many columns: one pass
+                            per panel over the entries whose columns lie in it, row sums
+                            carried between passes, so the gathers of a pass stay within
+                            one XCD's L2.  0 (default) auto: 262144 columns (2 MiB of x)
+                            for a matrix read as 8-byte CSR whose columns are scattered
+                            beyond 16-bit tile offsets, with x over 4 MiB and every row's
+                            columns ascending; -1 off.  Needs ascending columns (else
+                            off).  Exact order is bitwise unchanged (one lane sums a row
+                            in CSR order across the passes); the default order sums a
+                            row's panels in turn, a deterministic regrouping           */
 } hbicg_params;
 
 typedef struct hbicg_report {
@@ -103,6 +113,7 @@ typedef struct hbicg_report {
     int    q_unfused;    /* 1: most entries' columns lie beyond 16-bit offsets of their
                             tile, so SpMV 2 gathered q = r - alpha s formed beforehand
                             (one gather per entry instead of two); results unchanged */
+    int    panels;       /* column panels of the seed SpMVs (1: one pass)             */
 } hbicg_report;
 
 void hbicg_default_params(hbicg_params *p);

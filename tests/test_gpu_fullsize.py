@@ -31,8 +31,13 @@ The irregular power-law matrix (n = 1e6, 31M nonzeros, 128 shifts): the
 first iterations run by the oracle on the host and by the GPU with the same
 cap agree bit for bit in exact order and within 1e-9 in the default order;
 the complete solve meets the stopping rule for every shift with sigma >=
-0.025 (the smallest shifts, the seed among them, do not reach 1e-10 within
-the 2000-iteration cap, so the solve returns cap + 1 like the reference).
+0.05, and every shift it stops has a true residual within 10x the rule.
+The smallest shifts, the seed among them, do not reach 1e-10 within the
+2000-iteration cap, so the solve returns cap + 1 like the reference.  The
+shifts just above them stop near the cap, where summation order decides:
+measured on MI355X, sigma = 0.0284 stops at iteration 1903 in the default
+order with one SpMV pass, and not at all with column panels or in the
+reference's order; stop iterations there move by up to 318 between orders.
 """
 import os
 
@@ -221,7 +226,7 @@ def test_c2_full_solve_true_residuals(c2):
         tr = pl.true_residuals(b, c2.sigma, x)
         stopped = stop > 0
         assert rep["stop_count"] == int(stopped.sum())
-        assert np.all(stopped[c2.sigma >= 0.025])
+        assert np.all(stopped[c2.sigma >= 0.05])
         assert k == c2.max_iter + 1 if not np.all(stopped) else k < c2.max_iter + 1
         assert tr[stopped].max() <= 10 * c2.tol, tr[stopped].max()
         xs.append(x)
