@@ -106,9 +106,10 @@ def parse():
     ap.add_argument("--value-index", type=int, default=1, choices=[0, 1],
                     help="1 (library default): SpMV reads the matrix through its value table "
                          "(and diagonal tiles) where it qualifies; 0: 8-byte CSR values")
-    ap.add_argument("--fused", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--fused", type=int, default=1, choices=[0, 1, 2],
                     help="1 (library default): five launches per iteration (finalizes and combined "
-                         "coefficients run inside the neighbouring launches); 0: eight")
+                         "coefficients run inside the neighbouring launches); 2: six (alpha~ "
+                         "finalized on its own); 0: eight")
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="iteration cap of the timed oracle solve (the workload's own cap, 2000, "
                          "times the complete solve; 0: skip)")
@@ -537,7 +538,7 @@ def main():
     graph = 1 if args.variant == "dynamic" else 0
     window = args.window if args.window > 0 else (1 if args.variant == "normal" else 0)
     params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph,
-                         value_index=bool(args.value_index), fused=bool(args.fused))
+                         value_index=bool(args.value_index), fused=args.fused)
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)
     x = torch.zeros((M, n), dtype=torch.float64, device=dev)
@@ -572,7 +573,7 @@ def main():
     if value_table or not args.value_index:
         gplan = Plan(prob.A, M, make_params(tol=prob.tol, max_iter=prob.max_iter, window=window,
                                             graph=graph, value_index=not value_table,
-                                            fused=bool(args.fused)), device=dev)
+                                            fused=args.fused), device=dev)
         xg = torch.zeros_like(x)
         gplan.solve(b, prob.sigma, prob.seed, xg)
         torch.cuda.synchronize()
@@ -608,7 +609,7 @@ def main():
     kernels, roofline, flush_roofline = {}, None, None
     if not args.no_profile:
         pparams = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=0,
-                              profile=True, value_index=bool(args.value_index), fused=bool(args.fused))
+                              profile=True, value_index=bool(args.value_index), fused=args.fused)
         pplan = Plan(prob.A, M, pparams, device=dev)
         xp = torch.zeros_like(x)
         pplan.solve(b, prob.sigma, prob.seed, xp)
@@ -696,7 +697,7 @@ def main():
         "config": {"workload": f"conv-diff 3-D 7-point {args.grid}^3 (n={n}, nnz={prob.A.nnz}), "
                                f"{M} shifts log-spaced [1e-3,1], relres {prob.tol:g}, seed=smallest shift",
                    "variant": args.variant, "window": window, "hipgraph": bool(graph),
-                   "fused_launches": bool(args.fused),
+                   "fused_launches": args.fused,
                    "spmv_value_table": value_table, "spmv_diag_tiles": diag_tiles},
         "other_spmv_format": general,
         "iterations": iters,

@@ -766,6 +766,9 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
         }
     }
 
+    /* alpha~'s finalize joins SpMV 2 where that launch forms q at its
+     * gathered columns in one pass */
+    d->fused_a = d->fused && !d->q_unfused && d->npanel <= 1 && p->fused == 1;
     if (a_on_device) free(hptr);
     free(hblk);
     return pl;

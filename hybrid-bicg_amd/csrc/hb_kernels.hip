@@ -172,26 +172,30 @@ DEV void mail_put_d(unsigned long long *slot, double v)
     mail_put(slot, (unsigned long long)__double_as_longlong(v));
 }
 
+/* the published double, polled by the calling thread alone */
+DEV double mail_poll(unsigned long long *slot, hb_state *st)
+{
+    unsigned long long v;
+    for (int spins = 0;; ++spins) {
+        v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v != HB_MAIL_EMPTY) break;
+        if (spins == HB_SPIN_LIMIT || ((spins & 255) == 255 && sync_timed_out(st))) {
+            atomicOr(&st->status, HB_ST_SYNC_TIMEOUT);
+            break;
+        }
+        /* a few hundred cycles between polls: thousands of resident
+         * workgroups poll one word, and the publisher needs the memory
+         * system more than they do */
+        __builtin_amdgcn_s_sleep(8);
+    }
+    return __longlong_as_double((long long)v);
+}
+
 /* the published double; thread 0 of the workgroup waits, every thread gets it */
 DEV double mail_wait(unsigned long long *slot, hb_state *st)
 {
     __shared__ double sh_v;
-    if (threadIdx.x == 0) {
-        unsigned long long v;
-        for (int spins = 0;; ++spins) {
-            v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (v != HB_MAIL_EMPTY) break;
-            if (spins == HB_SPIN_LIMIT || ((spins & 255) == 255 && sync_timed_out(st))) {
-                atomicOr(&st->status, HB_ST_SYNC_TIMEOUT);
-                break;
-            }
-            /* a few hundred cycles between polls: thousands of resident
-             * workgroups poll one word, and the publisher needs the memory
-             * system more than they do */
-            __builtin_amdgcn_s_sleep(8);
-        }
-        sh_v = __longlong_as_double((long long)v);
-    }
+    if (threadIdx.x == 0) sh_v = mail_poll(slot, st);
     __syncthreads();
     return sh_v;
 }
@@ -232,7 +236,7 @@ enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
  * exactly the order of one pass. */
 enum { PASS_ONE = 0, PASS_FIRST = 1, PASS_MID = 2, PASS_LAST = 3 };
 
-template <int MODE, bool EXACT, bool VI = false, bool UNF = false, int PASS = PASS_ONE>
+template <int MODE, bool EXACT, bool VI = false, bool UNF = false, int PASS = PASS_ONE, bool FA = false>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ col, const short *__restrict__ col16,
                    const unsigned *__restrict__ cflag, const unsigned *__restrict__ ptr,
@@ -277,7 +281,9 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             zscale = zs;
             snap = !d.exact && k == wk0;
         } else {
-            nalpha = -d.aa[k];
+            /* FA: alpha~ comes from the launch's first workgroup, read once
+             * the tile's own loads are in flight */
+            if (!FA) nalpha = -d.aa[k];
             qout = qring + off(k % d.W, n);
             /* UNF: q was formed for every row by k_qform; gather it */
             xa = UNF ? qout : rprev;
@@ -397,6 +403,17 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                 xv[j] = xval((unsigned)min(max(cs, 0), n - 1));
             }
         };
+        /* FA: the first eight gathers load r and s apart; alpha~ arrives
+         * while they are in flight, and q = r - alpha~ s is formed after */
+        double sv8[FA ? 8 : 1];
+        auto gather8_rs = [&]() {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const unsigned c = (unsigned)min(max((int)i_me + o[j], 0), n - 1);
+                xv[j] = xa[c];
+                sv8[FA ? j : 0] = svec[c];
+            }
+        };
         double acc = 0.0;
         auto sum8 = [&](int h) {
 #pragma unroll
@@ -406,9 +423,20 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                 acc = ix != HB_DIAG_NONE ? t : acc;
             }
         };
-        gather8(0);
-        if (tid < d.nvdict) vd[tid] = vdv;
-        __syncthreads();
+        if constexpr (FA) {
+            __shared__ double sh_na;
+            gather8_rs();
+            if (tid < d.nvdict) vd[tid] = vdv;
+            if (tid == 0) sh_na = -mail_poll(&d.st->mail_a, d.st);
+            __syncthreads();
+            nalpha = sh_na;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) xv[j] = xv[j] + nalpha * sv8[j];
+        } else {
+            gather8(0);
+            if (tid < d.nvdict) vd[tid] = vdv;
+            __syncthreads();
+        }
         sum8(0);
         if (dk > 8) {
             gather8(8);
@@ -630,11 +658,13 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         if (dk) {
             tile_dia(dk, dbase, vdv);
         } else {
+            if (FA) nalpha = -mail_wait(&d.st->mail_a, d.st);
             if (tid < d.nvdict) vd[tid] = vdv;
             __syncthreads();
             tile_vi();
         }
     } else {
+        if (FA) nalpha = -mail_wait(&d.st->mail_a, d.st);
         tile();
     }
 
@@ -760,12 +790,13 @@ DEV void fin_spmv_body(const hb_dev &d)
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     hb_state *st = d.st;
     if (st->done) return;
-    /* PUB (omega~ only): this workgroup runs first in the seed update's
-     * launch and hands omega~ to that launch's other workgroups.  The
-     * mailbox is empty already (the scalar stage empties it); emptying it
-     * again here keeps a back-to-back repeat of the launch waiting too. */
-    static_assert(!PUB || MODE == SP_SEED2, "only omega~ is published");
-    if (PUB && threadIdx.x == 0) mail_put(&st->mail_w, HB_MAIL_EMPTY);
+    /* PUB: this workgroup runs first in the next phase's launch (SpMV 2
+     * for alpha~, the seed update for omega~) and hands the value to that
+     * launch's other workgroups.  The mailbox is empty already (the scalar
+     * stage empties it); emptying it again here keeps a back-to-back repeat
+     * of the launch waiting too. */
+    unsigned long long *mail = MODE == SP_SEED1 ? &st->mail_a : &st->mail_w;
+    if (PUB && threadIdx.x == 0) mail_put(mail, HB_MAIL_EMPTY);
     const int k = st->k, tid = threadIdx.x;
     const int switched = (MODE == SP_SEED1) ? st->switched : 0;
     const double rTr = st->rTr;
@@ -791,6 +822,7 @@ DEV void fin_spmv_body(const hb_dev &d)
             st->rTs = a0;
             d.aa[k] = alpha;
             if (a0 == 0.0 || !isfinite(alpha)) st->status |= HB_ST_BREAKDOWN_RTS;
+            if (PUB) mail_put_d(mail, alpha);
         } else {
             const double omega = a0 / a1;
             st->qTq = a0;
@@ -798,7 +830,7 @@ DEV void fin_spmv_body(const hb_dev &d)
             d.wa[k] = omega;
             st->rTr_old = rTr;
             if (a1 == 0.0 || !isfinite(omega)) st->status |= HB_ST_BREAKDOWN_QTY;
-            if (PUB) mail_put_d(&st->mail_w, omega);
+            if (PUB) mail_put_d(mail, omega);
         }
     }
 }
@@ -807,6 +839,24 @@ template <int MODE, bool EXACT>
 __global__ __launch_bounds__(HB_BLOCK) void k_fin_spmv(hb_dev d)
 {
     fin_spmv_body<MODE, EXACT>(d);
+}
+
+/* Fused SpMV 2 (default mode, q formed at the gathered columns): workgroup
+ * 0 finalizes alpha~ from SpMV 1's partials and publishes it; the others
+ * are the tiles, which issue their loads and gathers first and read alpha~
+ * after (src/solver_dynamic.c:163-184). */
+template <bool VI>
+__global__ __launch_bounds__(HB_BLOCK) void k_spmv2_fa(hb_dev d)
+{
+    if (blockIdx.x == 0) {
+        fin_spmv_body<SP_SEED1, false, true>(d);
+        return;
+    }
+    const unsigned b = xcd_tile(blockIdx.x - 1, gridDim.x - 1);
+    spmv_body<SP_SEED2, false, VI, false, PASS_ONE, true>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk,
+                                                         nullptr, d.rring, d.s, d.rhat, d.y, d.qring, d.part2,
+                                                         0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
+                                                         d.doff, d.didx);
 }
 
 /* ---- scalar stage (one workgroup per problem, after the seed update) -------
@@ -1106,18 +1156,48 @@ DEV void seed_update_body(const hb_dev &d, int b, const double *__restrict__ qri
     hb_state *st = d.st;
     if (st->done) return;
     const int k = st->k, s = st->seed, n = d.n, nb = d.vec_grid;
-    /* MAIL: omega~ is finalized by the launch's first workgroup */
-    const double a = d.aa[k], w = MAIL ? mail_wait(&st->mail_w, st) : d.wa[k], nw = -w;
     const double *q = qring + off(k % d.W, n);
     double *rn = rring + off(k % (d.W + 1), n);
     const double *ps = pset + off(s, n);
     double *xs = xset + off(s, n);
     double part0 = 0.0, part1 = 0.0;
+    int i = b * HB_BLOCK + threadIdx.x;
+    const int stride = nb * HB_BLOCK;
+    double w;
+    if (MAIL) {
+        /* omega~ comes from the launch's first workgroup: the first four
+         * elements' q, y and r^ are loaded while it is on its way, and
+         * summed in the same order as the loop below */
+        constexpr int E = 4;
+        double qv[E], yv[E], hv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ie = min(i + e * stride, n - 1);
+            qv[e] = q[ie];
+            yv[e] = y[ie];
+            hv[e] = rhat[ie];
+        }
+        w = mail_wait(&st->mail_w, st);
+        const double nw = -w;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (i < n) {
+                const double ri = qv[e] + nw * yv[e];
+                rn[i] = ri;
+                part0 += ri * ri;
+                part1 += hv[e] * ri;
+            }
+            i += stride;
+        }
+    } else {
+        w = d.wa[k];
+    }
+    const double a = d.aa[k], nw = -w;
     /* exact order updates x_s here, in the reference's order; the default
      * mode defers it to the window flush (the seed is listed there) */
     const bool upd_x = d.exact;
 #pragma unroll 2
-    for (int i = b * HB_BLOCK + threadIdx.x; i < n; i += nb * HB_BLOCK) {
+    for (; i < n; i += stride) {
         const double qi = q[i];
         if (upd_x) {
             double xi = xs[i];
@@ -1179,6 +1259,7 @@ DEV void fin_seed_body(const hb_dev &d)
         /* the fused launches' readers of omega~ and of the coefficient
          * count finished with the last launches */
         st->mail_w = HB_MAIL_EMPTY;
+        st->mail_a = HB_MAIL_EMPTY;
         st->coef_ready = 0;
     }
     __syncthreads();
@@ -1873,6 +1954,7 @@ DEV void init_fin_body(const hb_dev &d)
         st->flush_fma = 0.0;
         st->coef_ready = 0;
         st->mail_w = HB_MAIL_EMPTY;
+        st->mail_a = HB_MAIL_EMPTY;
     }
 }
 
@@ -2101,6 +2183,12 @@ static int stage_fused(const hb_dev *d, int stage, hipStream_t s)
 {
     const dim3 bl(HB_BLOCK);
     switch (stage) {
+    case HB_STAGE_FIN1:
+        break;
+    case HB_STAGE_SPMV2:
+        if (d->vidx) hipLaunchKernelGGL(k_spmv2_fa<true>, dim3(d->nblk + 1), bl, 0, s, *d);
+        else hipLaunchKernelGGL(k_spmv2_fa<false>, dim3(d->nblk + 1), bl, 0, s, *d);
+        break;
     case HB_STAGE_SEED:
         hipLaunchKernelGGL(k_seed_update_fin, dim3(d->vec_grid + 1), bl, 0, s, *d);
         break;
@@ -2120,7 +2208,8 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
 {
     const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK), one(1);
     if (d->fused && !d->exact &&
-        (stage == HB_STAGE_FIN2 || stage == HB_STAGE_SEED || stage == HB_STAGE_COEF || stage == HB_STAGE_SHIFT))
+        (stage == HB_STAGE_FIN2 || stage == HB_STAGE_SEED || stage == HB_STAGE_COEF || stage == HB_STAGE_SHIFT ||
+         (d->fused_a && (stage == HB_STAGE_FIN1 || stage == HB_STAGE_SPMV2))))
         return stage_fused(d, stage, s);
     if (d->npanel > 1 && (stage == HB_STAGE_SPMV1 || stage == HB_STAGE_SPMV2)) {
         if (stage == HB_STAGE_SPMV2 && d->q_unfused) hipLaunchKernelGGL(k_qform, gv, bl, 0, s, *d);

@@ -82,6 +82,7 @@ typedef struct hb_state {
      * published by the launch's first workgroup to its other workgroups;
      * HB_MAIL_EMPTY until then (the scalar stage empties it). */
     unsigned long long mail_w;
+    unsigned long long mail_a;   /* alpha~ handed from SpMV 2's first workgroup */
 } hb_state;
 
 /* Every pointer below addresses global device memory.  Device code sees
@@ -139,6 +140,9 @@ typedef struct hb_dev {
     unsigned pboff[HB_MAXPANEL];
     int npart;                       /* partial-sum pairs the seed SpMVs write: nblk, or
                                         the last panel's tile count                    */
+    int fused_a;                     /* the alpha~ finalize runs as SpMV 2's first
+                                        workgroup (fused launches, q formed at the
+                                        gathered columns, one pass)                    */
     HB_G const double   *sigma;/* [M]                                      */
     HB_G const double   *rhat; /* [n]   shadow residual r^ = b             */
     HB_G double *rring;        /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */

@@ -36,7 +36,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.trace = int(bool(trace))
     p.value_index = int(bool(value_index))
     p.q_mode = int(q_mode)
-    p.fused = int(bool(fused))
+    p.fused = int(fused) if not isinstance(fused, bool) else int(fused)
     p.panel_cols = int(panel_cols)
     return p
 

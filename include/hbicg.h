@@ -77,13 +77,17 @@ typedef struct hbicg_params {
                             within 16-bit column offsets of their tile (stencils), else
                             formed for every row first and gathered; 1: always at the
                             gathered columns; 2: always beforehand.  Same results.        */
-    int    fused;        /* default mode: 1 (default) six launches per iteration -- the
-                            omega~ finalize runs as the first workgroup of the seed
-                            update's launch, which hands omega~ to its other
-                            workgroups, and the combined window coefficients run in
-                            the shift kernel's first workgroups; 0: eight launches.
-                            Same results bit for bit; the batched solver always uses
-                            eight, exact order seven (no coefficient kernel)            */
+    int    fused;        /* default mode: 1 (default) five launches per iteration --
+                            the alpha~ finalize runs as the first workgroup of SpMV
+                            2's launch (where SpMV 2 forms q at its gathered columns
+                            in one pass; else it keeps its own launch), the omega~
+                            finalize as the first workgroup of the seed update's, each
+                            handing its value to the launch's other workgroups, and
+                            the combined window coefficients run in the shift
+                            kernel's first workgroups; 2: the last two only (six
+                            launches); 0: eight launches.  Same results bit for bit;
+                            the batched solver always uses eight, exact order seven
+                            (no coefficient kernel)                                     */
     int    panel_cols;   /* seed SpMVs in column panels of this many columns: one pass
                             per panel over the entries whose columns lie in it, row sums
                             carried between passes, so the gathers of a pass stay within
@@ -196,10 +200,11 @@ int hbicg_plan_spmv(hbicg_plan *pl, const double *x, double *y, double shift);
  * applied a window flush.  launches counts the launches (executed
  * iterations), total_ms the summed HIP-event durations, bytes the
  * algorithmic HBM bytes those launches had to move, flops ([8] only) twice
- * the flush's multiply-adds.  With the six fused launches (params.fused, the
- * default order) [3] and [6] launch nothing: the omega~ finalize is timed
- * in [4], the coefficients in [7] and [8].  Returns the number of entries
- * filled (at most 9). */
+ * the flush's multiply-adds.  With the fused launches (params.fused, the
+ * default order) [3] and [6] launch nothing, and [1] too where alpha~ joins
+ * SpMV 2: the alpha~ finalize is then timed in [2], omega~'s in [4], the
+ * coefficients in [7] and [8].  Returns the number of entries filled (at
+ * most 9). */
 typedef struct hbicg_kernel_stat {
     int    launches;
     double total_ms;

@@ -442,21 +442,48 @@ def test_ragged_sizes(M, seed):
 @pytest.mark.parametrize("value_index", [True, False])
 @pytest.mark.parametrize("seed", [0, 8])
 def test_fused_launches_bitwise(c0, seed, value_index):
-    """Default mode: the six-launch iteration (omega~ finalized by the first
-    workgroup of the seed update's launch and handed to its other
-    workgroups, combined coefficients formed in the shift kernel's first
-    workgroups) is the eight-launch iteration's arithmetic: k, stop
+    """Default mode: the five-launch iteration (alpha~ finalized by the first
+    workgroup of SpMV 2's launch, omega~ by the first of the seed update's,
+    each handed to the launch's other workgroups; combined coefficients
+    formed in the shift kernel's first workgroups) and the six-launch one
+    (alpha~ on its own) are the eight-launch iteration's arithmetic: k, stop
     iterations, switches, x and r bit for bit, and no workgroup timed out
-    waiting."""
+    waiting.  With the value table SpMV 2 reads diagonal tiles (alpha~ read
+    after the gathers are issued); without it, 8-byte CSR tiles (alpha~
+    read first)."""
     for variant in ("static", "dynamic"):
         print(f"case fused vs unfused c0 seed={seed} {variant} value_index={value_index}", flush=True)
         mk = lambda f: make_params(tol=c0.tol, fused=f, value_index=value_index)
-        a = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(True))
-        b = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(False))
-        assert a.k == b.k and a.switches == b.switches
-        np.testing.assert_array_equal(a.stop_iter, b.stop_iter)
-        assert np.array_equal(a.x, b.x) and np.array_equal(a.r, b.r)
-        assert not (a.report["status"] & 16) and not (b.report["status"] & 16)
+        ref = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(0))
+        assert not (ref.report["status"] & 16)
+        for level in (1, 2):
+            a = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(level))
+            assert a.k == ref.k and a.switches == ref.switches
+            np.testing.assert_array_equal(a.stop_iter, ref.stop_iter)
+            assert np.array_equal(a.x, ref.x) and np.array_equal(a.r, ref.r)
+            assert not (a.report["status"] & 16)
+
+
+def test_fused_alpha_value_table_csr_tiles():
+    """A matrix with a value table whose tiles are not stored by diagonals
+    (scattered columns, values from a set of five), with q formed at the
+    gathered columns so alpha~ joins SpMV 2: bit for bit the eight-launch
+    result."""
+    A0 = P.zipf_matrix(20000, 12.0, 300, seed=9)
+    rows = np.repeat(np.arange(A0.n), np.diff(A0.ptr.astype(np.int64)))
+    vals = np.where(A0.col == rows, 200.0, np.array([-1.0, -0.5, 0.25, 0.5])[A0.col % 4])
+    A = P.CSR(A0.ptr, A0.col, vals, A0.n)
+    b = P.uniform(9, 2, A.n, -1.0, 1.0)
+    sigma = P.uniform(9, 3, 16, 0.05, 0.5)
+    seed = int(np.argmin(sigma))
+    print("case fused alpha, value table, CSR tiles", flush=True)
+    mk = lambda f: make_params(tol=1e-10, fused=f, window=16, q_mode=1)
+    a = dropin_solve(A, b, sigma, seed, "dynamic", mk(1))
+    c = dropin_solve(A, b, sigma, seed, "dynamic", mk(0))
+    assert a.report["value_table"] == 5 and a.report["diag_tiles"] == 0 and a.report["q_unfused"] == 0
+    assert a.k == c.k and a.switches == c.switches and a.report["stop_count"] == len(sigma)
+    assert np.array_equal(a.x, c.x) and np.array_equal(a.r, c.r)
+    assert not (a.report["status"] & 16)
 
 
 def test_fused_launches_bitwise_gathered_q():
