@@ -124,11 +124,12 @@ def parse():
                     help="skip the host-buffer (PCIe-inclusive) drop-in call of the headline workload")
     ap.add_argument("--no-powerlaw", action="store_true",
                     help="skip the 1e6-row power-law matrix leg (128 shifts)")
-    ap.add_argument("--mode", default="headline", choices=["headline", "sweep", "repeat", "powerlaw"],
+    ap.add_argument("--mode", default="headline",
+                    choices=["headline", "sweep", "repeat", "powerlaw", "batched"],
                     help="headline: the 128^3 / 64-shift solve and its legs; sweep: shift counts "
                          "1..--sweep-max x {normal, dynamic} (src/main_seed_diff.c); repeat: one "
                          "solve of --repeat-shifts shifts timed --repeats times (src/main_repeat.c); "
-                         "powerlaw: the power-law leg alone (for counter runs)")
+                         "powerlaw, batched: that leg alone (for counter and trace runs)")
     ap.add_argument("--panel-cols", type=int, default=0,
                     help="power-law leg: SpMV column-panel width (0 auto, -1 off)")
     ap.add_argument("--pl-iters", type=int, default=0,
@@ -487,6 +488,15 @@ def main():
         return sweep_mode(args)
     if args.mode == "repeat":
         return repeat_mode(args)
+    if args.mode == "batched":
+        out = batched_leg(args, log)
+        print(json.dumps({
+            "metric": METRIC, "value": out["batched_shift_solves_per_s"], "unit": "shift-solves/s",
+            "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": out["batched_ms"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": f"{out['problems']} problems", "mode": "batched"},
+            "batched": out}))
+        return
     if args.mode == "powerlaw":
         out = powerlaw_leg(args, log)
         print(json.dumps({

@@ -1250,6 +1250,10 @@ int shifted_lopbicg_normal(CSR_Matrix *A_loc_diag, CSR_Matrix *A_loc_offd, INFO_
  * the tiles of every problem, found through (problem, tile) tables; done
  * problems' tiles return at once.  One chunk of iterations is replayed as a
  * hipGraph, and one device flag reports when every problem is done. */
+/* chunk graphs by launch width: level l gives the phase launches
+ * ceil(full / 2^l) workgroups */
+#define HB_GLEVELS 7
+
 struct hbicg_batch {
     int nprob, device, exact, chunk, graph, W, group_cap;
     hipStream_t stream;
@@ -1258,12 +1262,16 @@ struct hbicg_batch {
     hb_dev *hdevs;             /* host copies of the per-problem records */
     hb_dev *ddevs;             /* device array                           */
     int *t_spmv, *t_vec, *t_shift, *t_coef;
+    int *a_tab[HB_NPH], *act;  /* device: active lists and their lengths */
+    int *hpofs, *hpcnt;        /* host [HB_NPH][nprob]: each problem's range in the full tables */
+    int *pofs, *pcnt;          /* device copies                          */
     hb_batch B;
-    int *flag_host;
+    int *flag_host;            /* pinned: done, problems running, active entries per phase */
     hipEvent_t ev[2], t0, t1;
-    hipGraphExec_t gexec;
-    hipGraph_t graph_obj;
+    hipGraphExec_t gexec[HB_GLEVELS];
+    hipGraph_t graph_obj[HB_GLEVELS];
     int graph_dirty;
+    int levels_used;           /* bit l: the last run launched level l */
     double last_ms;
 };
 
@@ -1272,8 +1280,16 @@ void hbicg_batch_destroy(hbicg_batch *bt)
     if (!bt) return;
     hipSetDevice(bt->device);
     if (bt->stream) hipStreamSynchronize(bt->stream);
-    if (bt->gexec) hipGraphExecDestroy(bt->gexec);
-    if (bt->graph_obj) hipGraphDestroy(bt->graph_obj);
+    for (int l = 0; l < HB_GLEVELS; ++l) {
+        if (bt->gexec[l]) hipGraphExecDestroy(bt->gexec[l]);
+        if (bt->graph_obj[l]) hipGraphDestroy(bt->graph_obj[l]);
+    }
+    for (int ph = 0; ph < HB_NPH; ++ph) if (bt->a_tab[ph]) hipFree(bt->a_tab[ph]);
+    if (bt->act) hipFree(bt->act);
+    if (bt->pofs) hipFree(bt->pofs);
+    if (bt->pcnt) hipFree(bt->pcnt);
+    free(bt->hpofs);
+    free(bt->hpcnt);
     if (bt->plans)
         for (int p = 0; p < bt->nprob; ++p) hbicg_plan_destroy(bt->plans[p]);
     free(bt->plans);
@@ -1329,9 +1345,23 @@ static int batch_layout(hbicg_batch *bt, int W, int group_cap)
     }
     if (bt->t_shift) hipFree(bt->t_shift);
     bt->t_shift = (int *)dalloc(tiles * 2 * sizeof(int), &rc);
+    if (bt->a_tab[HB_PH_SHIFT]) hipFree(bt->a_tab[HB_PH_SHIFT]);
+    bt->a_tab[HB_PH_SHIFT] = (int *)dalloc(tiles * 2 * sizeof(int), &rc);
     if (rc) { free(tab); return rc; }
     HIPCK(hipMemcpy(bt->t_shift, tab, tiles * 2 * sizeof(int), hipMemcpyHostToDevice));
+    {
+        int o = 0;
+        for (int p = 0; p < bt->nprob; ++p) {
+            const int c = bt->hdevs[p].shift_gx * bt->hdevs[p].shift_gy;
+            bt->hpofs[HB_PH_SHIFT * bt->nprob + p] = o;
+            bt->hpcnt[HB_PH_SHIFT * bt->nprob + p] = c;
+            o += c;
+        }
+    }
+    HIPCK(hipMemcpy(bt->pofs, bt->hpofs, (size_t)HB_NPH * bt->nprob * sizeof(int), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(bt->pcnt, bt->hpcnt, (size_t)HB_NPH * bt->nprob * sizeof(int), hipMemcpyHostToDevice));
     bt->B.t_shift = bt->t_shift;
+    bt->B.a_tab[HB_PH_SHIFT] = bt->a_tab[HB_PH_SHIFT];
     bt->B.n_shift_tiles = (int)tiles;
     bt->B.W = W;
     bt->W = W;
@@ -1404,8 +1434,38 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
         HIPCK(hipMemcpy(bt->t_vec, tab, nvec * 2 * sizeof(int), hipMemcpyHostToDevice));
     }
     bt->ddevs = (hb_dev *)dalloc((size_t)nprob * sizeof(hb_dev), &rc);
+    bt->hpofs = (int *)calloc((size_t)HB_NPH * nprob, sizeof(int));
+    bt->hpcnt = (int *)calloc((size_t)HB_NPH * nprob, sizeof(int));
+    if (!bt->hpofs || !bt->hpcnt) rc = HBICG_E_NOMEM;
+    bt->pofs = (int *)dalloc((size_t)HB_NPH * nprob * sizeof(int), &rc);
+    bt->pcnt = (int *)dalloc((size_t)HB_NPH * nprob * sizeof(int), &rc);
+    bt->act = (int *)dalloc(HB_NPH * sizeof(int), &rc);
+    bt->a_tab[HB_PH_SPMV] = (int *)dalloc(nsp * 2 * sizeof(int), &rc);
+    bt->a_tab[HB_PH_VEC] = (int *)dalloc(nvec * 2 * sizeof(int), &rc);
+    bt->a_tab[HB_PH_COEF] = (int *)dalloc(ncoef * 2 * sizeof(int), &rc);
     if (rc) goto fail;
-    HIPCK(hipHostMalloc((void **)&bt->flag_host, 2 * sizeof(int), hipHostMallocDefault));
+    {
+        int os = 0, ov = 0, oc = 0;
+        for (int q = 0; q < nprob; ++q) {
+            bt->hpofs[HB_PH_SPMV * nprob + q] = os;
+            bt->hpcnt[HB_PH_SPMV * nprob + q] = bt->plans[q]->nblk;
+            os += bt->plans[q]->nblk;
+            bt->hpofs[HB_PH_VEC * nprob + q] = ov;
+            bt->hpcnt[HB_PH_VEC * nprob + q] = bt->plans[q]->dev.vec_grid;
+            ov += bt->plans[q]->dev.vec_grid;
+            bt->hpofs[HB_PH_COEF * nprob + q] = oc;
+            bt->hpcnt[HB_PH_COEF * nprob + q] = M[q];
+            oc += M[q];
+        }
+    }
+    bt->B.a_tab[HB_PH_SPMV] = bt->a_tab[HB_PH_SPMV];
+    bt->B.a_tab[HB_PH_VEC] = bt->a_tab[HB_PH_VEC];
+    bt->B.a_tab[HB_PH_COEF] = bt->a_tab[HB_PH_COEF];
+    bt->B.act = bt->act;
+    bt->B.pofs = bt->pofs;
+    bt->B.pcnt = bt->pcnt;
+    bt->B.div = 1;
+    HIPCK(hipHostMalloc((void **)&bt->flag_host, (2 + HB_NPH) * sizeof(int), hipHostMallocDefault));
     HIPCK(hipEventCreateWithFlags(&bt->ev[0], hipEventDisableTiming));
     HIPCK(hipEventCreateWithFlags(&bt->ev[1], hipEventDisableTiming));
     HIPCK(hipEventCreate(&bt->t0));
@@ -1437,6 +1497,7 @@ fail:
 
 static int batch_enqueue_chunk(hbicg_batch *bt)
 {
+    if (hb_kb_compact(&bt->B, bt->stream) != 0) return HBICG_E_HIP;
     for (int c = 0; c < bt->chunk; ++c) {
         const int e = hb_kb_iteration(&bt->B, bt->stream);
         if (e) return e;
@@ -1444,12 +1505,19 @@ static int batch_enqueue_chunk(hbicg_batch *bt)
     return hb_kb_publish(&bt->B, bt->flag_host, bt->stream);
 }
 
-static int batch_capture(hbicg_batch *bt)
+/* the chunk graph of width level l (captured on first use) */
+static int batch_capture(hbicg_batch *bt, int l)
 {
     int rc = 0;
-    if (bt->gexec && !bt->graph_dirty) return 0;
-    if (bt->gexec) { hipGraphExecDestroy(bt->gexec); bt->gexec = NULL; }
-    if (bt->graph_obj) { hipGraphDestroy(bt->graph_obj); bt->graph_obj = NULL; }
+    if (bt->graph_dirty) {
+        for (int i = 0; i < HB_GLEVELS; ++i) {
+            if (bt->gexec[i]) { hipGraphExecDestroy(bt->gexec[i]); bt->gexec[i] = NULL; }
+            if (bt->graph_obj[i]) { hipGraphDestroy(bt->graph_obj[i]); bt->graph_obj[i] = NULL; }
+        }
+        bt->graph_dirty = 0;
+    }
+    if (bt->gexec[l]) return 0;
+    bt->B.div = 1 << l;
     HIPCK(hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
     if (batch_enqueue_chunk(bt) != 0) {
         hipGraph_t g = NULL;
@@ -1457,12 +1525,22 @@ static int batch_capture(hbicg_batch *bt)
         if (g) hipGraphDestroy(g);
         return HBICG_E_HIP;
     }
-    HIPCK(hipStreamEndCapture(bt->stream, &bt->graph_obj));
-    HIPCK(hipGraphInstantiate(&bt->gexec, bt->graph_obj, NULL, NULL, 0));
-    bt->graph_dirty = 0;
+    HIPCK(hipStreamEndCapture(bt->stream, &bt->graph_obj[l]));
+    HIPCK(hipGraphInstantiate(&bt->gexec[l], bt->graph_obj[l], NULL, NULL, 0));
     return 0;
 fail:
     return rc;
+}
+
+/* Launch width for the next chunk: the largest level whose phase launches
+ * still give every active SpMV tile of the last published count a
+ * workgroup of its own (the lists only shrink, so an older count errs
+ * wide). */
+static int batch_level(const hbicg_batch *bt, int active_spmv)
+{
+    int l = 0;
+    while (l + 1 < HB_GLEVELS && ((bt->B.n_spmv_tiles + (2 << l) - 1) >> (l + 1)) >= active_spmv) ++l;
+    return l;
 }
 
 /* Runs the batch with the current layout, max_iter_override > 0 capping
@@ -1491,15 +1569,18 @@ static int batch_run(hbicg_batch *bt, const double *const *b, const double *cons
                          bt->stream));
     HIPCK(hipEventRecord(bt->t0, bt->stream));
     if (hb_kb_init(&bt->B, bt->stream) != 0) return HBICG_E_HIP;
-    if (bt->graph && (rc = batch_capture(bt)) != 0) return rc;
     bt->flag_host[0] = 0;
+    bt->levels_used = 0;
     const int max_chunks = max_cap / bt->chunk + 3;
-    int launched = 0, prev = -1;
+    int launched = 0, prev = -1, level = 0;
     for (;;) {
+        bt->levels_used |= 1 << level;
         if (bt->graph) {
-            HIPCK(hipGraphLaunch(bt->gexec, bt->stream));
-        } else if (batch_enqueue_chunk(bt) != 0) {
-            return HBICG_E_HIP;
+            if ((rc = batch_capture(bt, level)) != 0) return rc;
+            HIPCK(hipGraphLaunch(bt->gexec[level], bt->stream));
+        } else {
+            bt->B.div = 1 << level;
+            if (batch_enqueue_chunk(bt) != 0) return HBICG_E_HIP;
         }
         const int cur = launched & 1;
         HIPCK(hipEventRecord(bt->ev[cur], bt->stream));
@@ -1507,6 +1588,7 @@ static int batch_run(hbicg_batch *bt, const double *const *b, const double *cons
         if (prev >= 0) {
             HIPCK(hipEventSynchronize(bt->ev[prev]));
             if (__atomic_load_n(&bt->flag_host[0], __ATOMIC_ACQUIRE)) break;
+            level = batch_level(bt, __atomic_load_n(&bt->flag_host[2 + HB_PH_SPMV], __ATOMIC_ACQUIRE));
         }
         prev = cur;
         if (launched > max_chunks) break;

@@ -2029,19 +2029,23 @@ __global__ __launch_bounds__(HB_BLOCK) void k_resid_final(const double *part, in
 }
 
 /* ---- batched launches --------------------------------------------------------
- * Launch tile t of a phase works on problem tab[2t] tile tab[2t+1]; finalize
- * launches use workgroup p for problem p.  Problems that are done return at
- * once inside the bodies. */
+ * A phase launch walks its active list grid-stride: entry t works on
+ * problem tab[2t], tile tab[2t+1], for t < *cnt (the list holds the tiles
+ * of the problems still running at the chunk's start, kb_compact).
+ * Finalize launches use workgroup p for problem p.  Problems that finish
+ * inside a chunk return at once inside the bodies. */
 template <int MODE, bool EXACT, bool VI = false>
 __global__ __launch_bounds__(HB_BLOCK) void kb_spmv(const hb_dev *__restrict__ devs,
-                                                    const int *__restrict__ tab)
+                                                    const int *__restrict__ tab, const int *__restrict__ cnt)
 {
     /* hardware order: problems finish at different iterations, and the
      * tiles still working must stay spread over all XCDs (an XCD-contiguous
      * order measured 14% slower on the 256-problem batch) */
-    const unsigned t = blockIdx.x;
-    const hb_dev &d = devs[tab[2 * t]];
-    spmv_tile<MODE, EXACT, VI>(d, tab[2 * t + 1]);
+    const int nt = *cnt;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const hb_dev &d = devs[tab[2 * t]];
+        spmv_tile<MODE, EXACT, VI>(d, tab[2 * t + 1]);
+    }
 }
 
 template <int MODE, bool EXACT>
@@ -2052,10 +2056,11 @@ __global__ __launch_bounds__(HB_BLOCK) void kb_fin_spmv(const hb_dev *__restrict
 }
 
 __global__ __launch_bounds__(HB_BLOCK) void kb_seed_update(const hb_dev *__restrict__ devs,
-                                                           const int *__restrict__ tab)
+                                                           const int *__restrict__ tab,
+                                                           const int *__restrict__ cnt)
 {
-    const hb_dev &d = devs[tab[2 * blockIdx.x]];
-    seed_update_tile(d, tab[2 * blockIdx.x + 1]);
+    const int nt = *cnt;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) seed_update_tile(devs[tab[2 * t]], tab[2 * t + 1]);
 }
 
 template <bool EXACT>
@@ -2068,27 +2073,96 @@ __global__ __launch_bounds__(HB_BLOCK) void kb_fin_seed(const hb_dev *__restrict
 /* shift tiles: tab[2t+1] = by * gx + bx of the problem's (gx, gy) grid */
 template <int WM, int V, int U>
 __global__ __launch_bounds__(HB_BLOCK) void kb_shift(const hb_dev *__restrict__ devs,
-                                                     const int *__restrict__ tab)
+                                                     const int *__restrict__ tab, const int *__restrict__ cnt)
 {
-    const hb_dev &d = devs[tab[2 * blockIdx.x]];
-    const int t = tab[2 * blockIdx.x + 1];
-    shift_tile<WM, V, U>(d, t % d.shift_gx, t / d.shift_gx);
+    const int nt = *cnt;
+    for (int e = blockIdx.x; e < nt; e += gridDim.x) {
+        const hb_dev &d = devs[tab[2 * e]];
+        const int t = tab[2 * e + 1];
+        shift_tile<WM, V, U>(d, t % d.shift_gx, t / d.shift_gx);
+    }
 }
 
 /* combined-coefficient flush: tab[2t+1] = list position */
 __global__ __launch_bounds__(HB_BLOCK) void kb_flush_coef(const hb_dev *__restrict__ devs,
-                                                          const int *__restrict__ tab)
+                                                          const int *__restrict__ tab,
+                                                          const int *__restrict__ cnt)
 {
-    const hb_dev &d = devs[tab[2 * blockIdx.x]];
-    flush_coef_body(d, tab[2 * blockIdx.x + 1], d.coef, d.ccoef);
+    const int nt = *cnt;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const hb_dev &d = devs[tab[2 * t]];
+        flush_coef_body(d, tab[2 * t + 1], d.coef, d.ccoef);
+    }
 }
 
 /* tab[2t+1] = 256-element tile (the problem's shift grid has one row) */
 __global__ __launch_bounds__(HB_BLOCK) void kb_shift_cc(const hb_dev *__restrict__ devs,
-                                                        const int *__restrict__ tab)
+                                                        const int *__restrict__ tab, const int *__restrict__ cnt)
 {
-    const hb_dev &d = devs[tab[2 * blockIdx.x]];
-    shift_cc_tile(d, tab[2 * blockIdx.x + 1]);
+    const int nt = *cnt;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) shift_cc_tile(devs[tab[2 * t]], tab[2 * t + 1]);
+}
+
+/* Active sums over the problems still running: workgroup-wide, every
+ * thread gets the sums of problems before `p` (pre) and of all (tot). */
+DEV void active_sums(const hb_batch &B, int p, int pre[HB_NPH], int tot[HB_NPH], bool &p_active)
+{
+    __shared__ int sh[2 * HB_NPH][HB_BLOCK / 64];
+    __shared__ int sh_act;
+    const int tid = threadIdx.x;
+    int a[2 * HB_NPH] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (tid == 0) sh_act = 0;
+    __syncthreads();
+    for (int q = tid; q < B.nprob; q += HB_BLOCK) {
+        if (B.devs[q].st->done) continue;
+        if (q == p) sh_act = 1;
+#pragma unroll
+        for (int ph = 0; ph < HB_NPH; ++ph) {
+            const int c = B.pcnt[ph * B.nprob + q];
+            a[ph] += q < p ? c : 0;
+            a[HB_NPH + ph] += c;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * HB_NPH; ++i) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a[i] += __shfl_xor(a[i], o, 64);
+        if ((tid & 63) == 0) sh[i][tid >> 6] = a[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < HB_NPH; ++i) {
+        pre[i] = 0;
+        tot[i] = 0;
+#pragma unroll
+        for (int w = 0; w < HB_BLOCK / 64; ++w) {
+            pre[i] += sh[i][w];
+            tot[i] += sh[HB_NPH + i][w];
+        }
+    }
+    p_active = sh_act != 0;
+    __syncthreads();
+}
+
+/* Workgroup p copies problem p's entries of every table into the active
+ * lists, after those of the running problems before it; workgroup 0 stores
+ * the lists' lengths.  The done flags do not change during the launch. */
+__global__ __launch_bounds__(HB_BLOCK) void kb_compact(hb_batch B)
+{
+    const int p = blockIdx.x;
+    int pre[HB_NPH], tot[HB_NPH];
+    bool act;
+    active_sums(B, p, pre, tot, act);
+    if (p == 0 && threadIdx.x < HB_NPH) B.act[threadIdx.x] = tot[threadIdx.x];
+    if (!act) return;
+    const int *full[HB_NPH] = {B.t_spmv, B.t_vec, B.t_shift, B.t_coef};
+#pragma unroll
+    for (int ph = 0; ph < HB_NPH; ++ph) {
+        const int *src = full[ph] + 2 * (size_t)B.pofs[ph * B.nprob + p];
+        int *dst = B.a_tab[ph] + 2 * (size_t)pre[ph];
+        const int m = 2 * B.pcnt[ph * B.nprob + p];
+        for (int i = threadIdx.x; i < m; i += HB_BLOCK) dst[i] = src[i];
+    }
 }
 
 __global__ __launch_bounds__(HB_BLOCK) void kb_init(const hb_dev *__restrict__ devs,
@@ -2112,22 +2186,26 @@ __global__ __launch_bounds__(HB_BLOCK) void kb_finalize(const hb_dev *__restrict
     finalize_body(d, tab[2 * blockIdx.x + 1]);
 }
 
-/* flag[0] = 1 once every problem is done; flag[1] = problems still running */
-__global__ __launch_bounds__(HB_BLOCK) void kb_publish(const hb_dev *__restrict__ devs, int nprob,
-                                                       int *flag)
+/* flag[0] = 1 once every problem is done; flag[1] = problems still running;
+ * flag[2 + ph] = entries of phase ph's table those problems hold */
+__global__ __launch_bounds__(HB_BLOCK) void kb_publish(hb_batch B, int *flag)
 {
     __shared__ int cnt[HB_BLOCK];
     int c = 0;
-    for (int p = threadIdx.x; p < nprob; p += HB_BLOCK) c += devs[p].st->done ? 0 : 1;
+    for (int q = threadIdx.x; q < B.nprob; q += HB_BLOCK) c += B.devs[q].st->done ? 0 : 1;
     cnt[threadIdx.x] = c;
     __syncthreads();
     for (int w = HB_BLOCK / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) cnt[threadIdx.x] += cnt[threadIdx.x + w];
         __syncthreads();
     }
+    int pre[HB_NPH], tot[HB_NPH];
+    bool act;
+    active_sums(B, 0, pre, tot, act);
     if (threadIdx.x == 0) {
         flag[0] = cnt[0] == 0;
         flag[1] = cnt[0];
+        for (int ph = 0; ph < HB_NPH; ++ph) flag[2 + ph] = tot[ph];
         __threadfence_system();
     }
 }
@@ -2154,12 +2232,20 @@ static void launch_shift(const hb_dev *d, hipStream_t s)
     hipLaunchKernelGGL((k_shift<WM, V, U>), g, b, 0, s, *d);
 }
 
+/* workgroups of a phase launch over a table of `full` entries */
+static unsigned kb_grid(const hb_batch *B, int full)
+{
+    const int d = B->div > 1 ? B->div : 1;
+    const int g = (full + d - 1) / d;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
 template <int WM>
 static void launch_shift_b(const hb_batch *B, hipStream_t s)
 {
     constexpr int U = WM >= 8 ? 2 : 4;
-    hipLaunchKernelGGL((kb_shift<WM, 1, U>), dim3(B->n_shift_tiles), dim3(HB_BLOCK), 0, s,
-                       B->devs, B->t_shift);
+    hipLaunchKernelGGL((kb_shift<WM, 1, U>), dim3(kb_grid(B, B->n_shift_tiles)), dim3(HB_BLOCK), 0, s,
+                       B->devs, B->a_tab[HB_PH_SHIFT], B->act + HB_PH_SHIFT);
 }
 
 extern "C" {
@@ -2364,37 +2450,31 @@ int hb_kb_init(const hb_batch *B, hipStream_t s)
 int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s)
 {
     const dim3 bl(HB_BLOCK), gp(B->nprob);
+    const dim3 gs(kb_grid(B, B->n_spmv_tiles)), gv(kb_grid(B, B->n_vec_tiles));
+    const int *ts = B->a_tab[HB_PH_SPMV], *cs = B->act + HB_PH_SPMV;
     switch (stage) {
     case HB_STAGE_SPMV1:
-        if (B->exact && B->vi)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED1, true, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-        else if (B->exact)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED1, true, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-        else if (B->vi)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED1, false, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-        else
-            hipLaunchKernelGGL((kb_spmv<SP_SEED1, false, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        if (B->exact && B->vi) hipLaunchKernelGGL((kb_spmv<SP_SEED1, true, true>), gs, bl, 0, s, B->devs, ts, cs);
+        else if (B->exact) hipLaunchKernelGGL((kb_spmv<SP_SEED1, true, false>), gs, bl, 0, s, B->devs, ts, cs);
+        else if (B->vi) hipLaunchKernelGGL((kb_spmv<SP_SEED1, false, true>), gs, bl, 0, s, B->devs, ts, cs);
+        else hipLaunchKernelGGL((kb_spmv<SP_SEED1, false, false>), gs, bl, 0, s, B->devs, ts, cs);
         break;
     case HB_STAGE_FIN1:
         if (B->exact) hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, true>), gp, bl, 0, s, B->devs);
         else hipLaunchKernelGGL((kb_fin_spmv<SP_SEED1, false>), gp, bl, 0, s, B->devs);
         break;
     case HB_STAGE_SPMV2:
-        if (B->exact && B->vi)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED2, true, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-        else if (B->exact)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED2, true, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-        else if (B->vi)
-            hipLaunchKernelGGL((kb_spmv<SP_SEED2, false, true>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
-        else
-            hipLaunchKernelGGL((kb_spmv<SP_SEED2, false, false>), dim3(B->n_spmv_tiles), bl, 0, s, B->devs, B->t_spmv);
+        if (B->exact && B->vi) hipLaunchKernelGGL((kb_spmv<SP_SEED2, true, true>), gs, bl, 0, s, B->devs, ts, cs);
+        else if (B->exact) hipLaunchKernelGGL((kb_spmv<SP_SEED2, true, false>), gs, bl, 0, s, B->devs, ts, cs);
+        else if (B->vi) hipLaunchKernelGGL((kb_spmv<SP_SEED2, false, true>), gs, bl, 0, s, B->devs, ts, cs);
+        else hipLaunchKernelGGL((kb_spmv<SP_SEED2, false, false>), gs, bl, 0, s, B->devs, ts, cs);
         break;
     case HB_STAGE_FIN2:
         if (B->exact) hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, true>), gp, bl, 0, s, B->devs);
         else hipLaunchKernelGGL((kb_fin_spmv<SP_SEED2, false>), gp, bl, 0, s, B->devs);
         break;
     case HB_STAGE_SEED:
-        hipLaunchKernelGGL(kb_seed_update, dim3(B->n_vec_tiles), bl, 0, s, B->devs, B->t_vec);
+        hipLaunchKernelGGL(kb_seed_update, gv, bl, 0, s, B->devs, B->a_tab[HB_PH_VEC], B->act + HB_PH_VEC);
         break;
     case HB_STAGE_SCALAR:
         if (B->exact) hipLaunchKernelGGL(kb_fin_seed<true>, gp, bl, 0, s, B->devs);
@@ -2402,11 +2482,13 @@ int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s)
         break;
     case HB_STAGE_COEF:
         if (!B->exact)
-            hipLaunchKernelGGL(kb_flush_coef, dim3(B->n_coef_tiles), bl, 0, s, B->devs, B->t_coef);
+            hipLaunchKernelGGL(kb_flush_coef, dim3(kb_grid(B, B->n_coef_tiles)), bl, 0, s, B->devs,
+                               B->a_tab[HB_PH_COEF], B->act + HB_PH_COEF);
         break;
     default:
         if (!B->exact) {
-            hipLaunchKernelGGL(kb_shift_cc, dim3(B->n_shift_tiles), bl, 0, s, B->devs, B->t_shift);
+            hipLaunchKernelGGL(kb_shift_cc, dim3(kb_grid(B, B->n_shift_tiles)), bl, 0, s, B->devs,
+                               B->a_tab[HB_PH_SHIFT], B->act + HB_PH_SHIFT);
             break;
         }
         if (B->W <= 1) launch_shift_b<1>(B, s);
@@ -2429,7 +2511,13 @@ int hb_kb_iteration(const hb_batch *B, hipStream_t s)
 
 int hb_kb_publish(const hb_batch *B, int *flag_dev, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_publish, dim3(1), dim3(HB_BLOCK), 0, s, B->devs, B->nprob, flag_dev);
+    hipLaunchKernelGGL(kb_publish, dim3(1), dim3(HB_BLOCK), 0, s, *B, flag_dev);
+    return (int)hipGetLastError();
+}
+
+int hb_kb_compact(const hb_batch *B, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_compact, dim3(B->nprob), dim3(HB_BLOCK), 0, s, *B);
     return (int)hipGetLastError();
 }
 

@@ -65,6 +65,7 @@ int hb_kb_init(const hb_batch *B, hipStream_t s);
 int hb_kb_stage(const hb_batch *B, int stage, hipStream_t s);
 int hb_kb_iteration(const hb_batch *B, hipStream_t s);
 int hb_kb_publish(const hb_batch *B, int *flag_dev, hipStream_t s);
+int hb_kb_compact(const hb_batch *B, hipStream_t s);
 int hb_kb_finalize(const hb_batch *B, hipStream_t s);
 
 #ifdef __cplusplus

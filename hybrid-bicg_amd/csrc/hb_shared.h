@@ -180,6 +180,9 @@ typedef struct hb_dev {
 /* A batch of independent problems: per-problem hb_dev records in device
  * memory and, per phase, a table of (problem, tile) pairs so that one
  * launch covers the tiles of every problem. */
+/* the phases whose launches run over (problem, tile) tables */
+enum { HB_PH_SPMV = 0, HB_PH_VEC = 1, HB_PH_SHIFT = 2, HB_PH_COEF = 3, HB_NPH = 4 };
+
 typedef struct hb_batch {
     int nprob, exact, W;
     int vi;                    /* 1: every problem's SpMV reads its value table  */
@@ -189,6 +192,14 @@ typedef struct hb_batch {
     const int *t_vec;          /* [2*n_vec_tiles]   (problem, vector tile) */
     const int *t_shift;        /* [2*n_shift_tiles] (problem, by*gx + bx)  */
     const int *t_coef;         /* [2*n_coef_tiles]  (problem, list position) */
+    /* active lists: each table restricted to the problems still running,
+     * rebuilt on the device at the start of every chunk (kb_compact); the
+     * phase launches walk them grid-stride */
+    int *a_tab[HB_NPH];        /* [2 * entries of the full table]          */
+    int *act;                  /* [HB_NPH] entries in each active list     */
+    const int *pofs;           /* [HB_NPH][nprob] problem p's first entry in each full table */
+    const int *pcnt;           /* [HB_NPH][nprob] and its entries          */
+    int div;                   /* the phase launches take ceil(full / div) workgroups */
 } hb_batch;
 
 #ifdef __cplusplus
