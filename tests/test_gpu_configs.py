@@ -4,9 +4,18 @@ Batched: 256 independent problems, n log-uniform in [1k, 200k] (2-D 5-point
 and 3-D 7-point convection-diffusion, convection uniform in [0, 0.4]), 8 to
 64 shifts each log-spaced in [1e-3, 1], tolerance 1e-10, in one batch.
 Every problem converges, every shift's true residual (the checker that the
-full-size tests hold against the oracle's) is within 10x the stopping rule,
+full-size tests hold against the oracle's) is within 100x the stopping rule,
 and every problem equals a plain plan of its own run with the batch's
 window bit for bit: k, stop iterations, switch log and x.
+
+Why 100x: BiCG's true residual can drift from the recursive estimate that
+stops it (the residual gap), by an amount set by the rounding trajectory.
+The oracle (the reference's arithmetic) on this batch ends with true
+residuals of 5.5e-9 (problem 72) and 1.9e-9 (problem 206) on one rank, and
+2.1e-9 (problem 86) on 8 ranks, where one rank gave under 1e-10; which
+problems land there moves with the summation order.  The default order's
+trajectories here stay under 6.7e-10; a different one (four rows per thread
+in the seed update) ended problem 80 at 2.9e-9.
 
 Differentiable: 3-D 7-point convection-diffusion on 64^3 (n = 262,144),
 32 shifts log-spaced in [1e-2, 1], L = sum_k w_k c^T x_k with w, c uniform
@@ -41,7 +50,7 @@ def test_batched_256_problems():
         assert rep["stop_count"] == p.M, (i, p.name)
         tr = bt.true_residuals(i, xs[i])
         worst = max(worst, float(tr.max()))
-        assert tr.max() <= 10 * p.tol, (i, p.name, tr.max())
+        assert tr.max() <= 100 * p.tol, (i, p.name, tr.max())
         W = rep["window"] if W is None else W
         assert rep["window"] == W
     print(f"  batch {ms:.1f} ms, window {W}, max true residual {worst:.2e}", flush=True)
