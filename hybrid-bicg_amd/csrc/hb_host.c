@@ -1042,6 +1042,17 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
             K->bytes = per_launch[sg] * iters;
             K->flops = 0.0;
         }
+        /* stages that run inside a neighbouring launch (fused): no launches
+         * of their own; their bytes go to the launch that runs them */
+        static const int host_of[HB_NSTAGE] = {-1, HB_STAGE_SPMV2, -1, HB_STAGE_SEED, -1, -1, HB_STAGE_SHIFT, -1};
+        for (int sg = 0; sg < HB_NSTAGE; ++sg) {
+            if (hb_k_stage_runs(&pl->dev, sg)) continue;
+            hbicg_kernel_stat *K = &pl->kstat[sg];
+            if (host_of[sg] >= 0) pl->kstat[host_of[sg]].bytes += K->bytes;
+            K->launches = 0;
+            K->total_ms = 0.0;
+            K->bytes = 0.0;
+        }
         /* the shift kernel's launches split in two: [7] the seed p update
          * alone (p, r, s in, p out), [8] the launches that also ran a window
          * flush (the device-counted rest of the bytes, and its multiply-adds) */

@@ -1607,9 +1607,17 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
     const HB_K int *fl = read_only(flist), *al = read_only((const int *)d.act_len);
     const size_t cstride = (size_t)HB_CC_ROWS(W) * (2 * UG);
     for (int sub = 0; sub < HB_BLOCK / 64; ++sub) {
-        const int i = bx * HB_BLOCK + sub * 64 + lane;
-        if (i >= n) break;
-        for (int c = wave; c < nch; c += HB_BLOCK / 64) {
+        if (bx * HB_BLOCK + sub * 64 >= n) break;   /* the tile ends: uniform */
+        /* the wavefronts meet before each sub-tile, so none runs ahead into
+         * elements whose window vectors the others would fetch again */
+        if (sub) __syncthreads();
+        const int ir = bx * HB_BLOCK + sub * 64 + lane;
+        const bool has = ir < n;
+        const int i = has ? ir : n - 1;       /* a clamped duplicate past the end */
+        /* chunk residue rotated by tile: the list runs from the slowest
+         * shifts (the longest loops) to the fastest, so the long chunks land
+         * on different wavefronts, and SIMDs, from tile to tile */
+        for (int c = (wave + bx) % (HB_BLOCK / 64); c < nch; c += HB_BLOCK / 64) {
             const HB_K double *cb = cc + (size_t)c * cstride;
             const int nu = min(UG, cnt - c * UG);
             /* a shift's coefficients of window steps after the last one it
@@ -1688,7 +1696,7 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
             }
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
-                if (u < nu) {
+                if (u < nu && has) {
                     const int j = fl[c * UG + u];
                     __builtin_nontemporal_store(x[u], xset + off(j, n) + i);
                     /* the seed keeps its own p_s (the regrouped one is
@@ -1721,11 +1729,20 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
     const HB_K int *fl = read_only(flist), *al = read_only((const int *)d.act_len);
     const size_t cstride = (size_t)HB_CC_ROWS(W) * (2 * UG);
     for (int sub = 0; sub < HB_BLOCK / 64; sub += 2) {
-        const int i0 = bx * HB_BLOCK + sub * 64 + lane;
-        if (i0 >= n) break;
-        const bool has1 = i0 + 64 < n;
-        const int i1 = has1 ? i0 + 64 : i0;   /* a clamped duplicate when past the end */
-        for (int c = wave; c < nch; c += HB_BLOCK / 64) {
+        if (bx * HB_BLOCK + sub * 64 >= n) break;   /* the tile ends: uniform */
+        /* the four wavefronts walk the same elements through the window in
+         * step; they meet at a barrier before the next pair of sub-tiles, so
+         * the ones whose chunks end early do not run ahead into elements
+         * whose window vectors the others would fetch again */
+        if (sub) __syncthreads();
+        const int ir = bx * HB_BLOCK + sub * 64 + lane;
+        const bool has0 = ir < n, has1 = ir + 64 < n;
+        const int i0 = has0 ? ir : n - 1;     /* clamped duplicates past the end */
+        const int i1 = has1 ? ir + 64 : i0;
+        /* chunk residue rotated by tile: the list runs from the slowest
+         * shifts (the longest loops) to the fastest, so the long chunks land
+         * on different wavefronts, and SIMDs, from tile to tile */
+        for (int c = (wave + bx) % (HB_BLOCK / 64); c < nch; c += HB_BLOCK / 64) {
             const HB_K double *cb = cc + (size_t)c * cstride;
             const int nu = min(UG, cnt - c * UG);
             /* a shift's coefficients of window steps after the last one it
@@ -1785,7 +1802,7 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, cons
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
                     const int j = fl[c * UG + u];
-                    __builtin_nontemporal_store(x0[u], xset + off(j, n) + i0);
+                    if (has0) __builtin_nontemporal_store(x0[u], xset + off(j, n) + i0);
                     if (has1) __builtin_nontemporal_store(x1[u], xset + off(j, n) + i1);
                 }
             }
@@ -2379,6 +2396,15 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
         break;
     }
     return (int)hipGetLastError();
+}
+
+int hb_k_stage_runs(const hb_dev *d, int stage)
+{
+    if (stage == HB_STAGE_COEF) return !d->exact && !d->fused;
+    if (d->exact || !d->fused) return 1;
+    if (stage == HB_STAGE_FIN2) return 0;
+    if (stage == HB_STAGE_FIN1) return !d->fused_a;
+    return 1;
 }
 
 int hb_k_iteration(const hb_dev *d, hipStream_t s)

@@ -21,9 +21,14 @@ int hb_k_init(const hb_dev *d, hipStream_t s);
 /* One iteration: seed SpMV 1, fused q + SpMV 2, seed update, each with its
  * one-workgroup finalize (the last one is the scalar stage: shift
  * recurrences, convergence check, seed switch), then the window flush
- * (combined coefficients and the shift kernel).  Each kernel returns at once
- * when the solve is done. */
+ * (combined coefficients and the shift kernel).  With fused launches the
+ * alpha~ and omega~ finalizes and the combined coefficients run as the
+ * first workgroups of SpMV 2, the seed update and the shift kernel.  Each
+ * kernel returns at once when the solve is done. */
 int hb_k_iteration(const hb_dev *d, hipStream_t s);
+/* 1 if hb_k_stage launches a kernel for `stage` (fused launches run some
+ * stages inside a neighbouring one) */
+int hb_k_stage_runs(const hb_dev *d, int stage);
 
 /* The kernels of an iteration, one launch at a time (per-kernel timing). */
 #define HB_STAGE_SPMV1  0   /* SpMV s = (A + sigma_s I) p_s, (r^, s) partials   */
