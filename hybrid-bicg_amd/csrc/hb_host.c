@@ -41,7 +41,7 @@ void hbicg_default_params(hbicg_params *p)
     p->exit_on_error = 1;
     p->upload_x = 1;
     p->profile = 0;
-    p->growth_limit = 0.0;
+    p->growth_limit = 1e8;
     p->trace = 0;
     p->value_index = 1;
     p->q_mode = 0;

@@ -19,7 +19,7 @@ VARIANTS = ("normal", "static", "dynamic")
 
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
                 device=0, exit_on_error=False, upload_x=True, profile=False,
-                growth_limit=0.0, trace=False, value_index=True, q_mode=0, fused=True,
+                growth_limit=1e8, trace=False, value_index=True, q_mode=0, fused=True,
                 panel_cols=0) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)

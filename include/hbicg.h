@@ -63,8 +63,9 @@ typedef struct hbicg_params {
     int    profile;      /* 1: time every kernel with HIP events (direct launches)       */
     double growth_limit; /* default mode, > 0: a window closes early, its last iteration
                             applied on its own, when the size bound of a shift's regrouped
-                            p exceeds this multiple of one iteration's own terms (0: off;
-                            the regrouped coefficients are exact to double precision)    */
+                            p exceeds this multiple of one iteration's own terms, which
+                            bounds what regrouping can add to x's rounding error at about
+                            this multiple of the unit roundoff.  Default 1e8; 0: off    */
     int    trace;        /* 1: record the per-iteration residual estimate of every shift  */
     int    value_index;  /* 1 (default): a matrix with at most 256 distinct values is read
                             by the SpMV as one byte per entry into a table of its values,
