@@ -106,10 +106,11 @@ def parse():
     ap.add_argument("--value-index", type=int, default=1, choices=[0, 1],
                     help="1 (library default): SpMV reads the matrix through its value table "
                          "(and diagonal tiles) where it qualifies; 0: 8-byte CSR values")
-    ap.add_argument("--fused", type=int, default=1, choices=[0, 1, 2],
-                    help="1 (library default): five launches per iteration (finalizes and combined "
-                         "coefficients run inside the neighbouring launches); 2: six (alpha~ "
-                         "finalized on its own); 0: eight")
+    ap.add_argument("--fused", type=int, default=1, choices=[0, 1, 2, 3],
+                    help="1 (library default): four launches per iteration (the finalizes, the "
+                         "scalar stage and the combined coefficients run inside the neighbouring "
+                         "launches); 3: five (the scalar stage on its own); 2: six (alpha~ "
+                         "finalized on its own too); 0: eight")
     ap.add_argument("--cpu-iters", type=int, default=2000,
                     help="iteration cap of the timed oracle solve (the workload's own cap, 2000, "
                          "times the complete solve; 0: skip)")

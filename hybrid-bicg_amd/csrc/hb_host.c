@@ -768,7 +768,14 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
 
     /* alpha~'s finalize joins SpMV 2 where that launch forms q at its
      * gathered columns in one pass */
-    d->fused_a = d->fused && !d->q_unfused && d->npanel <= 1 && p->fused == 1;
+    d->fused_a = d->fused && !d->q_unfused && d->npanel <= 1 && (p->fused == 1 || p->fused == 3);
+    /* the scalar stage joins the shift kernel (level 1), whose tiles then
+     * walk the seed p update and the flush tiles grid-stride */
+    d->fused_s = d->fused && p->fused == 1;
+    if (d->fused_s) {
+        d->shift_tw = hb_k_shift_stage_tiles(d->shift_gx, M, pl->device);
+        if (d->shift_tw < 0) { rc = HBICG_E_HIP; goto fail; }
+    }
     if (a_on_device) free(hptr);
     free(hblk);
     return pl;
@@ -970,7 +977,11 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     if (pl->flush_log) HIPCK(hipMemsetAsync(pl->flush_log, 0, (size_t)pl->cap * sizeof(int), pl->stream));
     if (hb_k_init(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
     if ((rc = run_loop(pl)) != 0) goto fail;
+    /* profiled solves time the finalize: it holds the last window flush
+     * when the shift kernel runs the scalar stage */
+    if (pl->profile) HIPCK(hipEventRecord(pl->pev[0], pl->stream));
     if (hb_k_finalize(&pl->dev, pl->stream) != 0) { rc = HBICG_E_HIP; goto fail; }
+    if (pl->profile) HIPCK(hipEventRecord(pl->pev[1], pl->stream));
     HIPCK(hipEventRecord(pl->t1, pl->stream));
 
     hb_state st;
@@ -1044,7 +1055,8 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
         }
         /* stages that run inside a neighbouring launch (fused): no launches
          * of their own; their bytes go to the launch that runs them */
-        static const int host_of[HB_NSTAGE] = {-1, HB_STAGE_SPMV2, -1, HB_STAGE_SEED, -1, -1, HB_STAGE_SHIFT, -1};
+        static const int host_of[HB_NSTAGE] = {-1, HB_STAGE_SPMV2, -1, HB_STAGE_SEED, -1, HB_STAGE_SHIFT,
+                                               HB_STAGE_SHIFT, -1};
         for (int sg = 0; sg < HB_NSTAGE; ++sg) {
             if (hb_k_stage_runs(&pl->dev, sg)) continue;
             hbicg_kernel_stat *K = &pl->kstat[sg];
@@ -1070,6 +1082,11 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
                 Kp->launches += 1;
                 Kp->total_ms += t;
             }
+        }
+        if (pl->dev.fused_s && Kf->launches > 0) {
+            float fin = 0.0f;
+            HIPCK(hipEventElapsedTime(&fin, pl->pev[0], pl->pev[1]));
+            Kf->total_ms += fin;
         }
         Kp->bytes = 32.0 * n * Kp->launches;
         Kf->bytes = st.shift_bytes - Kp->bytes;

@@ -97,12 +97,15 @@ DEV void block_sum2(double &a, double &b)
  * the workgroup sum.  32 16-byte pairs are in flight per thread before any
  * is added (the combining workgroup is alone on its CU, so registers are
  * free), so 8192 partials cost one memory latency, not four. */
+template <bool DEEP = true>
 DEV void reduce_partials(const double *part, int nb, double &a, double &b)
 {
     const dbl2 *p2 = (const dbl2 *)part;
     double sa = 0.0, sb = 0.0;
     int i = threadIdx.x;
-    for (; i + 31 * HB_BLOCK < nb; i += 32 * HB_BLOCK) {
+    /* DEEP: 32 loads in flight per thread (128 registers).  Every depth
+     * adds a thread's pairs in ascending order: the sums are the same */
+    for (; DEEP && i + 31 * HB_BLOCK < nb; i += 32 * HB_BLOCK) {
         dbl2 v[32];
 #pragma unroll
         for (int u = 0; u < 32; ++u) v[u] = p2[i + u * HB_BLOCK];
@@ -172,8 +175,9 @@ DEV void mail_put_d(unsigned long long *slot, double v)
     mail_put(slot, (unsigned long long)__double_as_longlong(v));
 }
 
-/* the published double, polled by the calling thread alone */
-DEV double mail_poll(unsigned long long *slot, hb_state *st)
+/* the published word, polled by the calling thread alone (HB_MAIL_EMPTY
+ * after a time-out) */
+DEV unsigned long long mail_poll_bits(unsigned long long *slot, hb_state *st)
 {
     unsigned long long v;
     for (int spins = 0;; ++spins) {
@@ -188,7 +192,12 @@ DEV double mail_poll(unsigned long long *slot, hb_state *st)
          * system more than they do */
         __builtin_amdgcn_s_sleep(8);
     }
-    return __longlong_as_double((long long)v);
+    return v;
+}
+
+DEV double mail_poll(unsigned long long *slot, hb_state *st)
+{
+    return __longlong_as_double((long long)mail_poll_bits(slot, st));
 }
 
 /* the published double; thread 0 of the workgroup waits, every thread gets it */
@@ -200,12 +209,34 @@ DEV double mail_wait(unsigned long long *slot, hb_state *st)
     return sh_v;
 }
 
-/* a double the kernel's other workgroups read before the launch ends */
+/* the published word, to every thread of the workgroup */
+DEV unsigned long long mail_wait_bits(unsigned long long *slot, hb_state *st)
+{
+    __shared__ unsigned long long sh_u;
+    if (threadIdx.x == 0) sh_u = mail_poll_bits(slot, st);
+    __syncthreads();
+    return sh_u;
+}
+
+/* a value the kernel's other workgroups read before the launch ends */
 template <bool PUB>
 DEV void store_pub(double *p, double v)
 {
     if (PUB) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else *p = v;
+}
+
+template <bool PUB>
+DEV void store_pub(int *p, int v)
+{
+    if (PUB) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+
+/* a word another workgroup of the launch stored with store_pub<true> */
+DEV int load_pub(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 DEV size_t off(int j, int n) { return (size_t)j * (size_t)n; }
@@ -269,7 +300,14 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         const int sw = MODE == SP_SEED1 ? st->switched : 0;
         const double zs = MODE == SP_SEED1 ? st->zscale : 1.0;
         const int wk0 = MODE == SP_SEED1 ? st->win_k0 : 0;
-        if (MODE == SP_SEED1 && b == 0 && threadIdx.x == 0) st->k4_epoch = k;
+        if (MODE == SP_SEED1 && b == 0 && threadIdx.x == 0) {
+            st->k4_epoch = k;
+            /* the shift kernel's mailboxes and coefficient count: its
+             * readers finished with the last launch of it */
+            st->mail_b = HB_MAIL_EMPTY;
+            st->mail_f = HB_MAIL_EMPTY;
+            st->coef_ready = 0;
+        }
         /* k >= 1, seed >= 0 and switched >= 0 always hold; testing them
          * keeps their loads ahead of the branch, in one round trip with done */
         if (done | (k < 1) | (seed < 0) | (sw < 0)) return;
@@ -827,6 +865,9 @@ DEV void fin_spmv_body(const hb_dev &d)
             const double omega = a0 / a1;
             st->qTq = a0;
             st->qTy = a1;
+            st->pu_w = omega;
+            st->pu_k = k;
+            st->pu_seed = st->seed;
             d.wa[k] = omega;
             st->rTr_old = rTr;
             if (a1 == 0.0 || !isfinite(omega)) st->status |= HB_ST_BREAKDOWN_QTY;
@@ -861,14 +902,22 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv2_fa(hb_dev d)
 
 /* ---- scalar stage (one workgroup per problem, after the seed update) -------
  * src/solver_dynamic.c:204 (beta~), :243-253 (per-shift recurrences),
- * :278-303 (convergence check), :312-341 (seed switch). */
+ * :278-303 (convergence check), :312-341 (seed switch).
+ * PUB: the stage runs as the first workgroup of the shift kernel.  It hands
+ * beta~ to the seed p update at once; at its end it stores again, with
+ * write-through stores, what the launch's flush reads (this iteration's
+ * coefficients and update counts, the list and the window's extent), then
+ * publishes the window decision.  (Atomic stores inside the stage's body
+ * would cost it about 30 registers: the compiler stops keeping its uniform
+ * loads in scalar registers.) */
+template <bool PUB = false>
 DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
 {
     __shared__ double sh_beta;
     __shared__ double sh_zp[HB_BLOCK];
     __shared__ int sh_j[HB_BLOCK];
     __shared__ int sh_nc[HB_BLOCK];
-    __shared__ int sh_switch, sh_spike, sh_flush, sh_bad, sh_upd;
+    __shared__ int sh_switch, sh_spike, sh_flush, sh_xonly, sh_bad, sh_upd, sh_cnt;
     __shared__ double sh_rprev;
     hb_state *st = d.st;
     const int tid = threadIdx.x;
@@ -881,6 +930,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
 
     if (tid == 0) {
         const double beta = (ak / wk) * (rTr / rTr_old);
+        if (PUB) mail_put_d(&st->mail_b, beta);
         sh_rprev = st->dot_r;
         sh_spike = 0;
         sh_bad = 0;
@@ -940,6 +990,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             }
             st->win_seedpos = seedpos;
             st->flush_cnt = base;
+            sh_cnt = base;
         }
     }
 
@@ -1121,6 +1172,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             st->win_k0 = k + 1;
         }
         sh_flush = fl;
+        sh_xonly = xonly;
         st->k = k + 1;
         st->done = done_next;
         /* seed p update: p, r, s in, p out; at a flush also the window
@@ -1143,6 +1195,35 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         __syncthreads();
         if (sh_flush)
             for (int j = tid; j < M; j += HB_BLOCK) d.grow[j] = d.grow[M + j];
+    }
+    if (PUB) {
+        __syncthreads();
+        if (sh_flush) {
+            /* the list covers every position this iteration wrote (those of
+             * shifts that stopped in the window before are stored again
+             * unchanged) */
+            const int cnt = slot == 0 ? sh_cnt : st->flush_cnt;
+            for (int pos = tid; pos < cnt; pos += HB_BLOCK) {
+                double *c = coef + (size_t)pos * 6;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) store_pub<true>(&c[e], c[e]);
+                store_pub<true>(&d.act_len[pos], d.act_len[pos]);
+                if (slot == 0) store_pub<true>(&d.flist[pos], d.flist[pos]);
+            }
+            if (tid == 0) {
+                store_pub<true>(&st->flush_cnt, cnt);
+                store_pub<true>(&st->win_seedpos, slot == 0 ? seedpos : win_seedpos);
+                store_pub<true>(&st->flush_k0, win_k0);
+                store_pub<true>(&st->flush_len, len);
+                store_pub<true>(&st->flush_tail, sh_spike);
+            }
+        }
+        /* every store the launch's other workgroups read has landed before
+         * the decision moves */
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0)
+            mail_put(&st->mail_f, sh_flush ? (unsigned long long)(HB_FL_FLUSH | (sh_xonly ? HB_FL_XONLY : 0)) : 0ull);
     }
 }
 
@@ -1238,32 +1319,35 @@ __global__ __launch_bounds__(HB_BLOCK) void k_seed_update_fin(hb_dev d)
 }
 
 /* (r,r), (r^,r) from the seed-update partials, then the scalar stage. */
-template <bool EXACT>
-DEV void fin_seed_body(const hb_dev &d)
+/* false when the solve is done (nothing ran) */
+template <bool EXACT, bool PUB = false>
+DEV bool fin_seed_body(const hb_dev &d)
 {
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     __shared__ double sh[2];
     hb_state *st = d.st;
-    if (st->done) return;
+    if (st->done) return false;
     double dr, rr;
     if (EXACT) {
         const double *rn = d.rring + off(st->k % (d.W + 1), d.n);
         dr = serial_dot(rn, rn, d.n, lds);
         rr = serial_dot(d.rhat, rn, d.n, lds);
     } else {
-        reduce_partials(d.part3, d.vec_grid, dr, rr);
+        /* at most 2048 pairs: eight loads in flight per thread cover them,
+         * and the shift kernel this stage may run in keeps its registers */
+        reduce_partials<!PUB>(d.part3, d.vec_grid, dr, rr);
     }
     if (threadIdx.x == 0) {
         sh[0] = dr;
         sh[1] = rr;
-        /* the fused launches' readers of omega~ and of the coefficient
-         * count finished with the last launches */
+        /* the fused launches' readers of omega~ finished with the last
+         * launches */
         st->mail_w = HB_MAIL_EMPTY;
         st->mail_a = HB_MAIL_EMPTY;
-        st->coef_ready = 0;
     }
     __syncthreads();
-    scalar_stage(d, sh[0], sh[1]);
+    scalar_stage<PUB>(d, sh[0], sh[1]);
+    return true;
 }
 
 template <bool EXACT>
@@ -1520,16 +1604,32 @@ DEV dd dd_mul(dd a, double c)
     const double e = __builtin_fma(a.hi, c, -p);
     return quick_two_sum(p, __builtin_fma(a.lo, c, e));
 }
-template <bool PUB = false>
+/* PUB: the block is read by the flush workgroups of the same launch.  MAIL:
+ * the scalar stage runs in the launch's first workgroup; the decision and the
+ * window's extent come from it. */
+template <bool PUB = false, bool MAIL = false>
 DEV void flush_coef_body(const hb_dev &d, int pos, const double *__restrict__ coef,
                          double *__restrict__ ccoef)
 {
     hb_state *st = d.st;
-    if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
-    if (!st->flush || pos >= st->flush_cnt) return;
-    const int L = st->flush_len, M = d.M, W = d.W;
-    const int Lc = L - st->flush_tail;   /* regrouped iterations */
-    const int lv = d.act_len[pos];
+    int L, cnt, ntail, lv;
+    if (MAIL) {
+        const unsigned long long f = mail_wait_bits(&st->mail_f, st);
+        if (f == HB_MAIL_EMPTY || !(f & HB_FL_FLUSH)) return;
+        cnt = load_pub(&st->flush_cnt);
+        if (pos >= cnt) return;
+        L = load_pub(&st->flush_len);
+        ntail = load_pub(&st->flush_tail);
+        lv = load_pub(&d.act_len[pos]);
+    } else {
+        if (st->k4_epoch >= st->k) return;   /* no scalar stage since the last run */
+        if (!st->flush || pos >= st->flush_cnt) return;
+        L = st->flush_len;
+        ntail = st->flush_tail;
+        lv = d.act_len[pos];
+    }
+    const int M = d.M, W = d.W;
+    const int Lc = L - ntail;   /* regrouped iterations */
     const int t = threadIdx.x;
     const bool pch = t == HB_BLOCK - 1;   /* the p_j component */
     double *blk = ccoef + (size_t)(pos / HB_UG) * (size_t)HB_CC_ROWS(W) * (2 * HB_UG);
@@ -1582,6 +1682,20 @@ __global__ __launch_bounds__(HB_BLOCK) void k_flush_coef(hb_dev d)
     flush_coef_body(d, blockIdx.x, d.coef, d.ccoef);
 }
 
+/* The window a flush applies: its first iteration, list length, regrouped
+ * iterations, tail flag and the seed's list position, uniform across the
+ * workgroup (read once by the caller, kept in scalar registers). */
+struct hb_fl {
+    int k0, cnt, Lc, tail, sp;
+};
+
+DEV hb_fl flush_window(int k0, int cnt, int len, int tail, int sp)
+{
+    return {__builtin_amdgcn_readfirstlane(k0), __builtin_amdgcn_readfirstlane(cnt),
+            __builtin_amdgcn_readfirstlane(len - tail), __builtin_amdgcn_readfirstlane(tail),
+            __builtin_amdgcn_readfirstlane(sp)};
+}
+
 /* The window flush of tile bx (256 elements): the four wavefronts walk the
  * tile's four 64-element sub-tiles together, wavefront w taking shift
  * chunks w, w+4, ...: the wavefronts read the same g_k slices at about the
@@ -1591,15 +1705,13 @@ __global__ __launch_bounds__(HB_BLOCK) void k_flush_coef(hb_dev d)
  * readfirstlane so the compiler knows it is uniform.  The solve's last
  * flush, which needs x_j only, is flush_apply_x2 below. */
 template <int UG>
-DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double *__restrict__ pset,
-                     const double *__restrict__ rring, const double *__restrict__ qring,
-                     const double *__restrict__ ccoef, const int *__restrict__ flist)
+DEV void flush_apply(const hb_dev &d, int bx, const hb_fl &fw, double *__restrict__ xset,
+                     double *__restrict__ pset, const double *__restrict__ rring,
+                     const double *__restrict__ qring, const double *__restrict__ ccoef,
+                     const int *__restrict__ flist)
 {
-    const hb_state *st = d.st;
     const int n = d.n, W = d.W;
-    const int k0 = st->flush_k0, cnt = st->flush_cnt;
-    const int Lc = st->flush_len - st->flush_tail, tail = st->flush_tail;
-    const int sp = st->win_seedpos;
+    const int k0 = fw.k0, cnt = fw.cnt, Lc = fw.Lc, tail = fw.tail, sp = fw.sp;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
@@ -1713,15 +1825,13 @@ DEV void flush_apply(const hb_dev &d, int bx, double *__restrict__ xset, double 
  * lane carries two elements, i and i + 64: every wait then covers twice
  * the multiply-adds, and the loop runs half as many steps. */
 template <int UG>
-DEV void flush_apply_x2(const hb_dev &d, int bx, double *__restrict__ xset, const double *__restrict__ pset,
-                        const double *__restrict__ rring, const double *__restrict__ qring,
-                        const double *__restrict__ ccoef, const int *__restrict__ flist)
+DEV void flush_apply_x2(const hb_dev &d, int bx, const hb_fl &fw, double *__restrict__ xset,
+                        const double *__restrict__ pset, const double *__restrict__ rring,
+                        const double *__restrict__ qring, const double *__restrict__ ccoef,
+                        const int *__restrict__ flist)
 {
-    const hb_state *st = d.st;
     const int n = d.n, W = d.W;
-    const int k0 = st->flush_k0, cnt = st->flush_cnt;
-    const int Lc = st->flush_len - st->flush_tail;
-    const int sp = st->win_seedpos;
+    const int k0 = fw.k0, cnt = fw.cnt, Lc = fw.Lc, sp = fw.sp;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nch = (cnt + UG - 1) / UG;
@@ -1875,8 +1985,9 @@ DEV void shift_cc_body(const hb_dev &d, int bx, double *__restrict__ xset,
         }
         __syncthreads();
     }
-    if (st->flush_xonly) flush_apply_x2<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
-    else flush_apply<UG>(d, bx, xset, pset, rring, qring, ccoef, flist);
+    const hb_fl fw = flush_window(st->flush_k0, st->flush_cnt, st->flush_len, st->flush_tail, st->win_seedpos);
+    if (st->flush_xonly) flush_apply_x2<UG>(d, bx, fw, xset, pset, rring, qring, ccoef, flist);
+    else flush_apply<UG>(d, bx, fw, xset, pset, rring, qring, ccoef, flist);
 }
 
 template <bool FUSED = false>
@@ -1901,6 +2012,155 @@ __global__ __launch_bounds__(HB_BLOCK) void k_shift_fused(hb_dev d)
         return;
     }
     shift_cc_tile<true>(d, blockIdx.x - d.M);
+}
+
+/* p = beta~ p + r + (-beta~ omega~) s in the order of the shift kernel's
+ * seed p update above (src/solver_dynamic.c:208-210) */
+DEV double p_step(double p, double r, double s, double pb, double pbw)
+{
+    double pe = pb * p;
+    pe = pe + r;
+    return pe + pbw * s;
+}
+
+DEV dbl2 p_step2(dbl2 p, dbl2 r, dbl2 s, double pb, double pbw)
+{
+    p.x = p_step(p.x, r.x, s.x, pb, pbw);
+    p.y = p_step(p.y, r.y, s.y, pb, pbw);
+    return p;
+}
+
+/* Tile workgroup t of T in the shift kernel that runs the scalar stage.  The
+ * seed p update of the finished iteration takes elements in 256-thread
+ * chunks t, t + T, ... (pairs of elements when n is even, so every access is
+ * 16 bytes); each thread loads its first four chunks' p, r and s before
+ * beta~ arrives from the stage.  T is sized so that the tiles stay resident:
+ * their loads run while the stage does.  Then, when the stage has decided a
+ * flush and the combined coefficients are in, the workgroup applies flush
+ * tiles t, t + T, ...  The solve's last flush (x_j only, the longest) is left
+ * to k_flush_final after the loop: in this kernel the compiler keeps one of
+ * a window step's four coefficient blocks in flight instead of four, and
+ * that flush ran 6.9 ms here against 4.3 ms in a kernel of its own. */
+DEV void shift_stage_tiles(const hb_dev &d, int t, int T)
+{
+    hb_state *st = d.st;
+    const int n = d.n, W = d.W;
+    /* written by omega~'s finalize in the seed update's launch: the stage
+     * moves k and the seed, and rescales the archives at a switch */
+    const int k = st->pu_k, s = min(max(st->pu_seed, 0), d.M - 1);
+    const double w = st->pu_w;
+    double *ps = d.p_set + off(s, n);
+    const double *rk = d.rring + off(k % (W + 1), n);
+    const double *sv = d.s;
+    constexpr int E = 4;
+    const int stride = T * HB_BLOCK;
+    int q = t * HB_BLOCK + (int)threadIdx.x;
+    if (!(n & 1)) {
+        const int np = n >> 1;
+        dbl2 pv[E], rv[E], sw[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            /* clamped duplicates past the end (n >= 2 here) */
+            const int qe = min(q + e * stride, np - 1);
+            pv[e] = ((const dbl2 *)ps)[qe];
+            rv[e] = ((const dbl2 *)rk)[qe];
+            sw[e] = ((const dbl2 *)sv)[qe];
+        }
+        const unsigned long long bb = mail_wait_bits(&st->mail_b, st);
+        if (bb == HB_MAIL_SKIP || bb == HB_MAIL_EMPTY) return;
+        const double pb = __longlong_as_double((long long)bb), pbw = -pb * w;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (q < np) ((dbl2 *)ps)[q] = p_step2(pv[e], rv[e], sw[e], pb, pbw);
+            q += stride;
+        }
+        for (; q < np; q += stride)
+            ((dbl2 *)ps)[q] = p_step2(((const dbl2 *)ps)[q], ((const dbl2 *)rk)[q], ((const dbl2 *)sv)[q], pb, pbw);
+    } else {
+        double pv[E], rv[E], sw[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int ie = min(q + e * stride, n - 1);
+            pv[e] = ps[ie];
+            rv[e] = rk[ie];
+            sw[e] = sv[ie];
+        }
+        const unsigned long long bb = mail_wait_bits(&st->mail_b, st);
+        if (bb == HB_MAIL_SKIP || bb == HB_MAIL_EMPTY) return;
+        const double pb = __longlong_as_double((long long)bb), pbw = -pb * w;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (q < n) ps[q] = p_step(pv[e], rv[e], sw[e], pb, pbw);
+            q += stride;
+        }
+        for (; q < n; q += stride) ps[q] = p_step(ps[q], rk[q], sv[q], pb, pbw);
+    }
+
+    const unsigned long long f = mail_wait_bits(&st->mail_f, st);
+    if (f == HB_MAIL_EMPTY || !(f & HB_FL_FLUSH) || (f & HB_FL_XONLY)) return;
+    __shared__ int sh_w[5];
+    if (threadIdx.x == 0) {
+        /* the combined coefficients come from workgroups 1..M: wait until
+         * every listed position has stored its block (as shift_cc_body) */
+        const int want = load_pub(&st->flush_cnt);
+        for (int spins = 0;; ++spins) {
+            if (__hip_atomic_load(&st->coef_ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) break;
+            if (spins == HB_SPIN_LIMIT / 16 || ((spins & 15) == 15 && sync_timed_out(st))) {
+                atomicOr(&st->status, HB_ST_SYNC_TIMEOUT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(127);
+        }
+        sh_w[0] = load_pub(&st->flush_k0);
+        sh_w[1] = want;
+        sh_w[2] = load_pub(&st->flush_len);
+        sh_w[3] = load_pub(&st->flush_tail);
+        sh_w[4] = load_pub(&st->win_seedpos);
+    }
+    __syncthreads();
+    const hb_fl fw = flush_window(sh_w[0], sh_w[1], sh_w[2], sh_w[3], sh_w[4]);
+    for (int bx = t; bx < d.shift_gx; bx += T) {
+        /* the wavefronts start each tile together, as they walk its
+         * sub-tiles together */
+        if (bx != t) __syncthreads();
+        flush_apply<HB_UG>(d, bx, fw, d.x_set, d.p_set, d.rring, d.qring, d.ccoef, d.flist);
+    }
+}
+
+/* The solve's last window flush (x_j only) after the iteration loop, when the
+ * shift kernel runs the scalar stage: that iteration's stage marked it
+ * (flush, flush_xonly) and its coefficient workgroups formed the combined
+ * coefficients.  One workgroup per 256-element tile. */
+__global__ __launch_bounds__(HB_BLOCK) void k_flush_final(hb_dev d)
+{
+    const hb_state *st = d.st;
+    if (!st->flush || !st->flush_xonly) return;
+    const hb_fl fw = flush_window(st->flush_k0, st->flush_cnt, st->flush_len, st->flush_tail, st->win_seedpos);
+    flush_apply_x2<HB_UG>(d, blockIdx.x, fw, d.x_set, d.p_set, d.rring, d.qring, d.ccoef, d.flist);
+}
+
+/* Shift kernel with the scalar stage (default mode, fused level 1):
+ * workgroup 0 runs the stage, workgroups 1..M form the combined coefficients
+ * when it decides a flush, the rest are the shift_tw tiles above.  Five
+ * waves per SIMD, as the shift kernel without the stage: the compiler fits
+ * the three roles in 94 registers without spilling, where left alone it
+ * takes 108 (four waves). */
+__global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_shift_stage(hb_dev d)
+{
+    if (blockIdx.x == 0) {
+        /* a done solve: the launch's seed p update and flush have nothing
+         * to do */
+        if (!fin_seed_body<false, true>(d) && threadIdx.x == 0) {
+            mail_put(&d.st->mail_b, HB_MAIL_SKIP);
+            mail_put(&d.st->mail_f, 0ull);
+        }
+        return;
+    }
+    if ((int)blockIdx.x <= d.M) {
+        flush_coef_body<true, true>(d, blockIdx.x - 1, d.coef, d.ccoef);
+        return;
+    }
+    shift_stage_tiles(d, blockIdx.x - 1 - d.M, gridDim.x - 1 - d.M);
 }
 
 /* ---- initialisation (src/solver_dynamic.c:101-123) ----------------------- */
@@ -2275,18 +2535,21 @@ int hb_k_init(const hb_dev *d, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-/* The fused default-mode iteration: six launches (SpMV 1, its finalize,
- * SpMV 2, the seed update with the omega~ finalize as its first workgroup,
- * the scalar stage, the shift kernel with the combined coefficients in its
- * first workgroups).  SpMV 2 keeps its finalize as a launch of its own:
- * reading SpMV 1's 8192 partials takes a workgroup about 4 us, in which a
- * fused launch's first round of tiles would have nothing to do.  The
- * stages folded into another launch enqueue nothing. */
+/* The fused default-mode iteration: SpMV 1; SpMV 2 with the alpha~
+ * finalize as its first workgroup (fused_a; else the finalize keeps a launch
+ * of its own); the seed update with the omega~ finalize as its first
+ * workgroup; the shift kernel with the scalar stage as its first workgroup
+ * (fused_s; else the stage keeps a launch of its own) and the combined
+ * coefficients in the next ones.  The stages folded into another launch
+ * enqueue nothing. */
 static int stage_fused(const hb_dev *d, int stage, hipStream_t s)
 {
     const dim3 bl(HB_BLOCK);
     switch (stage) {
     case HB_STAGE_FIN1:
+        break;
+    case HB_STAGE_SCALAR:
+        if (!d->fused_s) hipLaunchKernelGGL(k_fin_seed<false>, dim3(1), bl, 0, s, *d);
         break;
     case HB_STAGE_SPMV2:
         if (d->vidx) hipLaunchKernelGGL(k_spmv2_fa<true>, dim3(d->nblk + 1), bl, 0, s, *d);
@@ -2296,7 +2559,8 @@ static int stage_fused(const hb_dev *d, int stage, hipStream_t s)
         hipLaunchKernelGGL(k_seed_update_fin, dim3(d->vec_grid + 1), bl, 0, s, *d);
         break;
     case HB_STAGE_SHIFT:
-        hipLaunchKernelGGL(k_shift_fused, dim3(d->M + d->shift_gx), bl, 0, s, *d);
+        if (d->fused_s) hipLaunchKernelGGL(k_shift_stage, dim3(1 + d->M + d->shift_tw), bl, 0, s, *d);
+        else hipLaunchKernelGGL(k_shift_fused, dim3(d->M + d->shift_gx), bl, 0, s, *d);
         break;
     case HB_STAGE_FIN2:
     case HB_STAGE_COEF:
@@ -2312,7 +2576,7 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
     const dim3 gs(d->nblk), gv(d->vec_grid), bl(HB_BLOCK), one(1);
     if (d->fused && !d->exact &&
         (stage == HB_STAGE_FIN2 || stage == HB_STAGE_SEED || stage == HB_STAGE_COEF || stage == HB_STAGE_SHIFT ||
-         (d->fused_a && (stage == HB_STAGE_FIN1 || stage == HB_STAGE_SPMV2))))
+         stage == HB_STAGE_SCALAR || (d->fused_a && (stage == HB_STAGE_FIN1 || stage == HB_STAGE_SPMV2))))
         return stage_fused(d, stage, s);
     if (d->npanel > 1 && (stage == HB_STAGE_SPMV1 || stage == HB_STAGE_SPMV2)) {
         if (stage == HB_STAGE_SPMV2 && d->q_unfused) hipLaunchKernelGGL(k_qform, gv, bl, 0, s, *d);
@@ -2404,7 +2668,23 @@ int hb_k_stage_runs(const hb_dev *d, int stage)
     if (d->exact || !d->fused) return 1;
     if (stage == HB_STAGE_FIN2) return 0;
     if (stage == HB_STAGE_FIN1) return !d->fused_a;
+    if (stage == HB_STAGE_SCALAR) return !d->fused_s;
     return 1;
+}
+
+int hb_k_shift_stage_tiles(int n_tiles, int M, int device)
+{
+    /* as many tile workgroups as stay resident next to the stage and the
+     * coefficient workgroups, at least one per compute unit, at most one
+     * per flush tile */
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_shift_stage, HB_BLOCK, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return -1;
+    long long t = (long long)per_cu * cus - 1 - M;
+    if (t < cus) t = cus;
+    if (t > n_tiles) t = n_tiles;
+    return t > 0 ? (int)t : 1;
 }
 
 int hb_k_iteration(const hb_dev *d, hipStream_t s)
@@ -2424,6 +2704,7 @@ int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s)
 
 int hb_k_finalize(const hb_dev *d, hipStream_t s)
 {
+    if (d->fused_s) hipLaunchKernelGGL(k_flush_final, dim3(d->shift_gx), dim3(HB_BLOCK), 0, s, *d);
     hipLaunchKernelGGL(k_finalize, dim3(d->vec_grid), dim3(HB_BLOCK), 0, s, *d);
     return (int)hipGetLastError();
 }

@@ -22,13 +22,17 @@ int hb_k_init(const hb_dev *d, hipStream_t s);
  * one-workgroup finalize (the last one is the scalar stage: shift
  * recurrences, convergence check, seed switch), then the window flush
  * (combined coefficients and the shift kernel).  With fused launches the
- * alpha~ and omega~ finalizes and the combined coefficients run as the
- * first workgroups of SpMV 2, the seed update and the shift kernel.  Each
- * kernel returns at once when the solve is done. */
+ * alpha~ and omega~ finalizes run as the first workgroups of SpMV 2 and the
+ * seed update, and the scalar stage and the combined coefficients as the
+ * first workgroups of the shift kernel.  Each kernel returns at once when
+ * the solve is done. */
 int hb_k_iteration(const hb_dev *d, hipStream_t s);
 /* 1 if hb_k_stage launches a kernel for `stage` (fused launches run some
  * stages inside a neighbouring one) */
 int hb_k_stage_runs(const hb_dev *d, int stage);
+/* Tile workgroups of the shift kernel that runs the scalar stage (d->shift_tw)
+ * for n_tiles flush tiles and M shifts on `device`; < 0 on a HIP error. */
+int hb_k_shift_stage_tiles(int n_tiles, int M, int device);
 
 /* The kernels of an iteration, one launch at a time (per-kernel timing). */
 #define HB_STAGE_SPMV1  0   /* SpMV s = (A + sigma_s I) p_s, (r^, s) partials   */
@@ -45,7 +49,9 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s);
 /* Copies {done, k} of the device state into host-visible memory. */
 int hb_k_publish(const hb_dev *d, int *flag_dev, hipStream_t s);
 
-/* Applies a seed-switch rescale of r left pending by the last iteration. */
+/* After the iteration loop: the solve's last, x-only window flush when the
+ * shift kernel runs the scalar stage (d->fused_s), then a seed-switch rescale
+ * of r left pending by the last iteration. */
 int hb_k_finalize(const hb_dev *d, hipStream_t s);
 
 /* out[j] = ||b - (A + sigma_j I) x_j|| / ||b|| for every shift j (sigma, x,

@@ -39,6 +39,12 @@ extern "C" {
 /* A mailbox word that holds no value yet: a signalling-NaN bit pattern, which
  * no arithmetic produces (NaN results are quiet). */
 #define HB_MAIL_EMPTY 0x7ff4dead00c0ffeeull
+/* beta~'s mailbox when the scalar stage did not run (the solve is done) */
+#define HB_MAIL_SKIP 0x7ff4dead00005c1full
+/* bits of the window decision the fused scalar stage publishes (mail_f);
+ * 0: no flush, or the stage did not run */
+#define HB_FL_FLUSH 1
+#define HB_FL_XONLY 2
 
 /*
  * Device-resident state of one shifted solve.  Every scalar the reference
@@ -77,12 +83,22 @@ typedef struct hb_state {
     int win_seedpos;   /* default mode: list position of the seed in the pending
                           window (its x is deferred like a shift's), else -1 */
     int coef_ready;    /* fused shift kernel: list positions whose combined
-                          coefficients are stored (reset by the scalar stage) */
+                          coefficients are stored (reset by SpMV 1) */
     /* Mailbox of the fused seed update: omega~ of the running iteration,
      * published by the launch's first workgroup to its other workgroups;
      * HB_MAIL_EMPTY until then (the scalar stage empties it). */
     unsigned long long mail_w;
     unsigned long long mail_a;   /* alpha~ handed from SpMV 2's first workgroup */
+    /* The shift kernel that runs the scalar stage in its first workgroup:
+     * beta~ (HB_MAIL_SKIP when the stage did not run), published as soon as
+     * it is formed, then the window decision (HB_FL_* bits); both emptied by
+     * SpMV 1. */
+    unsigned long long mail_b;
+    unsigned long long mail_f;
+    double pu_w;       /* omega~ of the running iteration, and its iteration and */
+    int pu_k, pu_seed; /* seed (written by omega~'s finalize, read by the seed p
+                          update of the shift kernel that runs the scalar stage:
+                          the stage rescales the archives at a switch)         */
 } hb_state;
 
 /* Every pointer below addresses global device memory.  Device code sees
@@ -143,6 +159,10 @@ typedef struct hb_dev {
     int fused_a;                     /* the alpha~ finalize runs as SpMV 2's first
                                         workgroup (fused launches, q formed at the
                                         gathered columns, one pass)                    */
+    int fused_s;                     /* the scalar stage runs as the shift kernel's
+                                        first workgroup                                */
+    int shift_tw;                    /* tile workgroups of that kernel: each walks the
+                                        seed p update and the flush tiles grid-stride */
     HB_G const double   *sigma;/* [M]                                      */
     HB_G const double   *rhat; /* [n]   shadow residual r^ = b             */
     HB_G double *rring;        /* [(W+1)][n] residual of iteration k at slot k mod (W+1) */

@@ -78,17 +78,21 @@ typedef struct hbicg_params {
                             within 16-bit column offsets of their tile (stencils), else
                             formed for every row first and gathered; 1: always at the
                             gathered columns; 2: always beforehand.  Same results.        */
-    int    fused;        /* default mode: 1 (default) five launches per iteration --
+    int    fused;        /* default mode: 1 (default) four launches per iteration --
                             the alpha~ finalize runs as the first workgroup of SpMV
                             2's launch (where SpMV 2 forms q at its gathered columns
                             in one pass; else it keeps its own launch), the omega~
-                            finalize as the first workgroup of the seed update's, each
-                            handing its value to the launch's other workgroups, and
-                            the combined window coefficients run in the shift
-                            kernel's first workgroups; 2: the last two only (six
-                            launches); 0: eight launches.  Same results bit for bit;
-                            the batched solver always uses eight, exact order seven
-                            (no coefficient kernel)                                     */
+                            finalize as the first workgroup of the seed update's, the
+                            scalar stage as the first workgroup of the shift kernel's
+                            (beta~ handed to the seed p update as soon as it is
+                            formed, the window decision after), each handing its
+                            values to the launch's other workgroups, and the combined
+                            window coefficients run in the shift kernel's next
+                            workgroups; 3: the same with the scalar stage in a launch
+                            of its own (five launches); 2: also alpha~ on its own
+                            (six); 0: eight launches.  Same results bit for bit; the
+                            batched solver always uses eight, exact order seven (no
+                            coefficient kernel)                                         */
     int    panel_cols;   /* seed SpMVs in column panels of this many columns: one pass
                             per panel over the entries whose columns lie in it, row sums
                             carried between passes, so the gathers of a pass stay within

@@ -168,6 +168,28 @@ def test_c1_default_order_plan_and_dropins(c1, gold, seed):
     assert np.array_equal(xs["static"].r, xs["dynamic"].r)
 
 
+def test_c1_fused_levels_bitwise(c1):
+    """At full size, with the seed switching (first seed 63): the four-launch
+    iteration (scalar stage inside the shift kernel, whose tiles walk the
+    seed p update and the flush grid-stride) against the eight-launch one,
+    bit for bit, as a device plan with hipGraph chunks."""
+    print("case c1 fused levels 1 and 0, seed 63", flush=True)
+    b = torch.as_tensor(c1.b).cuda()
+    out = []
+    for level in (1, 0):
+        pl = Plan(c1.A, c1.M, make_params(tol=c1.tol, max_iter=c1.max_iter, graph=1, fused=level))
+        r = torch.empty_like(b)
+        k, x, rep = pl.solve(b, c1.sigma, 63, r_out=r)
+        _, stop, sw = pl.report()
+        assert not (rep["status"] & 16)
+        out.append((k, stop, sw, x, r))
+        pl.close()
+    (k1, s1, w1, x1, r1), (k0, s0, w0, x0, r0) = out
+    assert k1 == k0 and w1 == w0 and len(w1) >= 1
+    assert np.array_equal(s1, s0)
+    assert torch.equal(x1, x0) and torch.equal(r1, r0)
+
+
 def test_true_residual_checker_against_oracle(c1):
     """hbicg_plan_true_residuals against the oracle's checker on x far from
     converged (every residual O(1)): the 64x64 workload's 16 shifts and 4

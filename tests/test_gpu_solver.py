@@ -145,6 +145,22 @@ def test_window_tail_path(c0, oracle_c0, seed):
     assert tr.max() <= 10 * c0.tol, tr.max()
 
 
+@pytest.mark.parametrize("seed", [0, 8])
+def test_fused_launches_bitwise_tail_path(c0, seed):
+    """The tail path (growth_limit 0.5: every window closes after its first
+    iteration, which the flush applies on its own) through the four- and
+    five-launch iterations: bit for bit the eight-launch result."""
+    print(f"case fused levels, tail path, seed={seed}", flush=True)
+    mk = lambda f: make_params(tol=c0.tol, window=64, growth_limit=0.5, fused=f)
+    ref = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", mk(0))
+    for level in (1, 3):
+        a = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", mk(level))
+        assert a.k == ref.k and a.switches == ref.switches
+        np.testing.assert_array_equal(a.stop_iter, ref.stop_iter)
+        assert np.array_equal(a.x, ref.x) and np.array_equal(a.r, ref.r)
+        assert not (a.report["status"] & 16)
+
+
 @pytest.mark.parametrize("seed", [0, 8, 15])
 def test_against_golden_fixture(c0, seed):
     """GPU results against the committed oracle fixture
@@ -442,21 +458,24 @@ def test_ragged_sizes(M, seed):
 @pytest.mark.parametrize("value_index", [True, False])
 @pytest.mark.parametrize("seed", [0, 8])
 def test_fused_launches_bitwise(c0, seed, value_index):
-    """Default mode: the five-launch iteration (alpha~ finalized by the first
+    """Default mode: the four-launch iteration (alpha~ finalized by the first
     workgroup of SpMV 2's launch, omega~ by the first of the seed update's,
-    each handed to the launch's other workgroups; combined coefficients
-    formed in the shift kernel's first workgroups) and the six-launch one
-    (alpha~ on its own) are the eight-launch iteration's arithmetic: k, stop
-    iterations, switches, x and r bit for bit, and no workgroup timed out
-    waiting.  With the value table SpMV 2 reads diagonal tiles (alpha~ read
-    after the gathers are issued); without it, 8-byte CSR tiles (alpha~
-    read first)."""
-    for variant in ("static", "dynamic"):
+    the scalar stage by the first of the shift kernel's, each handing its
+    values to the launch's other workgroups; combined coefficients formed in
+    the shift kernel's next workgroups), the five-launch one (the scalar
+    stage on its own) and the six-launch one (alpha~ on its own too) are the
+    eight-launch iteration's arithmetic: k, stop iterations, switches, x and
+    r bit for bit, and no workgroup timed out waiting.  With the value table
+    SpMV 2 reads diagonal tiles (alpha~ read after the gathers are issued);
+    without it, 8-byte CSR tiles (alpha~ read first).  `normal` opens and
+    flushes a window in every iteration, so the flush reads a list the
+    stage wrote in the same launch; seed 8 switches at iteration 102."""
+    for variant in ("normal", "static", "dynamic"):
         print(f"case fused vs unfused c0 seed={seed} {variant} value_index={value_index}", flush=True)
         mk = lambda f: make_params(tol=c0.tol, fused=f, value_index=value_index)
         ref = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(0))
         assert not (ref.report["status"] & 16)
-        for level in (1, 2):
+        for level in (1, 2, 3):
             a = dropin_solve(c0.A, c0.b, c0.sigma, seed, variant, mk(level))
             assert a.k == ref.k and a.switches == ref.switches
             np.testing.assert_array_equal(a.stop_iter, ref.stop_iter)
