@@ -910,28 +910,36 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv2_fa(hb_dev d)
  * publishes the window decision.  (Atomic stores inside the stage's body
  * would cost it about 30 registers: the compiler stops keeping its uniform
  * loads in scalar registers.) */
+/* The stage's inputs from earlier launches.  The state words are read with
+ * the done test, the archive entries they index next, both before the
+ * seed-update partials are reduced, so their round trips overlap. */
+struct hb_stage_in {
+    int k, s, win_k0, win_seedpos;
+    int stop_count, max_sigma, status, n_switch, flush_cnt;
+    double rTr_old, dot_zero, dot_r_prev, qTq, shift_bytes;
+    double ak, wk, bkm1, akm1, sig_s;
+    int stop_s;   /* the seed's stop flag (0 while it runs) */
+};
+
 template <bool PUB = false>
-DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
+DEV void scalar_stage(const hb_dev &d, const hb_stage_in &in, double dot_r, double rTr)
 {
     __shared__ double sh_beta;
     __shared__ double sh_zp[HB_BLOCK];
     __shared__ int sh_j[HB_BLOCK];
     __shared__ int sh_nc[HB_BLOCK];
-    __shared__ int sh_switch, sh_spike, sh_flush, sh_xonly, sh_bad, sh_upd, sh_cnt;
-    __shared__ double sh_rprev;
+    __shared__ int sh_switch, sh_spike, sh_flush, sh_xonly, sh_bad, sh_upd, sh_sc, sh_ms, sh_nst;
     hb_state *st = d.st;
     const int tid = threadIdx.x;
-    /* every value this stage reads, loaded before its first store */
-    const int k = st->k, s = st->seed, M = d.M;
-    const int win_k0 = st->win_k0, win_seedpos = st->win_seedpos;
-    const double rTr_old = st->rTr_old, dot_zero = st->dot_zero;
-    const double ak = d.aa[k], wk = d.wa[k], bkm1 = d.ba[k - 1], akm1 = d.aa[k - 1];
-    const double sig_s = d.sigma[s];
+    const int k = in.k, s = in.s, M = d.M;
+    const int win_k0 = in.win_k0, win_seedpos = in.win_seedpos;
+    const double rTr_old = in.rTr_old, dot_zero = in.dot_zero;
+    const double ak = in.ak, wk = in.wk, bkm1 = in.bkm1, akm1 = in.akm1;
+    const double sig_s = in.sig_s;
 
     if (tid == 0) {
         const double beta = (ak / wk) * (rTr / rTr_old);
         if (PUB) mail_put_d(&st->mail_b, beta);
-        sh_rprev = st->dot_r;
         sh_spike = 0;
         sh_bad = 0;
         st->dot_r = dot_r;
@@ -948,6 +956,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     const int slot = k - win_k0;
     double *coef = d.coef + (size_t)slot * (size_t)M * 6;
     int seedpos = slot == 0 ? -1 : win_seedpos;
+    int cnt_now = in.flush_cnt;   /* the window's list length */
 
     if (slot == 0) {
         /* A window opens: its shift list is every shift updated in the
@@ -990,8 +999,8 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             }
             st->win_seedpos = seedpos;
             st->flush_cnt = base;
-            sh_cnt = base;
         }
+        cnt_now = base;
     }
 
     /* Opt-in guard of the default mode (growth_limit > 0): how large the
@@ -1005,16 +1014,20 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
      * flush already matches the per-iteration accuracy, and closing a window
      * where the bound peaks lands on the iterations of heaviest
      * cancellation. */
-    const double nq = sqrt(st->qTq), nro = sqrt(sh_rprev), nrn = sqrt(dot_r);
+    const double nq = sqrt(in.qTq), nro = sqrt(in.dot_r_prev), nrn = sqrt(dot_r);
     double best_zp = 1.0;
     int best_j = -1, nconv = 0;
     for (int j = tid; j < M; j += HB_BLOCK) {
-        if (d.stop[j]) continue;
+        /* every operand is loaded ahead of the stop test: one round trip */
+        const int stopped = d.stop[j];
+        const double sgj = d.sigma[j], pim1 = PI(d, j, k - 1), etaj = d.eta[j], zj = d.zeta[j];
+        const int posj = d.fpos[j];
+        const double growj = d.growth_limit > 0.0 ? d.grow[j] : 0.0;
+        if (stopped) continue;
         double zp;
         if (j != s) {
-            const double dj = sig_s - d.sigma[j];
-            const double pim1 = PI(d, j, k - 1);
-            const double eta = ratio * ak * d.eta[j] - dj * ak * pim1;
+            const double dj = sig_s - sgj;
+            const double eta = ratio * ak * etaj - dj * ak * pim1;
             const double pik = eta + pim1;
             const double aj = (pim1 / pik) * ak;
             const double den = 1.0 - wk * dj;
@@ -1023,7 +1036,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
              * the arithmetic goes on as in the reference and the status says so */
             if (!(den != 0.0 && isfinite(den) && pik != 0.0 && isfinite(pik))) sh_bad = 1;
             const double wj = wk / den;
-            const double z = d.zeta[j];
+            const double z = zj;
             const double az = aj * z;
             const double c1 = wj / (pik * z);
             const double c2 = wj / (az * pik);
@@ -1035,13 +1048,13 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
             d.eta[j] = eta;
             PI(d, j, k) = pik;
             d.zeta[j] = znew;
-            const int pos = d.fpos[j];
+            const int pos = posj;
             double *c = coef + (size_t)pos * 6;
             c[0] = c1; c[1] = aj; c[2] = c2; c[3] = c3; c[4] = bj; c[5] = c4;
             d.act_len[pos] = slot + 1;
             if (d.growth_limit > 0.0) {
                 const double own = fabs(bj) * (fabs(c2) * nq + fabs(c3) * nro) + fabs(c4) * nrn;
-                const double prev = k == 1 ? sqrt(dot_zero) : d.grow[j];   /* p_j = b at k = 1 */
+                const double prev = k == 1 ? sqrt(dot_zero) : growj;   /* p_j = b at k = 1 */
                 const double bound = fabs(bj) * prev + own;
                 if (!(bound <= d.growth_limit * own)) sh_spike = 1;
                 d.grow[j] = bound;
@@ -1076,6 +1089,14 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
     sh_j[tid] = best_j;
     sh_nc[tid] = nconv;
     __syncthreads();
+    /* A window closes only at its full length, at the iteration cap, when
+     * the seed converged (a switch follows, or the end) or at a growth-guard
+     * tail: the decision below is a flush exactly when one of these holds
+     * (the seed's own test is zp = 1, i.e. dot_r <= thr).  Otherwise the
+     * launch's other workgroups learn here that no flush is due, without
+     * waiting for the rest of the stage. */
+    const bool may_close = slot + 1 >= d.W || k + 1 >= d.cap || dot_r <= thr || sh_spike;
+    if (PUB && !may_close && tid == 0) mail_put(&st->mail_f, 0ull);
     for (int w = HB_BLOCK / 2; w > 0; w >>= 1) {
         if (tid < w) {
             const double z2 = sh_zp[tid + w];
@@ -1089,21 +1110,33 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         __syncthreads();
     }
     if (tid == 0) {
-        st->stop_count += sh_nc[0];
-        if (sh_bad) st->status |= HB_ST_BREAKDOWN_SHIFT;
-        if (sh_j[0] >= 0) st->max_sigma = sh_j[0];
-        int sw = 0;
-        if (d.stop[s] && st->stop_count < M) {
-            if (st->max_sigma >= 0) sw = 1;
-            else st->status |= HB_ST_NO_SWITCH_TARGET;
+        /* the status word takes bits from other workgroups too: atomically */
+        const int sc = in.stop_count + sh_nc[0];
+        st->stop_count = sc;
+        if (sh_bad) atomicOr(&st->status, HB_ST_BREAKDOWN_SHIFT);
+        const int ms = sh_j[0] >= 0 ? sh_j[0] : in.max_sigma;
+        st->max_sigma = ms;
+        int sw = 0, nst = 0;
+        /* the seed stopped: before (never, while it is the seed), or in the
+         * loop above, whose test for it is dot_r <= thr (zp = 1) */
+        if ((in.stop_s || dot_r <= thr) && sc < M) {
+            if (ms >= 0) {
+                sw = 1;
+            } else {
+                atomicOr(&st->status, HB_ST_NO_SWITCH_TARGET);
+                nst = 1;
+            }
         }
         sh_switch = sw;
+        sh_sc = sc;
+        sh_ms = ms;
+        sh_nst = nst | ((in.status & HB_ST_NO_SWITCH_TARGET) != 0);
     }
     __syncthreads();
 
     const int do_switch = sh_switch;
     if (do_switch) {
-        const int m = st->max_sigma;
+        const int m = sh_ms;
         const double ds = sig_s - d.sigma[m];
         if (tid == 0) st->zscale = 1.0 / (d.zeta[m] * PI(d, m, k));
         for (int i = 1 + tid; i <= k; i += HB_BLOCK) {
@@ -1136,7 +1169,7 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         /* profiled solves: window steps the listed shifts were updated in,
          * for the flop count of a flush (the flush skips each chunk's steps
          * after its last update) */
-        const int Lc = len - sh_spike, cnt = st->flush_cnt;
+        const int Lc = len - sh_spike, cnt = cnt_now;
         int u = 0;
         for (int pos = tid; pos < cnt; pos += HB_BLOCK) u += min(d.act_len[pos], Lc);
         sh_nc[tid] = u;
@@ -1148,15 +1181,14 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         if (tid == 0) sh_upd = sh_nc[0];
     }
     if (tid == 0) {
-        const int done_next = (st->stop_count >= M) || (k + 1 >= d.cap) ||
-                              (st->status & HB_ST_NO_SWITCH_TARGET);
+        const int done_next = (sh_sc >= M) || (k + 1 >= d.cap) || sh_nst;
         st->kprev = k;
         st->prev_seed = s;
         if (do_switch) {
-            d.switch_k[st->n_switch] = k;
-            d.switch_to[st->n_switch] = st->max_sigma;
-            st->n_switch += 1;
-            st->seed = st->max_sigma;
+            d.switch_k[in.n_switch] = k;
+            d.switch_to[in.n_switch] = sh_ms;
+            st->n_switch = in.n_switch + 1;
+            st->seed = sh_ms;
             st->switched = 1;
         }
         const int spike = sh_spike;
@@ -1179,14 +1211,14 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
          * vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1} and x_j, p_j of every
          * listed shift, read and written once */
         double bytes = 32.0;
-        if (fl) bytes += 8.0 * (2 * len + 1) + (xonly ? 16.0 : 32.0) * st->flush_cnt;
-        st->shift_bytes += bytes * (double)d.n;
+        if (fl) bytes += 8.0 * (2 * len + 1) + (xonly ? 16.0 : 32.0) * cnt_now;
+        st->shift_bytes = in.shift_bytes + bytes * (double)d.n;
         /* multiply-adds per row and listed shift: x_j takes p_j, r_{k0-1}
          * and two window vectors per iteration it was updated in; p_j as
          * many again; a tail iteration applied on its own six more */
         if (fl && d.flush_log)
-            st->flush_fma += (2.0 * (double)sh_upd + 2.0 * st->flush_cnt) * (xonly ? 1.0 : 2.0) +
-                             (spike ? 6.0 * st->flush_cnt : 0.0);
+            st->flush_fma += (2.0 * (double)sh_upd + 2.0 * cnt_now) * (xonly ? 1.0 : 2.0) +
+                             (spike ? 6.0 * cnt_now : 0.0);
         if (d.flush_log && k < d.cap) d.flush_log[k] = fl;
     }
     if (d.growth_limit > 0.0) {
@@ -1196,13 +1228,13 @@ DEV void scalar_stage(const hb_dev &d, double dot_r, double rTr)
         if (sh_flush)
             for (int j = tid; j < M; j += HB_BLOCK) d.grow[j] = d.grow[M + j];
     }
-    if (PUB) {
+    if (PUB && may_close) {
         __syncthreads();
         if (sh_flush) {
             /* the list covers every position this iteration wrote (those of
              * shifts that stopped in the window before are stored again
              * unchanged) */
-            const int cnt = slot == 0 ? sh_cnt : st->flush_cnt;
+            const int cnt = cnt_now;
             for (int pos = tid; pos < cnt; pos += HB_BLOCK) {
                 double *c = coef + (size_t)pos * 6;
 #pragma unroll
@@ -1326,10 +1358,33 @@ DEV bool fin_seed_body(const hb_dev &d)
     __shared__ double lds[EXACT ? HB_NNZ_CAP : 1];
     __shared__ double sh[2];
     hb_state *st = d.st;
-    if (st->done) return false;
+    hb_stage_in in;
+    const int done = st->done;
+    in.k = st->k;
+    in.s = st->seed;
+    in.win_k0 = st->win_k0;
+    in.win_seedpos = st->win_seedpos;
+    in.rTr_old = st->rTr_old;
+    in.dot_zero = st->dot_zero;
+    in.dot_r_prev = st->dot_r;
+    in.qTq = st->qTq;
+    in.stop_count = st->stop_count;
+    in.max_sigma = st->max_sigma;
+    in.status = st->status;
+    in.n_switch = st->n_switch;
+    in.flush_cnt = st->flush_cnt;
+    in.shift_bytes = st->shift_bytes;
+    if (done) return false;
+    /* 1 <= k < cap while the solve runs */
+    in.ak = d.aa[in.k];
+    in.wk = d.wa[in.k];
+    in.bkm1 = d.ba[in.k - 1];
+    in.akm1 = d.aa[in.k - 1];
+    in.sig_s = d.sigma[in.s];
+    in.stop_s = d.stop[in.s];
     double dr, rr;
     if (EXACT) {
-        const double *rn = d.rring + off(st->k % (d.W + 1), d.n);
+        const double *rn = d.rring + off(in.k % (d.W + 1), d.n);
         dr = serial_dot(rn, rn, d.n, lds);
         rr = serial_dot(d.rhat, rn, d.n, lds);
     } else {
@@ -1346,7 +1401,7 @@ DEV bool fin_seed_body(const hb_dev &d)
         st->mail_a = HB_MAIL_EMPTY;
     }
     __syncthreads();
-    scalar_stage<PUB>(d, sh[0], sh[1]);
+    scalar_stage<PUB>(d, in, sh[0], sh[1]);
     return true;
 }
 
@@ -1744,6 +1799,9 @@ DEV void flush_apply(const hb_dev &d, int bx, const hb_fl &fw, double *__restric
             nbc = __builtin_amdgcn_readfirstlane(nbc);
             if (tail) nbc = Lc;   /* the tail starts from the ring slots the loop ends on */
             double x[UG], p[UG];
+            /* the first window starts from p_j = b for every shift (the
+             * initialisation stores only the seed's p) */
+            const double pb = k0 == 1 ? d.b_in[i] : 0.0;
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
@@ -1751,7 +1809,8 @@ DEV void flush_apply(const hb_dev &d, int bx, const hb_fl &fw, double *__restric
                     x[u] = __builtin_nontemporal_load(xset + off(j, n) + i);
                     /* the seed's window-start p is the snapshot (its p_s
                      * has moved on every iteration) */
-                    p[u] = __builtin_nontemporal_load(c * UG + u == sp ? d.p_snap + i : pset + off(j, n) + i);
+                    p[u] = k0 == 1 ? pb
+                                   : __builtin_nontemporal_load(c * UG + u == sp ? d.p_snap + i : pset + off(j, n) + i);
                 } else {
                     x[u] = 0.0;
                     p[u] = 0.0;
@@ -1866,18 +1925,35 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, const hb_fl &fw, double *__rest
              * coefficient addresses, stay scalar */
             nbc = __builtin_amdgcn_readfirstlane(nbc);
             double x0[UG], x1[UG];
+            if (k0 == 1) {
+                /* the first window starts from p_j = b for every shift (the
+                 * initialisation stores only the seed's p): one load */
+                const double b0 = d.b_in[i0], b1 = d.b_in[i1];
 #pragma unroll
-            for (int u = 0; u < UG; ++u) {
-                if (u < nu) {
-                    const int j = fl[c * UG + u];
-                    const double *pj = c * UG + u == sp ? d.p_snap : pset + off(j, n);
-                    x0[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i0),
-                                          __builtin_nontemporal_load(xset + off(j, n) + i0));
-                    x1[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i1),
-                                          __builtin_nontemporal_load(xset + off(j, n) + i1));
-                } else {
-                    x0[u] = 0.0;
-                    x1[u] = 0.0;
+                for (int u = 0; u < UG; ++u) {
+                    if (u < nu) {
+                        const int j = fl[c * UG + u];
+                        x0[u] = __builtin_fma(cb[u], b0, __builtin_nontemporal_load(xset + off(j, n) + i0));
+                        x1[u] = __builtin_fma(cb[u], b1, __builtin_nontemporal_load(xset + off(j, n) + i1));
+                    } else {
+                        x0[u] = 0.0;
+                        x1[u] = 0.0;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < UG; ++u) {
+                    if (u < nu) {
+                        const int j = fl[c * UG + u];
+                        const double *pj = c * UG + u == sp ? d.p_snap : pset + off(j, n);
+                        x0[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i0),
+                                              __builtin_nontemporal_load(xset + off(j, n) + i0));
+                        x1[u] = __builtin_fma(cb[u], __builtin_nontemporal_load(pj + i1),
+                                              __builtin_nontemporal_load(xset + off(j, n) + i1));
+                    } else {
+                        x0[u] = 0.0;
+                        x1[u] = 0.0;
+                    }
                 }
             }
             int rs = (k0 - 1) % (W + 1), qs = k0 % W;
@@ -2141,11 +2217,10 @@ __global__ __launch_bounds__(HB_BLOCK) void k_flush_final(hb_dev d)
 
 /* Shift kernel with the scalar stage (default mode, fused level 1):
  * workgroup 0 runs the stage, workgroups 1..M form the combined coefficients
- * when it decides a flush, the rest are the shift_tw tiles above.  Five
- * waves per SIMD, as the shift kernel without the stage: the compiler fits
- * the three roles in 94 registers without spilling, where left alone it
- * takes 108 (four waves). */
-__global__ __launch_bounds__(HB_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_shift_stage(hb_dev d)
+ * when it decides a flush, the rest are the shift_tw tiles above.  It takes
+ * about 106 registers (four waves per SIMD); held to five waves it spills
+ * in the stage and measured 0.4 ms per 128^3 solve slower. */
+__global__ __launch_bounds__(HB_BLOCK) void k_shift_stage(hb_dev d)
 {
     if (blockIdx.x == 0) {
         /* a done solve: the launch's seed p update and flush have nothing
@@ -2170,11 +2245,15 @@ DEV void init_body(const hb_dev &d, int blk)
     const double *b = d.b_in;
     double *rhat = (double *)d.rhat;
     double part0 = 0.0, part1 = 0.0;
+    /* p_j = b: exact order replays every shift's update from p_set; the
+     * default mode reads b as the first window's p_j in its flush, so only
+     * the seed's p (which its own update moves every iteration) is stored */
+    const int j0 = d.exact ? 0 : d.seed_in, j1 = d.exact ? M : d.seed_in + 1;
     for (int i = blk * HB_BLOCK + threadIdx.x; i < n; i += nb * HB_BLOCK) {
         const double bi = b[i];
         rhat[i] = bi;
         d.rring[i] = bi;
-        for (int j = 0; j < M; ++j) d.p_set[off(j, n) + i] = bi;
+        for (int j = j0; j < j1; ++j) d.p_set[off(j, n) + i] = bi;
         part0 += bi * bi;
     }
     block_sum2(part0, part1);
