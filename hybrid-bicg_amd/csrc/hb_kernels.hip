@@ -307,6 +307,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             st->mail_b = HB_MAIL_EMPTY;
             st->mail_f = HB_MAIL_EMPTY;
             st->coef_ready = 0;
+            st->pu_live = 0;
         }
         /* k >= 1, seed >= 0 and switched >= 0 always hold; testing them
          * keeps their loads ahead of the branch, in one round trip with done */
@@ -868,6 +869,7 @@ DEV void fin_spmv_body(const hb_dev &d)
             st->pu_w = omega;
             st->pu_k = k;
             st->pu_seed = st->seed;
+            st->pu_live = 1;
             d.wa[k] = omega;
             st->rTr_old = rTr;
             if (a1 == 0.0 || !isfinite(omega)) st->status |= HB_ST_BREAKDOWN_QTY;
@@ -2122,9 +2124,12 @@ DEV void shift_stage_tiles(const hb_dev &d, int t, int T)
     hb_state *st = d.st;
     const int n = d.n, W = d.W;
     /* written by omega~'s finalize in the seed update's launch: the stage
-     * moves k and the seed, and rescales the archives at a switch */
-    const int k = st->pu_k, s = min(max(st->pu_seed, 0), d.M - 1);
+     * moves k and the seed, and rescales the archives at a switch.  A
+     * launch after the solve ended (no finalize ran) returns before loading
+     * anything */
+    const int live = st->pu_live, k = st->pu_k, s = min(max(st->pu_seed, 0), d.M - 1);
     const double w = st->pu_w;
+    if (!live) return;
     double *ps = d.p_set + off(s, n);
     const double *rk = d.rring + off(k % (W + 1), n);
     const double *sv = d.s;

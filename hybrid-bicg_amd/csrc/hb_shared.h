@@ -99,6 +99,8 @@ typedef struct hb_state {
     int pu_k, pu_seed; /* seed (written by omega~'s finalize, read by the seed p
                           update of the shift kernel that runs the scalar stage:
                           the stage rescales the archives at a switch)         */
+    int pu_live;       /* 1 once omega~'s finalize ran this iteration (SpMV 1
+                          clears it): the shift kernel's tiles have work      */
 } hb_state;
 
 /* Every pointer below addresses global device memory.  Device code sees
