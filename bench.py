@@ -438,7 +438,7 @@ def batched_leg(args, log):
     log(f"batched: {len(probs)} problems generated in {time.perf_counter() - t:.1f} s "
         f"(sum n = {sum(p.n for p in probs)}, sum M = {sum(p.M for p in probs)})")
     bt = Batch(probs, make_params(tol=1e-10, graph=1))
-    tuning = bt.autotune(iters=32)
+    tuning = bt.autotune()
     log(f"batched: autotuned {tuning}")
     bt.solve()
     torch.cuda.synchronize()

@@ -1284,6 +1284,7 @@ int shifted_lopbicg_normal(CSR_Matrix *A_loc_diag, CSR_Matrix *A_loc_offd, INFO_
 
 struct hbicg_batch {
     int nprob, device, exact, chunk, graph, W, group_cap;
+    int wmax;                  /* the largest window the rings hold */
     hipStream_t stream;
     int own_stream;
     hbicg_plan **plans;
@@ -1350,7 +1351,7 @@ static int batch_layout(hbicg_batch *bt, int W, int group_cap)
     int rc = 0, *tab = NULL;
     size_t tiles = 0;
     if (!bt->exact) group_cap = 0;
-    if (W > (bt->exact ? HB_WMAX_EXACT : HB_WBATCH)) return HBICG_E_ARG;
+    if (W < 1 || W > bt->wmax) return HBICG_E_ARG;
     for (int p = 0; p < bt->nprob; ++p) {
         const hbicg_plan *pl = bt->plans[p];
         const int G = group_cap > 0 && group_cap < pl->M ? group_cap : pl->M;
@@ -1426,7 +1427,18 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     bt->hdevs = (hb_dev *)calloc((size_t)nprob, sizeof(hb_dev));
     if (!bt->plans || !bt->hdevs) { rc = HBICG_E_NOMEM; goto fail; }
     size_t nsp = 0, nvec = 0, ncoef = 0;
-    const int wmax = bt->exact ? HB_WMAX_EXACT : HB_WBATCH;
+    /* ring storage for the largest window the autotuner may pick: halved
+     * while the rings of all problems (2W+1 n-vectors each) would take more
+     * than a quarter of the free device memory */
+    int wmax = bt->exact ? HB_WMAX_EXACT : HB_WBATCH;
+    if (!bt->exact) {
+        size_t free_b = 0, total_b = 0;
+        double rows = 0.0;
+        for (int q = 0; q < nprob; ++q) rows += (double)A[q].n;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+        while (wmax > 16 && (2.0 * wmax + 1.0) * rows * 8.0 > 0.25 * (double)free_b) wmax /= 2;
+    }
+    bt->wmax = wmax;
     for (int q = 0; q < nprob; ++q) {
         bt->plans[q] = plan_create_v(&A[q], a_on_device, M[q], p, bt->stream, 1, wmax, wmax, p->value_index, 1);
         if (!bt->plans[q]) { rc = g_errno ? g_errno : HBICG_E_HIP; goto fail; }
@@ -1512,8 +1524,11 @@ hbicg_batch *hbicg_batch_create(const hbicg_csr *A, int a_on_device, int nprob, 
     bt->B.n_vec_tiles = (int)nvec;
     bt->B.t_coef = bt->t_coef;
     bt->B.n_coef_tiles = (int)ncoef;
-    if ((rc = batch_layout(bt, p->window > 0 && p->window < wmax ? p->window : wmax, 0)) != 0)
-        goto fail;
+    {
+        const int wdef = bt->exact || wmax < HB_WBATCH_DEF ? wmax : HB_WBATCH_DEF;
+        if ((rc = batch_layout(bt, p->window > 0 && p->window < wmax ? p->window : wdef, 0)) != 0)
+            goto fail;
+    }
     free(tab);
     return bt;
 fail:
@@ -1689,9 +1704,12 @@ int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *
                          const int *seed, double *const *x, int iters, hbicg_tuning *best)
 {
     /* exact order: window replay with (window, shifts per workgroup row);
-     * default mode: the combined-coefficient flush, window only */
+     * default mode: the combined-coefficient flush, window only (those the
+     * rings hold).  A window's flushes cost its trial only once it reaches
+     * them: time the default mode's candidates over whole solves (iters at
+     * least the solve's length) */
     static const int windows_exact[] = {4, 8, 16}, groups_exact[] = {0, 16, 4};
-    static const int windows_cc[] = {16, 32, 64}, groups_cc[] = {0};
+    static const int windows_cc[] = {32, 64, 128}, groups_cc[] = {0};
     const int *windows = bt->exact ? windows_exact : windows_cc;
     const int *groups = bt->exact ? groups_exact : groups_cc;
     const size_t nw = 3, ng = bt->exact ? 3 : 1;
@@ -1701,6 +1719,7 @@ int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *
     int bw = bt->W, bg = bt->group_cap;
     for (size_t wi = 0; wi < nw; ++wi)
         for (size_t gi = 0; gi < ng; ++gi) {
+            if (windows[wi] > bt->wmax) continue;
             if ((rc = batch_layout(bt, windows[wi], groups[gi])) != 0) return rc;
             double ms = 0.0;
             for (int rep = 0; rep < 2; ++rep) {   /* first run warms the graph */

@@ -15,7 +15,9 @@ extern "C" {
 #define HB_ROWS_CAP     256   /* rows per SpMV workgroup                       */
 #define HB_WMAX         1024  /* largest shift-update window                   */
 #define HB_WAUTO        512   /* default window of static/dynamic, memory permitting */
-#define HB_WBATCH       64    /* largest window of the batched solver          */
+#define HB_WBATCH       128   /* largest window of the batched solver (less where the
+                                 rings of all its problems would not fit)      */
+#define HB_WBATCH_DEF   64    /* the batched solver's window before autotuning    */
 #define HB_WMAX_EXACT   16    /* largest window of the exact-order replay      */
 #define HB_UG           16    /* shifts per chunk of the combined-coefficient flush */
 #define HB_VDICT        256   /* largest value table of the value-indexed SpMV */

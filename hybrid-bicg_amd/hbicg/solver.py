@@ -341,7 +341,9 @@ class Batch:
         _lib.check(rc, "hbicg_batch_solve")
         return k, xs, ms.value
 
-    def autotune(self, iters=32):
+    def autotune(self, iters=2000):
+        """hbicg_batch_autotune: each candidate layout timed over `iters`
+        iterations (the default covers whole solves, flushes included)."""
         xs = self.new_x()
         b, s, x = self._ptrs(xs)
         t = _lib.Tuning()

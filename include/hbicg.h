@@ -302,8 +302,12 @@ int  hbicg_csr_load_device(const char *filename, hbicg_csr *A, INFO_Matrix *info
 void hbicg_csr_free_device(hbicg_csr *A);
 
 /* Times candidate layouts on the batch for `iters` iterations each and keeps
- * the fastest (x is overwritten): windows 16/32/64 in the default mode,
- * (window 4/8/16) x (shifts per workgroup all/16/4) in exact order. */
+ * the fastest (x is overwritten): windows 32/64/128 in the default mode
+ * (those the batch's rings hold: at creation the largest window is halved
+ * from 128 while the rings of all problems would take over a quarter of the
+ * free device memory; the window before tuning is 64; a window's flushes
+ * only count once a trial reaches them, so give `iters` at least the solve's
+ * length), (window 4/8/16) x (shifts per workgroup all/16/4) in exact order. */
 int hbicg_batch_autotune(hbicg_batch *bt, const double *const *b, const double *const *sigma,
                          const int *seed, double *const *x, int iters, hbicg_tuning *best);
 

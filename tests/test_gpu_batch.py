@@ -30,10 +30,10 @@ def test_batch_equals_plain_bitwise(small_batch):
     bt = Batch(small_batch, params)
     k, xs, ms = bt.solve()
     for i, p in enumerate(small_batch):
-        # the batch's default window is 64 (the plan's default is longer)
-        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1, window=64))
-        kp, xp, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, p.seed)
+        # the plain plan at the batch's window (the largest its rings hold)
         rep, stop, sw = bt.report(i)
+        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1, window=rep["window"]))
+        kp, xp, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, p.seed)
         prep, pstop, psw = pl.report()
         assert k[i] == kp, (i, p.name)
         assert np.array_equal(stop, pstop)
@@ -65,8 +65,8 @@ def _rel(a, b):
 
 
 def test_autotune_keeps_results(small_batch):
-    """Tuning picks a window (default mode: 16/32/64, combined-coefficient
-    flush) and keeps the result.  Without a seed switch, iteration counts and
+    """Tuning picks a window (default mode: 32/64/128, combined-coefficient
+    flush; trials over whole solves) and keeps the result.  Without a seed switch, iteration counts and
     stop iterations come from the seed and the scalar recurrences, which no
     window changes, and x_j move only by roundings of the regrouped window
     sums; after a switch the new seed continues from its windowed p_j, so
@@ -75,8 +75,8 @@ def test_autotune_keeps_results(small_batch):
     bt = Batch(small_batch, make_params(tol=1e-10, graph=1))
     k0, x0, _ = bt.solve()
     rep0 = [bt.report(i) for i in range(len(small_batch))]
-    best = bt.autotune(iters=16)
-    assert best["window"] in (16, 32, 64) and best["group_cap"] == 0
+    best = bt.autotune()
+    assert best["window"] in (32, 64, 128) and best["group_cap"] == 0
     assert best["ms_per_iter"] > 0
     k1, x1, _ = bt.solve()
     for i, (a, b) in enumerate(zip(x0, x1)):
