@@ -548,36 +548,25 @@ def main():
     M, n = prob.M, prob.n
     graph = 1 if args.variant == "dynamic" else 0
     window = args.window if args.window > 0 else (1 if args.variant == "normal" else 0)
+    # x_zero: the solve starts every x_j from 0 without reading x (x_0 = 0,
+    # as the reference's driver sets it, src/main.c:144-146), so a step is
+    # the solve alone and x needs no clearing between steps
     params = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=graph,
-                         value_index=bool(args.value_index), fused=args.fused)
+                         value_index=bool(args.value_index), fused=args.fused, x_zero=True)
     plan = Plan(prob.A, M, params, device=dev)
     b = torch.as_tensor(prob.b).to(dev)
-    x = torch.zeros((M, n), dtype=torch.float64, device=dev)
+    x = torch.empty((M, n), dtype=torch.float64, device=dev)
     for _ in range(args.warmup):
-        x.zero_()
         plan.solve(b, prob.sigma, prob.seed, x)
-    # x_j = 0 is an input, like b: every timed step solves into its own
-    # zeroed x, made before the timed region (the reference's driver zeroes
-    # x_loc_set before its solver call, src/main.c:144-146, and times inside
-    # the solver).  Past XPOOL steps a buffer is zeroed again inside the
-    # timed region (1 GB memset per step).
-    XPOOL = 16
-    xs = [x] + [torch.zeros_like(x) for _ in range(min(args.steps, XPOOL) - 1)]
-    x.zero_()
     torch.cuda.synchronize()
     log("warmup done")
     t0 = time.perf_counter()
     ks = []
-    for i in range(args.steps):
-        xi = xs[i % len(xs)]
-        if i >= len(xs):
-            xi.zero_()
-        k, _, rep = plan.solve(b, prob.sigma, prob.seed, xi)
+    for _ in range(args.steps):
+        k, _, rep = plan.solve(b, prob.sigma, prob.seed, x)
         ks.append(k)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    x = xs[(args.steps - 1) % len(xs)]
-    del xs
     ms_per_step = elapsed / args.steps * 1e3
     value = M * args.steps / elapsed
 
@@ -593,22 +582,16 @@ def main():
     if value_table or not args.value_index:
         gplan = Plan(prob.A, M, make_params(tol=prob.tol, max_iter=prob.max_iter, window=window,
                                             graph=graph, value_index=not value_table,
-                                            fused=args.fused), device=dev)
-        xg = torch.zeros_like(x)
+                                            fused=args.fused, x_zero=True), device=dev)
+        xg = torch.empty_like(x)
         gplan.solve(b, prob.sigma, prob.seed, xg)
-        # timed like the headline: each step into its own zeroed x
-        xgs = [torch.zeros_like(x) for _ in range(min(args.steps, XPOOL))]
         torch.cuda.synchronize()
         tg = time.perf_counter()
-        for i in range(args.steps):
-            xg = xgs[i % len(xgs)]
-            if i >= len(xgs):
-                xg.zero_()
+        for _ in range(args.steps):
             gplan.solve(b, prob.sigma, prob.seed, xg)
         torch.cuda.synchronize()
         tg = (time.perf_counter() - tg) / args.steps
         same = bool(torch.equal(xg, x))
-        del xgs
         grep = gplan.report()[0]
         general = {"ms_per_step": round(tg * 1e3, 3), "value": round(M / tg, 3), "x_bitwise_equal": same,
                    "value_index": int(not value_table), "value_table": grep["value_table"],
@@ -634,7 +617,8 @@ def main():
     kernels, roofline, flush_roofline = {}, None, None
     if not args.no_profile:
         pparams = make_params(tol=prob.tol, max_iter=prob.max_iter, window=window, graph=0,
-                              profile=True, value_index=bool(args.value_index), fused=args.fused)
+                              profile=True, value_index=bool(args.value_index), fused=args.fused,
+                              x_zero=True)
         pplan = Plan(prob.A, M, pparams, device=dev)
         xp = torch.zeros_like(x)
         pplan.solve(b, prob.sigma, prob.seed, xp)

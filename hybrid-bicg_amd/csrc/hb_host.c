@@ -97,7 +97,7 @@ static double now_ms(void)
 
 /* ---- plan ------------------------------------------------------------------ */
 struct hbicg_plan {
-    int n, M, W, cap, exact, graph, chunk, upload_x, device, profile, vec2, want_trace;
+    int n, M, W, cap, exact, graph, chunk, upload_x, x_zero, device, profile, vec2, want_trace;
     double tol, growth_limit;
     unsigned nz;
     hipStream_t stream;
@@ -594,6 +594,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     pl->graph = p->graph < 0 ? (variant == 2) : (p->graph ? 1 : 0);
     pl->chunk = p->chunk > 0 ? p->chunk : 16;
     pl->upload_x = p->upload_x;
+    pl->x_zero = p->x_zero != 0;
     pl->profile = p->profile ? 1 : 0;
     if (pl->profile) pl->graph = 0;
     if (stream) {
@@ -699,6 +700,7 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     d->exact = pl->exact;
     d->vec_grid = vec_grid;
     d->shift_gx = gx; d->shift_gy = gy; d->shift_group = G; d->vec2 = vec2;
+    d->x_zero = p->x_zero != 0;
     pl->vec2 = vec2;
     d->tol = pl->tol;
     d->growth_limit = pl->growth_limit;
@@ -957,8 +959,13 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
             if (rc) return rc;
         }
         dx = pl->x_host_buf;
-        if (pl->upload_x) HIPCK(hipMemcpyAsync(dx, x_set, xb, hipMemcpyHostToDevice, pl->stream));
-        else HIPCK(hipMemsetAsync(dx, 0, xb, pl->stream));
+        if (pl->x_zero) {
+            /* nothing to move: the solve writes every x_j before reading it */
+        } else if (pl->upload_x) {
+            HIPCK(hipMemcpyAsync(dx, x_set, xb, hipMemcpyHostToDevice, pl->stream));
+        } else {
+            HIPCK(hipMemsetAsync(dx, 0, xb, pl->stream));
+        }
         HIPCK(hipMemcpyAsync(pl->b_buf, b, nb, hipMemcpyHostToDevice, pl->stream));
         db = pl->b_buf;
     }

@@ -1210,10 +1210,16 @@ DEV void scalar_stage(const hb_dev &d, const hb_stage_in &in, double dot_r, doub
         st->k = k + 1;
         st->done = done_next;
         /* seed p update: p, r, s in, p out; at a flush also the window
-         * vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1} and x_j, p_j of every
-         * listed shift, read and written once */
+         * vectors r_{k0-1..k0+L-1}, q_{k0..k0+L-1} and, per listed shift,
+         * x_j out and in (not read in the first window with x_zero), p_j in
+         * (b once instead in the first window) and p_j out (not in the
+         * last, x-only flush) */
         double bytes = 32.0;
-        if (fl) bytes += 8.0 * (2 * len + 1) + (xonly ? 16.0 : 32.0) * cnt_now;
+        if (fl) {
+            const bool first = win_k0 == 1;
+            const double per = 8.0 + (first && d.x_zero ? 0.0 : 8.0) + (first ? 0.0 : 8.0) + (xonly ? 0.0 : 8.0);
+            bytes += 8.0 * (2 * len + 1) + (first ? 8.0 : 0.0) + per * cnt_now;
+        }
         st->shift_bytes = in.shift_bytes + bytes * (double)d.n;
         /* multiply-adds per row and listed shift: x_j takes p_j, r_{k0-1}
          * and two window vectors per iteration it was updated in; p_j as
@@ -1802,13 +1808,15 @@ DEV void flush_apply(const hb_dev &d, int bx, const hb_fl &fw, double *__restric
             if (tail) nbc = Lc;   /* the tail starts from the ring slots the loop ends on */
             double x[UG], p[UG];
             /* the first window starts from p_j = b for every shift (the
-             * initialisation stores only the seed's p) */
+             * initialisation stores only the seed's p), and with x_zero from
+             * x_j = 0 */
             const double pb = k0 == 1 ? d.b_in[i] : 0.0;
+            const bool xz = k0 == 1 && d.x_zero;
 #pragma unroll
             for (int u = 0; u < UG; ++u) {
                 if (u < nu) {
                     const int j = fl[c * UG + u];
-                    x[u] = __builtin_nontemporal_load(xset + off(j, n) + i);
+                    x[u] = xz ? 0.0 : __builtin_nontemporal_load(xset + off(j, n) + i);
                     /* the seed's window-start p is the snapshot (its p_s
                      * has moved on every iteration) */
                     p[u] = k0 == 1 ? pb
@@ -1929,14 +1937,16 @@ DEV void flush_apply_x2(const hb_dev &d, int bx, const hb_fl &fw, double *__rest
             double x0[UG], x1[UG];
             if (k0 == 1) {
                 /* the first window starts from p_j = b for every shift (the
-                 * initialisation stores only the seed's p): one load */
+                 * initialisation stores only the seed's p): one load; and
+                 * with x_zero from x_j = 0, not read */
                 const double b0 = d.b_in[i0], b1 = d.b_in[i1];
+                const bool xz = d.x_zero;
 #pragma unroll
                 for (int u = 0; u < UG; ++u) {
                     if (u < nu) {
                         const int j = fl[c * UG + u];
-                        x0[u] = __builtin_fma(cb[u], b0, __builtin_nontemporal_load(xset + off(j, n) + i0));
-                        x1[u] = __builtin_fma(cb[u], b1, __builtin_nontemporal_load(xset + off(j, n) + i1));
+                        x0[u] = __builtin_fma(cb[u], b0, xz ? 0.0 : __builtin_nontemporal_load(xset + off(j, n) + i0));
+                        x1[u] = __builtin_fma(cb[u], b1, xz ? 0.0 : __builtin_nontemporal_load(xset + off(j, n) + i1));
                     } else {
                         x0[u] = 0.0;
                         x1[u] = 0.0;
@@ -2254,11 +2264,16 @@ DEV void init_body(const hb_dev &d, int blk)
      * default mode reads b as the first window's p_j in its flush, so only
      * the seed's p (which its own update moves every iteration) is stored */
     const int j0 = d.exact ? 0 : d.seed_in, j1 = d.exact ? M : d.seed_in + 1;
+    /* x_zero: exact order reads x_j every iteration, so it starts them at 0
+     * here; the default mode's first flush writes them without reading */
+    const bool xz = d.exact && d.x_zero;
     for (int i = blk * HB_BLOCK + threadIdx.x; i < n; i += nb * HB_BLOCK) {
         const double bi = b[i];
         rhat[i] = bi;
         d.rring[i] = bi;
         for (int j = j0; j < j1; ++j) d.p_set[off(j, n) + i] = bi;
+        if (xz)
+            for (int j = 0; j < M; ++j) d.x_set[off(j, n) + i] = 0.0;
         part0 += bi * bi;
     }
     block_sum2(part0, part1);

@@ -130,6 +130,8 @@ typedef struct hb_dev {
     int shift_gx, shift_gy;    /* grid of the shift kernel                 */
     int shift_group;           /* shifts per shift-kernel workgroup row    */
     int vec2;                  /* 1: shift kernel moves 2 doubles per access (n even) */
+    int x_zero;                /* 1: x_set starts from 0 and is never read before its
+                                  first write (params.x_zero)                */
     double tol;
     double growth_limit;       /* > 0: combined-window growth that closes a window (tail) */
     HB_G const double   *val;

@@ -47,7 +47,7 @@ class Params(C.Structure):
                 ("device", C.c_int), ("exit_on_error", C.c_int), ("upload_x", C.c_int),
                 ("profile", C.c_int), ("growth_limit", C.c_double), ("trace", C.c_int),
                 ("value_index", C.c_int), ("q_mode", C.c_int), ("fused", C.c_int),
-                ("panel_cols", C.c_int)]
+                ("panel_cols", C.c_int), ("x_zero", C.c_int)]
 
 
 class KernelStat(C.Structure):
@@ -106,6 +106,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libhbicg.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    # torch carries its own HIP runtime under the same soname
+    # (libamdhip64.so.7) as the one the library links: whichever is loaded
+    # first serves the whole process, and torch's device initialisation
+    # fails on the other one ("No HIP GPUs are available").  Load torch's
+    # first, when torch is installed, so that both work in one process.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     sig = [C.POINTER(CSR_Matrix), C.POINTER(CSR_Matrix), C.POINTER(INFO_Matrix), _dp, _dp, _dp,
            C.c_int, C.c_int]

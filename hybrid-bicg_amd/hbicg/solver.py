@@ -20,7 +20,7 @@ VARIANTS = ("normal", "static", "dynamic")
 def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1, chunk=16,
                 device=0, exit_on_error=False, upload_x=True, profile=False,
                 growth_limit=1e8, trace=False, value_index=True, q_mode=0, fused=True,
-                panel_cols=0) -> _lib.Params:
+                panel_cols=0, x_zero=False) -> _lib.Params:
     p = _lib.default_params()
     p.tol = float(tol)
     p.max_iter = int(max_iter)
@@ -38,6 +38,7 @@ def make_params(tol=1e-12, max_iter=2000, window=0, exact_order=False, graph=-1,
     p.q_mode = int(q_mode)
     p.fused = int(fused) if not isinstance(fused, bool) else int(fused)
     p.panel_cols = int(panel_cols)
+    p.x_zero = int(bool(x_zero))
     return p
 
 

@@ -103,6 +103,12 @@ typedef struct hbicg_params {
                             off).  Exact order is bitwise unchanged (one lane sums a row
                             in CSR order across the passes); the default order sums a
                             row's panels in turn, a deterministic regrouping           */
+    int    x_zero;       /* 1: x_set's incoming contents are never read -- every x_j
+                            starts from 0, so the caller need not clear it (the default
+                            mode's first window flush writes x_j without reading it,
+                            exact order clears x_set first; a host x_loc_set is neither
+                            uploaded nor cleared).  0 (default): device plans accumulate
+                            onto x_set; the drop-ins upload x_loc_set (upload_x)      */
 } hbicg_params;
 
 typedef struct hbicg_report {

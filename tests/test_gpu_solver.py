@@ -161,6 +161,37 @@ def test_fused_launches_bitwise_tail_path(c0, seed):
         assert not (a.report["status"] & 16)
 
 
+@pytest.mark.parametrize("seed", [0, 8])
+def test_x_zero_never_reads_x(c0, seed):
+    """params.x_zero: x_set's incoming contents are never read.  A device
+    plan handed an x_set full of NaN returns bit for bit what it returns
+    from a zeroed one: in the default mode with the window of 512 (the
+    first window's flush writes x_j) and window 1 (a flush every
+    iteration), and in exact order (x_j cleared at the start); seed 8
+    switches at iteration 102.  The host drop-in with x_zero neither uploads
+    nor clears x_loc_set: NaN in, the zeroed run's x out."""
+    import torch
+
+    from hbicg import Plan
+    b = torch.as_tensor(c0.b).cuda()
+    for exact in (False, True):
+        for window in (0, 1):
+            print(f"case x_zero seed={seed} exact={exact} window={window}", flush=True)
+            mk = lambda xz: make_params(tol=c0.tol, exact_order=exact, window=window, graph=1, x_zero=xz)
+            pl0 = Plan(c0.A, c0.M, mk(False))
+            k0, x0, _ = pl0.solve(b, c0.sigma, seed)
+            pl0.close()
+            pl1 = Plan(c0.A, c0.M, mk(True))
+            xn = torch.full((c0.M, c0.A.n), float("nan"), dtype=torch.float64, device="cuda")
+            k1, x1, rep = pl1.solve(b, c0.sigma, seed, x=xn)
+            pl1.close()
+            assert k1 == k0 and torch.equal(x1, x0), (exact, window)
+    ref = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", make_params(tol=c0.tol))
+    got = dropin_solve(c0.A, c0.b, c0.sigma, seed, "dynamic", make_params(tol=c0.tol, x_zero=True),
+                       x0=np.full((c0.M, c0.A.n), np.nan))
+    assert got.k == ref.k and np.array_equal(got.x, ref.x)
+
+
 @pytest.mark.parametrize("seed", [0, 8, 15])
 def test_against_golden_fixture(c0, seed):
     """GPU results against the committed oracle fixture
