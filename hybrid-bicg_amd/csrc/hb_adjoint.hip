@@ -92,7 +92,8 @@ __global__ __launch_bounds__(TB) void k_scan_final(const unsigned *__restrict__ 
 
 /* one thread per row of A; entries land in their column's segment of A^T
  * in arbitrary order, each with its source entry index e (sorted
- * afterwards) */
+ * afterwards).  VALS false: the pattern only (tval, val unused). */
+template <bool VALS>
 __global__ __launch_bounds__(TB) void k_scatter(const unsigned *__restrict__ ptr,
                                                 const unsigned *__restrict__ col,
                                                 const double *__restrict__ val, int n,
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(TB) void k_scatter(const unsigned *__restrict__ ptr
     for (unsigned e = ptr[i]; e < ptr[i + 1]; ++e) {
         const unsigned pos = atomicAdd(&fill[col[e]], 1u);
         tcol[pos] = (unsigned)i;
-        tval[pos] = val[e];
+        if (VALS) tval[pos] = val[e];
         tsrc[pos] = e;
     }
 }
@@ -115,7 +116,9 @@ __global__ __launch_bounds__(TB) void k_scatter(const unsigned *__restrict__ ptr
  * are numbered row by row, so the row of A^T lists its entries by ascending
  * source row and, for duplicate entries of one row, in their CSR order --
  * the order a stable transpose produces, whatever order the scatter's
- * atomics placed them in */
+ * atomics placed them in.  tsrc[e'] is then the entry of A that entry e' of
+ * A^T holds. */
+template <bool VALS>
 __global__ __launch_bounds__(TB) void k_sort_rows(const unsigned *__restrict__ tptr, int n,
                                                   unsigned *__restrict__ tcol,
                                                   double *__restrict__ tval,
@@ -126,18 +129,28 @@ __global__ __launch_bounds__(TB) void k_sort_rows(const unsigned *__restrict__ t
     const unsigned lo = tptr[i], hi = tptr[i + 1];
     for (unsigned a = lo + 1; a < hi; ++a) {
         const unsigned c = tcol[a], e = tsrc[a];
-        const double v = tval[a];
+        const double v = VALS ? tval[a] : 0.0;
         unsigned b = a;
         while (b > lo && tsrc[b - 1] > e) {
             tcol[b] = tcol[b - 1];
-            tval[b] = tval[b - 1];
+            if (VALS) tval[b] = tval[b - 1];
             tsrc[b] = tsrc[b - 1];
             --b;
         }
         tcol[b] = c;
-        tval[b] = v;
+        if (VALS) tval[b] = v;
         tsrc[b] = e;
     }
+}
+
+/* out[e] = val[src[e]]: A^T's values from A's through the source indices
+ * of a pattern-only transpose */
+__global__ __launch_bounds__(TB) void k_gather_vals(const double *__restrict__ val,
+                                                    const unsigned *__restrict__ src, size_t nnz,
+                                                    double *__restrict__ out)
+{
+    for (size_t e = (size_t)blockIdx.x * TB + threadIdx.x; e < nnz; e += (size_t)gridDim.x * TB)
+        out[e] = val[src[e]];
 }
 
 /* ---- gradients -------------------------------------------------------------- */
@@ -194,8 +207,13 @@ __global__ __launch_bounds__(TB) void k_grad_sigma_fin(const double *__restrict_
     if (threadIdx.x == 0) gs[k] = -(w ? w[k] * sh[0] : sh[0]);
 }
 
-/* one thread per row i: grad_val[e] = -sum_k lam_k[i] * x_k[col[e]] for the
- * row's nonzeros, shifts in ascending order */
+/* grad_val[e] = -sum_k lam_k[row(e)] * x_k[col[e]], shifts in ascending
+ * order.  One workgroup per 256 rows of A; its entries are taken 256 at a
+ * time, one per thread, each thread finding its entry's row among the
+ * tile's row pointers (LDS binary search), so the work per thread does not
+ * follow the row lengths.  KC shifts' loads of lam and x are issued
+ * together; the shifts past M in the last round are loaded clamped and
+ * left out of the sum. */
 __global__ __launch_bounds__(TB) void k_grad_vals(const unsigned *__restrict__ ptr,
                                                   const unsigned *__restrict__ col, int n, int M,
                                                   const double *__restrict__ mu,
@@ -203,15 +221,38 @@ __global__ __launch_bounds__(TB) void k_grad_vals(const unsigned *__restrict__ p
                                                   const double *__restrict__ x,
                                                   double *__restrict__ gv)
 {
-    const int i = blockIdx.x * TB + threadIdx.x;
-    if (i >= n) return;
-    const unsigned lo = ptr[i], hi = ptr[i + 1];
-    for (unsigned e = lo; e < hi; ++e) {
-        const unsigned c = col[e];
+    constexpr int KC = 16;
+    __shared__ unsigned rp[TB + 1];
+    const int r0 = blockIdx.x * TB;
+    const int R = min(TB, n - r0);
+    for (int t = threadIdx.x; t <= R; t += TB) rp[t] = ptr[r0 + t];
+    __syncthreads();
+    const unsigned e1 = rp[R];
+    for (unsigned e = rp[0] + threadIdx.x; e < e1; e += TB) {
+        /* row: the largest r with rp[r] <= e (empty rows share a pointer
+         * with the next row and lose to it) */
+        int lo = 0, hi = R;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rp[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        const size_t i = (size_t)(r0 + lo), c = col[e];
         double acc = 0.0;
-        for (int k = 0; k < M; ++k) {
-            const double l = w ? w[k] * mu[(size_t)k * n + i] : mu[(size_t)k * n + i];
-            acc = acc - l * x[(size_t)k * n + c];
+        for (int k0 = 0; k0 < M; k0 += KC) {
+            double l[KC], xv[KC];
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+                const int k = min(k0 + j, M - 1);
+                const double m = mu[(size_t)k * n + i];
+                l[j] = w ? w[k] * m : m;
+                xv[j] = x[(size_t)k * n + c];
+            }
+#pragma unroll
+            for (int j = 0; j < KC; ++j) {
+                const double t = acc - l[j] * xv[j];
+                acc = k0 + j < M ? t : acc;
+            }
         }
         gv[e] = acc;
     }
@@ -220,20 +261,35 @@ __global__ __launch_bounds__(TB) void k_grad_vals(const unsigned *__restrict__ p
 extern "C" {
 
 int hb_transpose(int n, unsigned nnz, const unsigned *ptr, const unsigned *col, const double *val,
-                 unsigned *tptr, unsigned *tcol, double *tval, unsigned *work, hipStream_t s)
+                 unsigned *tptr, unsigned *tcol, double *tval, unsigned *tsrc, unsigned *work,
+                 hipStream_t s)
 {
-    /* work: cnt[n] | fill[n] | bsum[ceil(n/TB)] | tsrc[nnz] */
+    /* work: cnt[n] | fill[n] | bsum[ceil(n/TB)] */
     const int nb = (n + TB - 1) / TB;
     unsigned *cnt = work, *fill = work + n, *bsum = work + 2 * (size_t)n;
-    unsigned *tsrc = bsum + nb;
     hipLaunchKernelGGL(k_zero_u, dim3(nb < 4096 ? nb : 4096), dim3(TB), 0, s, cnt, n);
     const int g = nnz ? (int)((nnz + TB - 1) / TB < 65536 ? (nnz + TB - 1) / TB : 65536) : 1;
     hipLaunchKernelGGL(k_count_cols, dim3(g), dim3(TB), 0, s, col, nnz, cnt);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(TB), 0, s, cnt, n, bsum);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(TB), 0, s, bsum, nb);
     hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(TB), 0, s, cnt, n, bsum, tptr, fill);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(TB), 0, s, ptr, col, val, n, fill, tcol, tval, tsrc);
-    hipLaunchKernelGGL(k_sort_rows, dim3(nb), dim3(TB), 0, s, tptr, n, tcol, tval, tsrc);
+    if (tval) {
+        hipLaunchKernelGGL(k_scatter<true>, dim3(nb), dim3(TB), 0, s, ptr, col, val, n, fill, tcol, tval, tsrc);
+        hipLaunchKernelGGL(k_sort_rows<true>, dim3(nb), dim3(TB), 0, s, tptr, n, tcol, tval, tsrc);
+    } else {
+        hipLaunchKernelGGL(k_scatter<false>, dim3(nb), dim3(TB), 0, s, ptr, col, val, n, fill, tcol, tval, tsrc);
+        hipLaunchKernelGGL(k_sort_rows<false>, dim3(nb), dim3(TB), 0, s, tptr, n, tcol, tval, tsrc);
+    }
+    return (int)hipGetLastError();
+}
+
+int hb_gather_vals(size_t nnz, const double *val, const unsigned *src, double *out, hipStream_t s)
+{
+    if (nnz) {
+        const size_t b = (nnz + TB - 1) / TB;
+        hipLaunchKernelGGL(k_gather_vals, dim3(b < 65536 ? (unsigned)b : 65536u), dim3(TB), 0, s, val, src,
+                           nnz, out);
+    }
     return (int)hipGetLastError();
 }
 

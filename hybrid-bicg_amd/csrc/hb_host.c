@@ -1752,10 +1752,11 @@ int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, doub
     if (!A || !A->ptr || !tptr || (A->nz && (!A->col || !A->val || !tcol || !tval)) || A->n == 0)
         return HBICG_E_ARG;
     const hipStream_t s = (hipStream_t)stream;
-    const size_t n = A->n;
-    work = (unsigned *)dalloc((2 * n + (n + HB_BLOCK - 1) / HB_BLOCK + (size_t)A->nz + 1) * sizeof(unsigned), &rc);
+    const size_t n = A->n, nb = (n + HB_BLOCK - 1) / HB_BLOCK;
+    /* cnt | fill | bsum | the source index of every entry */
+    work = (unsigned *)dalloc((2 * n + nb + (size_t)A->nz + 1) * sizeof(unsigned), &rc);
     if (rc) return rc;
-    if (hb_transpose((int)n, A->nz, A->ptr, A->col, A->val, tptr, tcol, tval, work, s) != 0) {
+    if (hb_transpose((int)n, A->nz, A->ptr, A->col, A->val, tptr, tcol, tval, work + 2 * n + nb, work, s) != 0) {
         rc = HBICG_E_HIP;
         goto fail;
     }
@@ -1765,22 +1766,61 @@ fail:
     return rc;
 }
 
+int hbicg_csr_transpose_pattern(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, unsigned *tsrc,
+                                void *stream)
+{
+    int rc = 0;
+    unsigned *work = NULL;
+    if (!A || !A->ptr || !tptr || (A->nz && (!A->col || !tcol || !tsrc)) || A->n == 0) return HBICG_E_ARG;
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t n = A->n, nb = (n + HB_BLOCK - 1) / HB_BLOCK;
+    work = (unsigned *)dalloc((2 * n + nb) * sizeof(unsigned), &rc);
+    if (rc) return rc;
+    if (hb_transpose((int)n, A->nz, A->ptr, A->col, NULL, tptr, tcol, NULL, tsrc, work, s) != 0) {
+        rc = HBICG_E_HIP;
+        goto fail;
+    }
+    HIPCK(hipStreamSynchronize(s));
+fail:
+    hipFree(work);
+    return rc;
+}
+
+int hbicg_gather_values(const double *val, const unsigned *src, unsigned nz, double *out, void *stream)
+{
+    if (nz && (!val || !src || !out)) return HBICG_E_ARG;
+    return hb_gather_vals(nz, val, src, out, (hipStream_t)stream) != 0 ? HBICG_E_HIP : 0;
+}
+
+size_t hbicg_adjoint_work_bytes(unsigned n, int M)
+{
+    const size_t nb = ((size_t)n + HB_BLOCK - 1) / HB_BLOCK;
+    return (M > 0 ? (size_t)M : 0) * (nb < 256 ? nb : 256) * sizeof(double);
+}
+
+int hbicg_adjoint_gradients_ws(const hbicg_csr *A, int M, const double *mu, const double *w,
+                               const double *x, double *grad_b, double *grad_sigma, double *grad_val,
+                               void *work, size_t work_bytes, void *stream)
+{
+    if (!A || !A->ptr || M <= 0 || !mu || !x || A->n == 0 || (grad_val && A->nz && !A->col)) return HBICG_E_ARG;
+    if (grad_sigma && (!work || work_bytes < hbicg_adjoint_work_bytes(A->n, M))) return HBICG_E_ARG;
+    return hb_adjoint_grads((int)A->n, M, A->ptr, A->col, mu, w, x, grad_b, grad_sigma, grad_val,
+                            (double *)work, (hipStream_t)stream) != 0 ? HBICG_E_HIP : 0;
+}
+
 int hbicg_adjoint_gradients(const hbicg_csr *A, int M, const double *mu, const double *w,
                             const double *x, double *grad_b, double *grad_sigma, double *grad_val,
                             void *stream)
 {
     int rc = 0;
-    double *work = NULL;
+    void *work = NULL;
     if (!A || !A->ptr || M <= 0 || !mu || !x || A->n == 0) return HBICG_E_ARG;
     const hipStream_t s = (hipStream_t)stream;
-    const int n = (int)A->n;
-    const int nb = (n + HB_BLOCK - 1) / HB_BLOCK;
-    work = (double *)dalloc((size_t)M * (nb < 256 ? nb : 256) * sizeof(double), &rc);
+    const size_t bytes = hbicg_adjoint_work_bytes(A->n, M);
+    work = dalloc(bytes, &rc);
     if (rc) return rc;
-    if (hb_adjoint_grads(n, M, A->ptr, A->col, mu, w, x, grad_b, grad_sigma, grad_val, work, s) != 0) {
-        rc = HBICG_E_HIP;
-        goto fail;
-    }
+    rc = hbicg_adjoint_gradients_ws(A, M, mu, w, x, grad_b, grad_sigma, grad_val, work, bytes, stream);
+    if (rc) goto fail;
     HIPCK(hipStreamSynchronize(s));
 fail:
     hipFree(work);

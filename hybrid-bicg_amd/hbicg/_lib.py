@@ -92,7 +92,8 @@ EXPORTS = {
                 "hbicg_batch_create",
                 "hbicg_batch_destroy", "hbicg_batch_solve", "hbicg_batch_report",
                 "hbicg_batch_true_residuals", "hbicg_batch_autotune", "hbicg_csr_transpose",
-                "hbicg_adjoint_gradients", "hbicg_coo2csr_device", "hbicg_csr_load_device",
+                "hbicg_csr_transpose_pattern", "hbicg_gather_values", "hbicg_adjoint_gradients",
+                "hbicg_adjoint_work_bytes", "hbicg_adjoint_gradients_ws", "hbicg_coo2csr_device", "hbicg_csr_load_device",
                 "hbicg_csr_free_device"],
 }
 
@@ -185,6 +186,16 @@ def load():
     L.hbicg_adjoint_gradients.argtypes = [C.POINTER(Csr), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.hbicg_adjoint_gradients.restype = C.c_int
+    L.hbicg_csr_transpose_pattern.argtypes = [C.POINTER(Csr), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.hbicg_csr_transpose_pattern.restype = C.c_int
+    L.hbicg_gather_values.argtypes = [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p]
+    L.hbicg_gather_values.restype = C.c_int
+    L.hbicg_adjoint_work_bytes.argtypes = [C.c_uint, C.c_int]
+    L.hbicg_adjoint_work_bytes.restype = C.c_size_t
+    L.hbicg_adjoint_gradients_ws.argtypes = [C.POINTER(Csr), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                             C.c_void_p]
+    L.hbicg_adjoint_gradients_ws.restype = C.c_int
     L.hbicg_coo2csr_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
                                        C.c_uint, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.hbicg_coo2csr_device.restype = C.c_int

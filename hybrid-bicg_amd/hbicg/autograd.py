@@ -55,7 +55,8 @@ def transpose(val, col, ptr):
 
 
 def gradients(val, col, ptr, mu, w, x, need_b=True, need_sigma=True, need_val=True):
-    """(grad_b, grad_sigma, grad_val) from adjoint solutions lam_k = w_k mu_k."""
+    """(grad_b, grad_sigma, grad_val) from adjoint solutions lam_k = w_k mu_k,
+    enqueued on the current stream (no allocation in the library, no sync)."""
     L = _lib.load()
     n = int(ptr.numel()) - 1
     M = int(x.shape[0])
@@ -63,26 +64,31 @@ def gradients(val, col, ptr, mu, w, x, need_b=True, need_sigma=True, need_val=Tr
     gb = torch.empty(n, dtype=torch.float64, device=dev) if need_b else None
     gs = torch.empty(M, dtype=torch.float64, device=dev) if need_sigma else None
     gv = torch.empty_like(val) if need_val else None
+    nbytes = int(L.hbicg_adjoint_work_bytes(n, M)) if need_sigma else 0
+    work = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device=dev) if need_sigma else None
     p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
     s = torch.cuda.current_stream(dev)
-    rc = L.hbicg_adjoint_gradients(C.byref(_csr(val, col, ptr)), M, p(mu), p(w), p(x), p(gb), p(gs),
-                                   p(gv), C.c_void_p(s.cuda_stream))
-    _lib.check(rc, "hbicg_adjoint_gradients")
+    rc = L.hbicg_adjoint_gradients_ws(C.byref(_csr(val, col, ptr)), M, p(mu), p(w), p(x), p(gb), p(gs),
+                                      p(gv), p(work), nbytes, C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "hbicg_adjoint_gradients_ws")
     return gb, gs, gv
 
 
-def collinear_split(g, rtol=1e-13):
+def collinear_split(g, rtol=1e-13, norms=None):
     """(w, c) with g_k = w_k c when the rows of g are collinear, else None.
-    g must have a nonzero row (the caller handles an all-zero gradient)."""
-    norms = torch.linalg.vector_norm(g, dim=1)
-    k0 = int(torch.argmax(norms))
-    c = g[k0]
+    g must have a nonzero row (the caller handles an all-zero gradient).
+    norms: the rows' 2-norms, when the caller has them.  One host sync."""
+    if norms is None:
+        norms = torch.linalg.vector_norm(g, dim=1)
+    k0 = torch.argmax(norms)
+    c = g.index_select(0, k0.view(1))[0]
     cc = torch.dot(c, c)
-    if float(cc) == 0.0:
-        raise ValueError("collinear_split: every row of g is zero")
     w = (g @ c) / cc
     resid = torch.linalg.vector_norm(g - w[:, None] * c)
-    if float(resid) <= rtol * float(torch.linalg.vector_norm(g)):
+    chk = torch.stack([cc, resid, torch.linalg.vector_norm(g)]).cpu()
+    if float(chk[0]) == 0.0:
+        raise ValueError("collinear_split: every row of g is zero")
+    if float(chk[1]) <= rtol * float(chk[2]):
         return w.contiguous(), c.contiguous()
     return None
 
@@ -122,24 +128,31 @@ def _plan_for(val, col, ptr, M, tol, max_iter):
 
 
 def _transpose_into(val, col, ptr):
-    """A^T in buffers kept per (val, col, ptr) so that the adjoint plan on
-    them can be reused; the values are rebuilt on every call."""
+    """A^T in buffers kept per (val, col, ptr), so that the adjoint plan on
+    them can be reused.  The pattern of A^T and the source entry of each of
+    its entries are built once (the pattern of a cached (col, ptr) is taken
+    as fixed, as the cached plans take it); the values are gathered from val
+    on every call."""
     key = (val.data_ptr(), col.data_ptr(), ptr.data_ptr(), int(val.numel()))
+    L = _lib.load()
+    s = torch.cuda.current_stream(val.device)
     bufs = _transposes.get(key)
     if bufs is None:
         if len(_transposes) >= 4:
             _transposes.pop(next(iter(_transposes)))
         n = int(ptr.numel()) - 1
         bufs = (torch.empty_like(val), torch.empty_like(col),
-                torch.empty(n + 1, dtype=torch.int32, device=val.device))
+                torch.empty(n + 1, dtype=torch.int32, device=val.device), torch.empty_like(col))
+        tval, tcol, tptr, tsrc = bufs
+        rc = L.hbicg_csr_transpose_pattern(C.byref(_csr(val, col, ptr)), C.c_void_p(tptr.data_ptr()),
+                                           C.c_void_p(tcol.data_ptr()), C.c_void_p(tsrc.data_ptr()),
+                                           C.c_void_p(s.cuda_stream))
+        _lib.check(rc, "hbicg_csr_transpose_pattern")
         _transposes[key] = bufs
-    tval, tcol, tptr = bufs
-    L = _lib.load()
-    s = torch.cuda.current_stream(val.device)
-    rc = L.hbicg_csr_transpose(C.byref(_csr(val, col, ptr)), C.c_void_p(tptr.data_ptr()),
-                               C.c_void_p(tcol.data_ptr()), C.c_void_p(tval.data_ptr()),
-                               C.c_void_p(s.cuda_stream))
-    _lib.check(rc, "hbicg_csr_transpose")
+    tval, tcol, tptr, tsrc = bufs
+    rc = L.hbicg_gather_values(C.c_void_p(val.data_ptr()), C.c_void_p(tsrc.data_ptr()), int(val.numel()),
+                               C.c_void_p(tval.data_ptr()), C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "hbicg_gather_values")
     return tval, tcol, tptr
 
 
@@ -173,7 +186,11 @@ class ShiftedSolve(torch.autograd.Function):
         ptr, col, sig = ctx.ptr, ctx.col, ctx.sig
         M = x.shape[0]
         gx = gx.contiguous()
-        live = torch.any(gx != 0, dim=1).cpu().numpy()
+        norms = torch.linalg.vector_norm(gx, dim=1)
+        live = norms.cpu().numpy() > 0
+        if not live.all():
+            # a row whose squares all underflow has norm 0 but is not zero
+            live = torch.any(gx != 0, dim=1).cpu().numpy()
         if not live.any():
             zero = lambda t, need: torch.zeros_like(t) if need else None
             gs = zero(torch.empty(M, dtype=ctx.sigma_dtype, device=ctx.sigma_device), ctx.needs_input_grad[2])
@@ -181,7 +198,7 @@ class ShiftedSolve(torch.autograd.Function):
                     None, None, None, None, None, None)
         tval, tcol, tptr = _transpose_into(vd, col, ptr)
         params = make_params(tol=ctx.adj_tol, max_iter=ctx.max_iter, graph=1, value_index=False)
-        split = collinear_split(gx)
+        split = collinear_split(gx, norms=norms)
         if split is not None:
             w, c = split
             plan = _plan_for(tval, tcol, tptr, M, ctx.adj_tol, ctx.max_iter)

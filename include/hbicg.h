@@ -274,6 +274,17 @@ int hbicg_batch_true_residuals(hbicg_batch *bt, int p, const double *b, const do
  * NULL.  Synchronises the stream. */
 int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, double *tval, void *stream);
 
+/* The same transpose in two steps, for a pattern that stays fixed while its
+ * values change (an optimiser updating A in place): the pattern tptr[n+1],
+ * tcol[nz] and tsrc[nz], the entry of A that each entry of A^T holds (A->val
+ * may be NULL; synchronises the stream), then, for every new set of values,
+ * tval[e] = val[tsrc[e]] with hbicg_gather_values (only enqueued on the
+ * stream; allocates nothing).  Together they give hbicg_csr_transpose's
+ * tptr, tcol and tval bit for bit. */
+int hbicg_csr_transpose_pattern(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, unsigned *tsrc,
+                                void *stream);
+int hbicg_gather_values(const double *val, const unsigned *src, unsigned nz, double *out, void *stream);
+
 /* Gradients of a loss through x_k = (A + sigma_k I)^{-1} b from the adjoint
  * solutions lam_k = w_k mu_k (w NULL: w_k = 1), all [M][n] shift-major:
  *   grad_b = sum_k lam_k,  grad_sigma_k = -lam_k . x_k,
@@ -282,6 +293,14 @@ int hbicg_csr_transpose(const hbicg_csr *A, unsigned *tptr, unsigned *tcol, doub
 int hbicg_adjoint_gradients(const hbicg_csr *A, int M, const double *mu, const double *w,
                             const double *x, double *grad_b, double *grad_sigma, double *grad_val,
                             void *stream);
+
+/* hbicg_adjoint_gradients with a caller's device workspace of work_bytes
+ * (at least hbicg_adjoint_work_bytes(n, M) when grad_sigma is wanted, else
+ * work may be NULL): allocates nothing and only enqueues on the stream. */
+size_t hbicg_adjoint_work_bytes(unsigned n, int M);
+int hbicg_adjoint_gradients_ws(const hbicg_csr *A, int M, const double *mu, const double *w,
+                               const double *x, double *grad_b, double *grad_sigma, double *grad_val,
+                               void *work, size_t work_bytes, void *stream);
 
 /* ---- Matrix Market ingest on the GPU ------------------------------------------ */
 /* COO -> CSR on the device: the arrays the reference's coo2csr builds

@@ -1,6 +1,9 @@
 """GPU checks of the hand-written adjoint (hbicg.autograd).
 
-* the device transpose equals a stable host transpose bit for bit;
+* the device transpose equals a stable host transpose bit for bit, in one
+  step and as a pattern plus per-call value gather;
+* the gradient assembly equals host sums in the kernels' order bit for bit
+  (grad_val, grad_b) and to 1e-12 (grad_sigma, tree-summed);
 * gradients w.r.t. b, every sigma_k and the nonzeros of A match central
   finite differences (h = 1e-6) to 1e-6 relative, for the collinear loss
   L = sum_k w_k c^T x_k (one shifted adjoint solve) and for a general loss
@@ -248,3 +251,96 @@ def test_transpose_with_duplicate_entries_is_stable():
         assert np.array_equal(tptr.cpu().numpy().view(np.uint32), T.ptr)
         assert np.array_equal(tcol.cpu().numpy().view(np.uint32), T.col)
         assert np.array_equal(tval.cpu().numpy(), T.val)
+
+
+def _ragged_csr(seed, n, max_len, ncols=None):
+    """CSR with empty rows, rows of up to max_len entries and duplicates."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, max_len + 1, n)
+    lens[rng.integers(0, n, n // 10)] = 0
+    rows = np.repeat(np.arange(n), lens)
+    cols = rng.integers(0, ncols or n, rows.size)
+    order = np.lexsort((cols, rows))
+    rows, cols = rows[order], cols[order]
+    ptr = np.zeros(n + 1, dtype=np.uint32)
+    ptr[1:] = np.cumsum(lens)
+    return P.CSR(ptr, cols.astype(np.uint32), rng.uniform(-1, 1, rows.size), n)
+
+
+def test_transpose_pattern_then_gather_equals_transpose():
+    """The two-step transpose (pattern and source indices once, values
+    gathered per call) gives the one-step transpose's arrays bit for bit,
+    also after the values change in place."""
+    import ctypes as C
+    from hbicg import _lib
+    from hbicg.autograd import _csr
+    L = _lib.load()
+    for Am in (P.conv_diff_3d(12, 12, 12, 6.0, 0.2), P.zipf_matrix(3000, 12.0, 200, seed=5),
+               _ragged_csr(8, 1500, 30, ncols=64)):
+        print("case transpose pattern n", Am.n, "nnz", Am.nnz, flush=True)
+        val, col, ptr = dev_csr(Am)
+        n = Am.n
+        tptr = torch.empty(n + 1, dtype=torch.int32, device="cuda")
+        tcol = torch.empty_like(col)
+        tsrc = torch.empty_like(col)
+        tval = torch.empty_like(val)
+        s = torch.cuda.current_stream()
+        rc = L.hbicg_csr_transpose_pattern(C.byref(_csr(val, col, ptr)), C.c_void_p(tptr.data_ptr()),
+                                           C.c_void_p(tcol.data_ptr()), C.c_void_p(tsrc.data_ptr()),
+                                           C.c_void_p(s.cuda_stream))
+        assert rc == 0
+        src = tsrc.cpu().numpy().view(np.uint32)
+        assert np.array_equal(np.sort(src), np.arange(Am.nnz, dtype=np.uint32))
+        for step in range(2):
+            rc = L.hbicg_gather_values(C.c_void_p(val.data_ptr()), C.c_void_p(tsrc.data_ptr()), Am.nnz,
+                                       C.c_void_p(tval.data_ptr()), C.c_void_p(s.cuda_stream))
+            assert rc == 0
+            rval, rcol, rptr = transpose(val, col, ptr)
+            assert torch.equal(rptr, tptr) and torch.equal(rcol, tcol) and torch.equal(rval, tval)
+            T = P.CSR(Am.ptr, Am.col, val.cpu().numpy(), n).transpose()
+            assert np.array_equal(tval.cpu().numpy(), T.val)
+            val.mul_(-1.5).add_(0.25)
+
+
+def test_gradient_assembly_against_host_sums():
+    """grad_val and grad_b bit for bit against host sums in the kernels'
+    order (shifts ascending, products rounded on their own), grad_sigma to
+    1e-12; the workspace entry point equals the allocating one bit for bit.
+    Ragged rows (empty ones included), n not a multiple of 256, and M = 20
+    (a partial last round of the kernel's 16 shifts), with and without w."""
+    import ctypes as C
+    from hbicg import _lib
+    from hbicg.autograd import _csr, gradients
+    L = _lib.load()
+    A = _ragged_csr(9, 3001, 25)
+    M = 20
+    rng = np.random.default_rng(10)
+    mu = rng.standard_normal((M, A.n))
+    x = rng.standard_normal((M, A.n))
+    wv = rng.uniform(0.5, 2.0, M)
+    val, col, ptr = dev_csr(A)
+    mu_d, x_d = torch.as_tensor(mu).cuda(), torch.as_tensor(x).cuda()
+    rows = np.repeat(np.arange(A.n), np.diff(A.ptr.astype(np.int64)))
+    cols = A.col.astype(np.int64)
+    for w in (None, wv):
+        print("case grads ragged n", A.n, "M", M, "w", w is not None, flush=True)
+        lam = mu if w is None else w[:, None] * mu
+        gv_ref = np.zeros(A.nnz)
+        gb_ref = np.zeros(A.n)
+        for k in range(M):
+            gv_ref = gv_ref - lam[k][rows] * x[k][cols]
+            gb_ref = gb_ref + lam[k]
+        gs_ref = -np.einsum("kn,kn->k", lam, x)
+        w_d = None if w is None else torch.as_tensor(w).cuda()
+        gb, gs, gv = gradients(val, col, ptr, mu_d, w_d, x_d)
+        torch.cuda.synchronize()
+        assert np.array_equal(gv.cpu().numpy(), gv_ref)
+        assert np.array_equal(gb.cpu().numpy(), gb_ref)
+        # tree-summed on the GPU: bounded by 1e-12 of the sum of |terms|
+        assert np.all(np.abs(gs.cpu().numpy() - gs_ref) <= 1e-12 * np.abs(lam * x).sum(axis=1))
+        ob, os_, ov = torch.empty_like(gb), torch.empty_like(gs), torch.empty_like(gv)
+        p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None
+        rc = L.hbicg_adjoint_gradients(C.byref(_csr(val, col, ptr)), M, p(mu_d), p(w_d), p(x_d), p(ob),
+                                       p(os_), p(ov), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        assert rc == 0
+        assert torch.equal(ob, gb) and torch.equal(os_, gs) and torch.equal(ov, gv)
