@@ -126,11 +126,11 @@ def parse():
     ap.add_argument("--no-powerlaw", action="store_true",
                     help="skip the 1e6-row power-law matrix leg (128 shifts)")
     ap.add_argument("--mode", default="headline",
-                    choices=["headline", "sweep", "repeat", "powerlaw", "batched"],
+                    choices=["headline", "sweep", "repeat", "powerlaw", "batched", "adjoint"],
                     help="headline: the 128^3 / 64-shift solve and its legs; sweep: shift counts "
                          "1..--sweep-max x {normal, dynamic} (src/main_seed_diff.c); repeat: one "
                          "solve of --repeat-shifts shifts timed --repeats times (src/main_repeat.c); "
-                         "powerlaw, batched: that leg alone (for counter and trace runs)")
+                         "powerlaw, batched, adjoint: that leg alone (for counter and trace runs)")
     ap.add_argument("--panel-cols", type=int, default=0,
                     help="power-law leg: SpMV column-panel width (0 auto, -1 off)")
     ap.add_argument("--pl-iters", type=int, default=0,
@@ -497,6 +497,15 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic", "config": {"workload": f"{out['problems']} problems", "mode": "batched"},
             "batched": out}))
+        return
+    if args.mode == "adjoint":
+        out = adjoint_leg(args, log)
+        tot = out["forward_ms"] + out["backward_ms"]
+        print(json.dumps({
+            "metric": METRIC, "value": round(1e3 / tot, 3), "unit": "forward+backward passes/s",
+            "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": round(tot, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": out["workload"], "mode": "adjoint"}, "adjoint": out}))
         return
     if args.mode == "powerlaw":
         out = powerlaw_leg(args, log)
