@@ -582,7 +582,7 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step")
     rep, stop_iter, switches = plan.report()
     window = rep["window"]
-    value_table, diag_tiles = rep["value_table"], rep["diag_tiles"]
+    value_table, diag_tiles, const_tiles = rep["value_table"], rep["diag_tiles"], rep["const_tiles"]
 
     # the same solve with the other SpMV format (8-byte CSR values, or the
     # value table with diagonal tiles), for comparison: the format changes
@@ -716,7 +716,8 @@ def main():
                                f"{M} shifts log-spaced [1e-3,1], relres {prob.tol:g}, seed=smallest shift",
                    "variant": args.variant, "window": window, "hipgraph": bool(graph),
                    "fused_launches": args.fused,
-                   "spmv_value_table": value_table, "spmv_diag_tiles": diag_tiles},
+                   "spmv_value_table": value_table, "spmv_diag_tiles": diag_tiles,
+                   "spmv_const_tiles": const_tiles},
         "other_spmv_format": general,
         "iterations": iters,
         "seed_switches": switches,

@@ -117,7 +117,8 @@ struct hbicg_plan {
     unsigned *dtile;     /* diagonal tiles of the value-table SpMV: records, offsets, */
     int *doff;           /* index bytes                                               */
     unsigned char *didx;
-    int n_diag_tiles;
+    double *dval;        /* constant diagonal tiles: their diagonals' values        */
+    int n_diag_tiles, n_const_tiles;
     double *pval;        /* column panels: the entries panel by panel, each      */
     unsigned *pcol, *pptr, *pblk; /* panel's row starts and tiles                */
     int own_matrix, nblk;
@@ -192,7 +193,7 @@ void hbicg_plan_destroy(hbicg_plan *pl)
     if (pl->own_matrix) { hipFree(pl->val); hipFree(pl->col); hipFree(pl->ptr); }
     void *bufs[] = {pl->blk, pl->col16, pl->cflag, pl->rhat, pl->rring, pl->qring, pl->s, pl->y, pl->p_set, pl->x_host_buf,
                     pl->b_buf, pl->sigma, pl->aa, pl->ba, pl->wa, pl->pi, pl->eta, pl->zeta,
-                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->pval, pl->pcol, pl->pptr, pl->pblk, pl->flush_log, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
+                    pl->coef, pl->ccoef, pl->grow, pl->part, pl->trace, pl->p_snap, pl->vidx, pl->vdict, pl->col16p, pl->tpad, pl->dtile, pl->doff, pl->didx, pl->dval, pl->pval, pl->pcol, pl->pptr, pl->pblk, pl->flush_log, pl->stop, pl->stop_iter, pl->act_len, pl->flist, pl->fpos,
                     pl->switch_k, pl->switch_to, pl->st};
     for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i)
         if (bufs[i]) hipFree(bufs[i]);
@@ -287,6 +288,34 @@ static int tile_diagonals(const unsigned *ptr, const unsigned *col, unsigned r0,
     return k;
 }
 
+#ifndef HB_CONST_TILES
+#define HB_CONST_TILES 1
+#endif
+
+/* A diagonal tile whose every diagonal holds one value (the constant-
+ * coefficient stencils): returns 1 and that value's table index per
+ * diagonal in cidx (0 for a diagonal no row has an entry on). */
+static int tile_constant(const unsigned *ptr, const unsigned *col, const unsigned char *idx, unsigned r0,
+                         unsigned r1, const int *off, int k, unsigned char *cidx)
+{
+    int seen[HB_DIAG_K] = {0};
+    for (int s = 0; s < k; ++s) cidx[s] = 0;
+    for (unsigned i = r0; i < r1; ++i) {
+        int s = 0;
+        for (unsigned e = ptr[i]; e < ptr[i + 1]; ++e, ++s) {
+            const long long o = (long long)col[e] - (long long)i;
+            while (off[s] != o) ++s;
+            if (!seen[s]) {
+                seen[s] = 1;
+                cidx[s] = idx[e];
+            } else if (cidx[s] != idx[e]) {
+                return 0;
+            }
+        }
+    }
+    return 1;
+}
+
 /* Matrix layouts of the value-table SpMV.  A matrix with at most HB_VDICT
  * distinct values (device matrices: a prefix is checked first) is read
  * through a table of them, one index byte per entry.  Tiles that qualify
@@ -295,6 +324,11 @@ static int tile_diagonals(const unsigned *ptr, const unsigned *col, unsigned r0,
  * KP = 8 or 16 index bytes at dtile[2t] + KP*i, K = dtile[2t+1] > 0 diagonals
  * at doff[HB_DIAG_K*t ...]; the kernel reads no columns and no row pointers
  * for them, unless KP*rows exceeds the 3 bytes per entry of the CSR form.
+ * When every diagonal tile holds one value per diagonal (tile_constant: the
+ * constant-coefficient stencils), they store, instead of index bytes, two
+ * bytes of presence bits per row (bit s: the row has an entry on diagonal
+ * s) at dtile[2t] + 2i, and their diagonals' values at dval[HB_DIAG_K*t ...];
+ * dev.cdia is then set and the SpMVs read that form.
  * The other tiles keep CSR order, their index bytes and 16-bit columns in a
  * tile-padded copy: tile t at tpad[t], a multiple of 8.  Leaves the plan
  * unchanged where the matrix has more values. */
@@ -309,6 +343,9 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     double *tmp = NULL;
     unsigned *tcol = NULL, *htp = NULL, *hdt = NULL, *hf = NULL;
     int *hoff = NULL;
+    double *hdv = NULL;
+    unsigned char *hcidx = NULL;
+    int *hk = NULL;
     unsigned char *hidx = (unsigned char *)malloc(nzs), *hvp = NULL, *hdi = NULL;
     short *hc16 = NULL, *hcp = NULL;
     if (!hidx) return HBICG_E_NOMEM;
@@ -341,7 +378,10 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     hoff = (int *)calloc((size_t)nb * HB_DIAG_K, sizeof(int));
     hf = (unsigned *)malloc((size_t)nb * 4);
     hc16 = (short *)malloc(nzs * 2);
-    if (!htp || !hdt || !hoff || !hf || !hc16) { rc = HBICG_E_NOMEM; goto out; }
+    hdv = (double *)calloc((size_t)nb * HB_DIAG_K, sizeof(double));
+    hcidx = (unsigned char *)calloc((size_t)nb * HB_DIAG_K, 1);
+    hk = (int *)calloc((size_t)nb, sizeof(int));
+    if (!htp || !hdt || !hoff || !hf || !hc16 || !hdv || !hcidx || !hk) { rc = HBICG_E_NOMEM; goto out; }
     if (hipMemcpy(hc16, pl->col16, nzs * 2, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(hf, pl->cflag, (size_t)nb * 4, hipMemcpyDeviceToHost) != hipSuccess) {
         rc = HBICG_E_HIP;
@@ -350,6 +390,10 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     size_t P = 0, D = 0;
     double vb = 0.0, cb = 0.0, pb = 0.0;
     int ndiag = 0;
+    /* which tiles go by diagonals (their index bytes would not exceed the
+     * CSR form's 3 bytes per entry), and whether every one of them holds a
+     * single value per diagonal: then all of them store presence bits */
+    int all_const = HB_CONST_TILES;
     for (int t = 0; t < nb; ++t) {
         const unsigned r0 = hblk[t], r1 = hblk[t + 1];
         const size_t R = r1 - r0, N = hptr[r1] - hptr[r0];
@@ -357,8 +401,22 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
          * [0, n): the kernel forms it in int */
         const int dia_ok = nd < HB_VDICT && (long long)pl->n + HB_ROWS_CAP < (long long)INT_MAX;
         int k = dia_ok ? tile_diagonals(hptr, hcol, r0, r1, hoff + (size_t)HB_DIAG_K * t) : 0;
-        const size_t KP = k <= 8 ? 8 : 16;
-        if (k > 0 && KP * R > 3 * N) k = 0;
+        if (k > 0 && (size_t)(k <= 8 ? 8 : 16) * R > 3 * N) k = 0;
+        hk[t] = k;
+        if (k > 0) {
+            ++ndiag;
+            if (all_const && !tile_constant(hptr, hcol, hidx, r0, r1, hoff + (size_t)HB_DIAG_K * t, k,
+                                            hcidx + (size_t)HB_DIAG_K * t))
+                all_const = 0;
+        }
+    }
+    if (!ndiag) all_const = 0;
+    for (int t = 0; t < nb; ++t) {
+        const unsigned r0 = hblk[t], r1 = hblk[t + 1];
+        const size_t R = r1 - r0, N = hptr[r1] - hptr[r0];
+        const int k = hk[t];
+        /* bytes per row: index bytes, or two bytes of presence bits */
+        const size_t KP = all_const ? 2 : (k <= 8 ? 8 : 16);
         htp[t] = (unsigned)P;
         if (k > 0) {
             D = (D + 15) & ~(size_t)15;
@@ -366,8 +424,12 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
             hdt[2 * t + 1] = (unsigned)k;
             D += KP * R;
             vb += (double)(KP * R) + 4.0 * k + 8.0;
+            if (all_const) {
+                for (int s2 = 0; s2 < k; ++s2)
+                    hdv[(size_t)HB_DIAG_K * t + s2] = dict[hcidx[(size_t)HB_DIAG_K * t + s2]];
+                vb += 8.0 * k;
+            }
             pb += 8.0;
-            ++ndiag;
         } else {
             memset(hoff + (size_t)HB_DIAG_K * t, 0, HB_DIAG_K * sizeof(int));
             P += (N + 7) & ~(size_t)7;
@@ -385,7 +447,21 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     memset(hdi, HB_DIAG_NONE, D ? D : 1);
     for (int t = 0; t < nb; ++t) {
         const unsigned r0 = hblk[t], r1 = hblk[t + 1];
-        if (hdt[2 * t + 1]) {
+        if (hdt[2 * t + 1] && all_const) {
+            const int *off = hoff + (size_t)HB_DIAG_K * t;
+            for (unsigned i = r0; i < r1; ++i) {
+                unsigned m = 0;
+                int s = 0;
+                for (unsigned e = hptr[i]; e < hptr[i + 1]; ++e, ++s) {
+                    const long long o = (long long)hcol[e] - (long long)i;
+                    while (off[s] != o) ++s;
+                    m |= 1u << s;
+                }
+                unsigned char *dst = hdi + hdt[2 * t] + 2 * (size_t)(i - r0);
+                dst[0] = (unsigned char)(m & 255u);
+                dst[1] = (unsigned char)(m >> 8);
+            }
+        } else if (hdt[2 * t + 1]) {
             const int *off = hoff + (size_t)HB_DIAG_K * t;
             const size_t KP = hdt[2 * t + 1] <= 8 ? 8 : 16;
             for (unsigned i = r0; i < r1; ++i) {
@@ -412,6 +488,7 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
         pl->doff = (int *)dalloc((size_t)nb * HB_DIAG_K * sizeof(int), &rc);
         pl->didx = (unsigned char *)dalloc(D, &rc);
     }
+    if (all_const) pl->dval = (double *)dalloc((size_t)nb * HB_DIAG_K * sizeof(double), &rc);
     if (rc) goto out;
     if (hipMemcpy(pl->vidx, hvp, P, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(pl->col16p, hcp, P * 2, hipMemcpyHostToDevice) != hipSuccess ||
@@ -419,7 +496,9 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
         hipMemcpy(pl->vdict, dict, (size_t)nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
         (ndiag && (hipMemcpy(pl->dtile, hdt, (size_t)nb * 8, hipMemcpyHostToDevice) != hipSuccess ||
                    hipMemcpy(pl->doff, hoff, (size_t)nb * HB_DIAG_K * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-                   hipMemcpy(pl->didx, hdi, D, hipMemcpyHostToDevice) != hipSuccess))) {
+                   hipMemcpy(pl->didx, hdi, D, hipMemcpyHostToDevice) != hipSuccess)) ||
+        (all_const && hipMemcpy(pl->dval, hdv, (size_t)nb * HB_DIAG_K * sizeof(double), hipMemcpyHostToDevice) !=
+                       hipSuccess)) {
         rc = HBICG_E_HIP;
         goto out;
     }
@@ -432,8 +511,11 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
         d->dtile = pl->dtile;
         d->doff = pl->doff;
         d->didx = pl->didx;
+        d->dval = pl->dval;
+        d->cdia = all_const;
     }
     pl->n_diag_tiles = ndiag;
+    pl->n_const_tiles = all_const ? ndiag : 0;
     pl->val_bytes = vb;
     pl->col_bytes = cb + 4.0 * (double)nb;
     pl->ptr_bytes = pb;
@@ -444,6 +526,9 @@ out:
     free(hdt);
     free(hf);
     free(hoff);
+    free(hdv);
+    free(hcidx);
+    free(hk);
     free(hidx);
     free(hvp);
     free(hdi);
@@ -1021,6 +1106,7 @@ int hbicg_plan_solve(hbicg_plan *pl, const double *b, const double *sigma, int s
     R->diag_tiles = pl->n_diag_tiles;
     R->q_unfused = pl->dev.q_unfused;
     R->panels = pl->dev.npanel > 1 ? pl->dev.npanel : 1;
+    R->const_tiles = pl->n_const_tiles;
     R->dot_r = st.dot_r;
     R->dot_zero = st.dot_zero;
     R->rTr = st.rTr;

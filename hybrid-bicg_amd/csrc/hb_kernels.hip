@@ -267,7 +267,8 @@ enum { SP_SEED1 = 0, SP_SEED2 = 1, SP_PLAIN = 2 };
  * exactly the order of one pass. */
 enum { PASS_ONE = 0, PASS_FIRST = 1, PASS_MID = 2, PASS_LAST = 3 };
 
-template <int MODE, bool EXACT, bool VI = false, bool UNF = false, int PASS = PASS_ONE, bool FA = false>
+template <int MODE, bool EXACT, bool VI = false, bool UNF = false, int PASS = PASS_ONE, bool FA = false,
+          bool CD = false>
 DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned *__restrict__ col, const short *__restrict__ col16,
                    const unsigned *__restrict__ cflag, const unsigned *__restrict__ ptr,
@@ -278,7 +279,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
                    const unsigned char *__restrict__ vidx = nullptr, const short *__restrict__ col16p = nullptr,
                    const unsigned *__restrict__ tpad = nullptr, const double *__restrict__ vdict = nullptr,
                    const unsigned *__restrict__ dtile = nullptr, const int *__restrict__ doff = nullptr,
-                   const unsigned char *__restrict__ didx = nullptr)
+                   const unsigned char *__restrict__ didx = nullptr, const double *__restrict__ dval = nullptr)
 {
     __shared__ double prod[VI ? 1 : HB_NNZ_CAP + 1];
     __shared__ __attribute__((aligned(16))) short scol[HB_NNZ_CAP + 8];
@@ -420,8 +421,21 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         int o[HB_DIAG_K];
 #pragma unroll
         for (int j = 0; j < HB_DIAG_K; ++j) o[j] = offs[j];
+        /* CD (every diagonal tile holds one value per diagonal): the
+         * diagonals' values come by scalar loads issued with the offsets (the
+         * same round trip), the row's two bytes of presence bits replace its
+         * index bytes, and there is no value table stage */
+        const double *__restrict__ dv = CD ? dval + (size_t)HB_DIAG_K * b : nullptr;
+        double dvr[CD ? 8 : 1];
+        if constexpr (CD) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dvr[j] = dv[j];
+        }
         unsigned w[4];
-        if (dk <= 8) {
+        if constexpr (CD) {
+            w[0] = *(const unsigned short *)(didx + dbase + 2u * (unsigned)min(tid, R - 1));
+            w[1] = w[2] = w[3] = 0u;
+        } else if (dk <= 8) {
             typedef unsigned u2 __attribute__((ext_vector_type(2)));
             const u2 v = *(const u2 *)(didx + dbase + 8u * (unsigned)min(tid, R - 1));
             w[0] = v.x; w[1] = v.y; w[2] = 0xffffffffu; w[3] = 0xffffffffu;
@@ -455,6 +469,15 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         };
         double acc = 0.0;
         auto sum8 = [&](int h) {
+            if constexpr (CD) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    /* diagonals 8..15 (tiles of more than 8) load theirs here */
+                    const double t = acc + (h ? dv[8 + j] : dvr[j]) * xv[j];
+                    acc = (w[0] >> (8 * h + j)) & 1u ? t : acc;
+                }
+                return;
+            }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const unsigned ix = (w[2 * h + j / 4] >> (8 * (j % 4))) & 255u;
@@ -465,7 +488,7 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
         if constexpr (FA) {
             __shared__ double sh_na;
             gather8_rs();
-            if (tid < d.nvdict) vd[tid] = vdv;
+            if (!CD && tid < d.nvdict) vd[tid] = vdv;
             if (tid == 0) sh_na = -mail_poll(&d.st->mail_a, d.st);
             __syncthreads();
             nalpha = sh_na;
@@ -473,8 +496,10 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
             for (int j = 0; j < 8; ++j) xv[j] = xv[j] + nalpha * sv8[j];
         } else {
             gather8(0);
-            if (tid < d.nvdict) vd[tid] = vdv;
-            __syncthreads();
+            if constexpr (!CD) {
+                if (tid < d.nvdict) vd[tid] = vdv;
+                __syncthreads();
+            }
         }
         sum8(0);
         if (dk > 8) {
@@ -713,17 +738,18 @@ DEV void spmv_body(const hb_dev &d, unsigned b, const double *__restrict__ val,
     if (tid == 0) { part[2 * b] = part0; part[2 * b + 1] = part1; }
 }
 
-template <int MODE, bool EXACT, bool VI = false, bool UNF = false>
+template <int MODE, bool EXACT, bool VI = false, bool UNF = false, bool CD = false>
 DEV void spmv_tile(const hb_dev &d, unsigned b)
 {
     if (MODE == SP_SEED1)
-        spmv_body<MODE, EXACT, VI>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, d.p_set, d.rring, nullptr,
+        spmv_body<MODE, EXACT, VI, false, PASS_ONE, false, CD>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, d.p_set, d.rring, nullptr,
                                    d.rhat, d.s, nullptr, d.part, 0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
-                                   d.doff, d.didx);
+                                   d.doff, d.didx, d.dval);
     else
-        spmv_body<MODE, EXACT, VI, UNF>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk, nullptr, d.rring,
-                                              d.s, d.rhat, d.y, d.qring, d.part2, 0.0, d.vidx, d.col16p, d.tpad,
-                                              d.vdict, d.dtile, d.doff, d.didx);
+        spmv_body<MODE, EXACT, VI, UNF, PASS_ONE, false, CD>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk,
+                                                             nullptr, d.rring, d.s, d.rhat, d.y, d.qring, d.part2,
+                                                             0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
+                                                             d.doff, d.didx, d.dval);
 }
 
 /* Workgroup b runs on XCD b % 8 (the dispatcher deals workgroups round
@@ -737,10 +763,10 @@ DEV unsigned xcd_tile(unsigned b, unsigned nb)
     return x * q + min(x, r) + b / 8;
 }
 
-template <int MODE, bool EXACT, bool VI, bool UNF = false>
+template <int MODE, bool EXACT, bool VI, bool UNF = false, bool CD = false>
 __global__ __launch_bounds__(HB_BLOCK) void k_spmv(hb_dev d)
 {
-    spmv_tile<MODE, EXACT, VI, UNF>(d, xcd_tile(blockIdx.x, gridDim.x));
+    spmv_tile<MODE, EXACT, VI, UNF, CD>(d, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 /* Pass of column panel `panel` (of d.npanel): the panel's entries of every
@@ -888,7 +914,7 @@ __global__ __launch_bounds__(HB_BLOCK) void k_fin_spmv(hb_dev d)
  * 0 finalizes alpha~ from SpMV 1's partials and publishes it; the others
  * are the tiles, which issue their loads and gathers first and read alpha~
  * after (src/solver_dynamic.c:163-184). */
-template <bool VI>
+template <bool VI, bool CD = false>
 __global__ __launch_bounds__(HB_BLOCK) void k_spmv2_fa(hb_dev d)
 {
     if (blockIdx.x == 0) {
@@ -896,10 +922,10 @@ __global__ __launch_bounds__(HB_BLOCK) void k_spmv2_fa(hb_dev d)
         return;
     }
     const unsigned b = xcd_tile(blockIdx.x - 1, gridDim.x - 1);
-    spmv_body<SP_SEED2, false, VI, false, PASS_ONE, true>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk,
+    spmv_body<SP_SEED2, false, VI, false, PASS_ONE, true, CD>(d, b, d.val, d.col, d.col16, d.cflag, d.ptr, d.blk,
                                                          nullptr, d.rring, d.s, d.rhat, d.y, d.qring, d.part2,
                                                          0.0, d.vidx, d.col16p, d.tpad, d.vdict, d.dtile,
-                                                         d.doff, d.didx);
+                                                         d.doff, d.didx, d.dval);
 }
 
 /* ---- scalar stage (one workgroup per problem, after the seed update) -------
@@ -2651,7 +2677,8 @@ static int stage_fused(const hb_dev *d, int stage, hipStream_t s)
         if (!d->fused_s) hipLaunchKernelGGL(k_fin_seed<false>, dim3(1), bl, 0, s, *d);
         break;
     case HB_STAGE_SPMV2:
-        if (d->vidx) hipLaunchKernelGGL(k_spmv2_fa<true>, dim3(d->nblk + 1), bl, 0, s, *d);
+        if (d->cdia) hipLaunchKernelGGL((k_spmv2_fa<true, true>), dim3(d->nblk + 1), bl, 0, s, *d);
+        else if (d->vidx) hipLaunchKernelGGL(k_spmv2_fa<true>, dim3(d->nblk + 1), bl, 0, s, *d);
         else hipLaunchKernelGGL(k_spmv2_fa<false>, dim3(d->nblk + 1), bl, 0, s, *d);
         break;
     case HB_STAGE_SEED:
@@ -2694,10 +2721,12 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
     switch (stage) {
     case HB_STAGE_SPMV1:
         if (d->exact) {
-            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED1, true, true>), gs, bl, 0, s, *d);
+            if (d->cdia) hipLaunchKernelGGL((k_spmv<SP_SEED1, true, true, false, true>), gs, bl, 0, s, *d);
+            else if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED1, true, true>), gs, bl, 0, s, *d);
             else hipLaunchKernelGGL((k_spmv<SP_SEED1, true, false>), gs, bl, 0, s, *d);
         } else {
-            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED1, false, true>), gs, bl, 0, s, *d);
+            if (d->cdia) hipLaunchKernelGGL((k_spmv<SP_SEED1, false, true, false, true>), gs, bl, 0, s, *d);
+            else if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED1, false, true>), gs, bl, 0, s, *d);
             else hipLaunchKernelGGL((k_spmv<SP_SEED1, false, false>), gs, bl, 0, s, *d);
         }
         break;
@@ -2708,13 +2737,16 @@ int hb_k_stage(const hb_dev *d, int stage, hipStream_t s)
     case HB_STAGE_SPMV2:
         if (d->q_unfused && !d->exact) {
             hipLaunchKernelGGL(k_qform, gv, bl, 0, s, *d);
-            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true, true>), gs, bl, 0, s, *d);
+            if (d->cdia) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true, true, true>), gs, bl, 0, s, *d);
+            else if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true, true>), gs, bl, 0, s, *d);
             else hipLaunchKernelGGL((k_spmv<SP_SEED2, false, false, true>), gs, bl, 0, s, *d);
         } else if (d->exact) {
-            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, true, true>), gs, bl, 0, s, *d);
+            if (d->cdia) hipLaunchKernelGGL((k_spmv<SP_SEED2, true, true, false, true>), gs, bl, 0, s, *d);
+            else if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, true, true>), gs, bl, 0, s, *d);
             else hipLaunchKernelGGL((k_spmv<SP_SEED2, true, false>), gs, bl, 0, s, *d);
         } else {
-            if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true>), gs, bl, 0, s, *d);
+            if (d->cdia) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true, false, true>), gs, bl, 0, s, *d);
+            else if (d->vidx) hipLaunchKernelGGL((k_spmv<SP_SEED2, false, true>), gs, bl, 0, s, *d);
             else hipLaunchKernelGGL((k_spmv<SP_SEED2, false, false>), gs, bl, 0, s, *d);
         }
         break;

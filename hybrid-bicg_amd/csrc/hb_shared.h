@@ -148,6 +148,10 @@ typedef struct hb_dev {
                                         K = 0: CSR tile.  NULL: no diagonal tiles    */
     HB_G const int      *doff;       /* [HB_DIAG_K*nblk] diagonals (column - row)    */
     HB_G const unsigned char *didx;  /* index bytes of the diagonal tiles            */
+    HB_G const double   *dval;       /* cdia: [HB_DIAG_K*nblk] each diagonal's value  */
+    int cdia;                        /* 1: every diagonal tile holds one value per
+                                        diagonal; didx holds 2 bytes of presence bits
+                                        per row instead of index bytes               */
     HB_G const double   *vdict;      /* [nvdict <= HB_VDICT] the matrix's distinct values */
     int nvdict;
     int npanel;                      /* > 1: the seed SpMVs run as this many column-panel

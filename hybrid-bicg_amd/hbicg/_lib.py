@@ -63,7 +63,8 @@ class Report(C.Structure):
                 ("dot_r", C.c_double), ("dot_zero", C.c_double), ("rTr", C.c_double),
                 ("t_setup_ms", C.c_double), ("t_solve_ms", C.c_double), ("t_total_ms", C.c_double),
                 ("window", C.c_int), ("value_table", C.c_int),
-                ("diag_tiles", C.c_int), ("q_unfused", C.c_int), ("panels", C.c_int)]
+                ("diag_tiles", C.c_int), ("q_unfused", C.c_int), ("panels", C.c_int),
+                ("const_tiles", C.c_int)]
 
 
 class Tuning(C.Structure):

@@ -129,6 +129,8 @@ typedef struct hbicg_report {
                             tile, so SpMV 2 gathered q = r - alpha s formed beforehand
                             (one gather per entry instead of two); results unchanged */
     int    panels;       /* column panels of the seed SpMVs (1: one pass)             */
+    int    const_tiles;  /* of diag_tiles, those whose every diagonal holds one value:
+                            read as presence bits per row and the diagonals' values */
 } hbicg_report;
 
 void hbicg_default_params(hbicg_params *p);

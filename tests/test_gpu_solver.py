@@ -304,13 +304,16 @@ def _rows_csr(rows, n):
     return P.CSR(ptr=ptr, col=col, val=val, n=n)
 
 
-def _mixed_tile_matrix():
+def _mixed_tile_matrix(vary=False):
     """5-point 64x64 operator edited so that its 256-row tiles take every
     path of the value-table SpMV: rows of tile 3 list their entries in
     descending column order (CSR form), one row of tile 5 has entries on 20
     more diagonals (more than 16: CSR form), every row of tile 7 has two more
-    diagonals and one of them four more (11 diagonals: 16 index bytes per
-    row), boundary rows miss diagonals."""
+    diagonals and one of them four more (11 diagonals), boundary rows miss
+    diagonals.  Every diagonal tile then holds one value per diagonal (read
+    as presence bits and the diagonals' values); vary: the main diagonal of
+    tile 9 alternates between two values, so every diagonal tile is read
+    through index bytes (16 per row in tile 7)."""
     A = P.conv_diff_2d(64, 64)
     n = A.n
     ptr = np.asarray(A.ptr, np.int64)
@@ -322,27 +325,33 @@ def _mixed_tile_matrix():
     for i in range(7 * 256, 8 * 256):
         extra = [(i - 2, -0.02), (i + 2, 0.02)] + ([(i + o, 0.01) for o in (-4, -3, 3, 4)] if i == 7 * 256 + 9 else [])
         rows[i] = sorted(rows[i] + extra, key=lambda e: e[0])
+    if vary:
+        for i in range(9 * 256, 10 * 256, 2):
+            rows[i] = [(c, v + 0.5 if c == i else v) for c, v in rows[i]]
     return _rows_csr(rows, n)
 
 
+@pytest.mark.parametrize("vary", [False, True])
 @pytest.mark.parametrize("exact", [False, True])
-def test_value_table_diagonal_tiles(exact):
-    """Tiles stored by diagonals (one index byte per row and diagonal, no
-    columns) and tiles kept in CSR form in one matrix: results bitwise equal
-    to reading the 8-byte values; in exact order also equal to the oracle."""
-    A = _mixed_tile_matrix()
+def test_value_table_diagonal_tiles(exact, vary):
+    """Tiles stored by diagonals (presence bits and one value per diagonal,
+    or one index byte per row and diagonal; no columns) and tiles kept in
+    CSR form in one matrix: results bitwise equal to reading the 8-byte
+    values; in exact order also equal to the oracle."""
+    A = _mixed_tile_matrix(vary)
     plain = P.conv_diff_2d(64, 64)
     rng = np.random.default_rng(21)
     b = rng.uniform(-1.0, 1.0, A.n)
     sigma = np.logspace(-2, 0, 9)
-    print(f"case diagonal tiles exact={exact}", flush=True)
+    print(f"case diagonal tiles exact={exact} vary={vary}", flush=True)
     mk = lambda vi: make_params(tol=1e-10, exact_order=exact, value_index=vi)
     a = dropin_solve(A, b, sigma, 0, "dynamic", mk(True))
     c = dropin_solve(A, b, sigma, 0, "dynamic", mk(False))
     full = dropin_solve(plain, b, sigma, 0, "dynamic", mk(True))
     assert a.report["value_table"] > 0 and c.report["value_table"] == 0
-    assert full.report["diag_tiles"] == 16
+    assert full.report["diag_tiles"] == 16 and full.report["const_tiles"] == 16
     assert 0 < a.report["diag_tiles"] <= 14, a.report["diag_tiles"]
+    assert a.report["const_tiles"] == (0 if vary else a.report["diag_tiles"])
     assert a.k == c.k and a.switches == c.switches
     assert np.array_equal(a.x, c.x) and np.array_equal(a.r, c.r)
     if exact:
