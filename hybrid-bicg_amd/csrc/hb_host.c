@@ -333,7 +333,7 @@ static int tile_constant(const unsigned *ptr, const unsigned *col, const unsigne
  * tile-padded copy: tile t at tpad[t], a multiple of 8.  Leaves the plan
  * unchanged where the matrix has more values. */
 static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_device, const unsigned *hptr,
-                              const unsigned *hblk)
+                              const unsigned *hblk, int allow_const)
 {
     const size_t nzs = pl->nz;
     const int nb = pl->nblk;
@@ -393,7 +393,7 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     /* which tiles go by diagonals (their index bytes would not exceed the
      * CSR form's 3 bytes per entry), and whether every one of them holds a
      * single value per diagonal: then all of them store presence bits */
-    int all_const = HB_CONST_TILES;
+    int all_const = HB_CONST_TILES && allow_const;
     for (int t = 0; t < nb; ++t) {
         const unsigned r0 = hblk[t], r1 = hblk[t + 1];
         const size_t R = r1 - r0, N = hptr[r1] - hptr[r0];
@@ -537,9 +537,6 @@ out:
     return rc;
 }
 
-/* w_alloc: ring storage for windows up to this (the batched solver's
- * autotuner); w_cap: the largest window the plan may choose; allow_vi: the
- * SpMV may read the matrix through a value table (not in batches). */
 /* Column panels of `width` columns (params.panel_cols): the entries of
  * every row split by column into panels, stored panel by panel (each
  * panel's rows in order, a row's entries in CSR order) with one row-pointer
@@ -643,6 +640,11 @@ done:
     return rc;
 }
 
+/* w_alloc: ring storage for windows up to this (the batched solver's
+ * autotuner); w_cap: the largest window the plan may choose; allow_vi: the
+ * SpMV may read the matrix through a value table (params.value_index);
+ * batched: the plan is one problem of a batch, whose launches run its tiles
+ * (index bytes, no column panels, no fused launches of its own). */
 static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
                                  const hbicg_params *p, void *stream, int variant, int w_alloc,
                                  int w_cap, int allow_vi, int batched)
@@ -841,7 +843,10 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
 
     pl->val_bytes = 8.0 * (double)pl->nz;
     pl->ptr_bytes = 4.0 * ((double)n + 1.0);
-    if (allow_vi && pl->nz > 0 && (rc = build_value_layout(pl, A, a_on_device, hptr, hblk)) != 0) goto fail;
+    /* batched plans keep index bytes: the batch's SpMV launch covers the
+     * tiles of many problems with one kernel, which reads index bytes */
+    if (allow_vi && pl->nz > 0 && (rc = build_value_layout(pl, A, a_on_device, hptr, hblk, !batched)) != 0)
+        goto fail;
     /* column panels: 8-byte CSR reads whose gathers would otherwise range
      * over an x larger than one XCD's L2 (4 MiB) */
     if (!batched && !d->vidx && p->panel_cols >= 0) {
