@@ -288,10 +288,6 @@ static int tile_diagonals(const unsigned *ptr, const unsigned *col, unsigned r0,
     return k;
 }
 
-#ifndef HB_CONST_TILES
-#define HB_CONST_TILES 1
-#endif
-
 /* A diagonal tile whose every diagonal holds one value (the constant-
  * coefficient stencils): returns 1 and that value's table index per
  * diagonal in cidx (0 for a diagonal no row has an entry on). */
@@ -393,7 +389,7 @@ static int build_value_layout(hbicg_plan *pl, const hbicg_csr *A, int a_on_devic
     /* which tiles go by diagonals (their index bytes would not exceed the
      * CSR form's 3 bytes per entry), and whether every one of them holds a
      * single value per diagonal: then all of them store presence bits */
-    int all_const = HB_CONST_TILES && allow_const;
+    int all_const = allow_const;
     for (int t = 0; t < nb; ++t) {
         const unsigned r0 = hblk[t], r1 = hblk[t + 1];
         const size_t R = r1 - r0, N = hptr[r1] - hptr[r0];
