@@ -1875,7 +1875,8 @@ fail:
 
 int hbicg_gather_values(const double *val, const unsigned *src, unsigned nz, double *out, void *stream)
 {
-    if (nz && (!val || !src || !out)) return HBICG_E_ARG;
+    if (nz == 0) return 0;
+    if (!val || !src || !out) return HBICG_E_ARG;
     return hb_gather_vals(nz, val, src, out, (hipStream_t)stream) != 0 ? HBICG_E_HIP : 0;
 }
 

@@ -237,3 +237,26 @@ def test_plan_without_device_reports_nodevice():
     h = L.hbicg_plan_create(C.byref(csr), 0, 2, C.byref(p), None)
     assert not h
     assert L.hbicg_errno() == -6
+
+
+def test_adjoint_entry_points_check_arguments_before_gpu():
+    """The two-step transpose, the value gather and the workspace gradient
+    entry point reject missing or short arguments with HBICG_E_ARG before
+    any device call; the workspace size is a pure function."""
+    L = _lib.load()
+    E_ARG = -1
+    A = _lib.Csr(None, None, None, 10, 20)
+    assert L.hbicg_csr_transpose_pattern(None, None, None, None, None) == E_ARG
+    assert L.hbicg_csr_transpose_pattern(C.byref(A), None, None, None, None) == E_ARG
+    assert L.hbicg_gather_values(None, None, 5, None, None) == E_ARG
+    assert L.hbicg_gather_values(None, None, 0, None, None) == 0     # nothing to gather
+    nb = (1000 + 255) // 256
+    assert L.hbicg_adjoint_work_bytes(1000, 3) == 3 * nb * 8
+    assert L.hbicg_adjoint_work_bytes(10 ** 6, 32) == 32 * 256 * 8
+    assert L.hbicg_adjoint_work_bytes(1000, 0) == 0
+    buf = C.create_string_buffer(8)
+    ptr = C.addressof(buf)
+    B = _lib.Csr(ptr, ptr, ptr, 1000, 1)
+    # grad_sigma wanted with a workspace smaller than the bound
+    assert L.hbicg_adjoint_gradients_ws(C.byref(B), 3, ptr, None, ptr, None, ptr, None, ptr, 8, None) == E_ARG
+    assert L.hbicg_adjoint_gradients_ws(C.byref(B), 0, ptr, None, ptr, None, None, None, None, 0, None) == E_ARG
