@@ -119,3 +119,33 @@ def test_plan_device_path_matches_dropin():
     assert k == got.k
     assert np.array_equal(x.cpu().numpy(), got.x)
     pl.close()
+
+
+def test_batch_reads_index_bytes_where_plans_read_constant_diagonals():
+    """Batched plans store their diagonal tiles with index bytes, where a
+    plain plan of a constant-coefficient stencil stores presence bits and
+    the diagonals' values: two such stencils and one matrix whose main
+    diagonal varies in one tile, each bitwise equal to its plain plan, and
+    the batch report says which SpMV form the batch read."""
+    from test_gpu_solver import _mixed_tile_matrix
+    rng = np.random.default_rng(31)
+    mats = [P.conv_diff_2d(48, 40), P.conv_diff_3d(14, 12, 16, 6.0, 0.2), _mixed_tile_matrix(vary=True)]
+    probs = [P.ShiftedProblem(f"forms-{i}", A, rng.uniform(-1.0, 1.0, A.n), np.logspace(-2, 0, 6 + i), 0, 1e-10)
+             for i, A in enumerate(mats)]
+    print("case batch diagonal forms, 3 problems", flush=True)
+    bt = Batch(probs, make_params(tol=1e-10, graph=1))
+    k, xs, _ = bt.solve()
+    for i, p in enumerate(probs):
+        rep, stop, sw = bt.report(i)
+        assert rep["value_table"] > 0 and rep["diag_tiles"] > 0 and rep["const_tiles"] == 0, (i, rep)
+        pl = Plan(p.A, p.M, make_params(tol=1e-10, graph=1, window=rep["window"]))
+        kp, xp, _ = pl.solve(torch.as_tensor(p.b).cuda(), p.sigma, p.seed)
+        prep, pstop, psw = pl.report()
+        if i == 2:
+            assert prep["const_tiles"] == 0
+        else:
+            assert prep["const_tiles"] == prep["diag_tiles"] > 0
+        assert k[i] == kp and np.array_equal(stop, pstop) and sw == psw, (i, k[i], kp)
+        assert torch.equal(xs[i], xp), i
+        pl.close()
+    bt.close()
