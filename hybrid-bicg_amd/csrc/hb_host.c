@@ -862,6 +862,9 @@ static hbicg_plan *plan_create_v(const hbicg_csr *A, int a_on_device, int M,
     if (d->fused_s) {
         d->shift_tw = hb_k_shift_stage_tiles(d->shift_gx, M, pl->device);
         if (d->shift_tw < 0) { rc = HBICG_E_HIP; goto fail; }
+        /* too many shifts for the stage's launch: the stage runs in a
+         * launch of its own (the five-launch form, bitwise the same) */
+        if (d->shift_tw == 0) d->fused_s = 0;
     }
     if (a_on_device) free(hptr);
     free(hblk);

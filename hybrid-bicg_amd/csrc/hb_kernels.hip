@@ -2806,14 +2806,16 @@ int hb_k_stage_runs(const hb_dev *d, int stage)
 int hb_k_shift_stage_tiles(int n_tiles, int M, int device)
 {
     /* as many tile workgroups as stay resident next to the stage and the
-     * coefficient workgroups, at least one per compute unit, at most one
-     * per flush tile */
+     * coefficient workgroups, at most one per flush tile; 0 when the M
+     * coefficient workgroups would leave fewer than three tiles per compute
+     * unit (the flushes inside the kernel then lack the wavefronts to hide
+     * their loads: 1024 shifts at 128^3 took 670 against 275 ms) */
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_shift_stage, HB_BLOCK, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return -1;
     long long t = (long long)per_cu * cus - 1 - M;
-    if (t < cus) t = cus;
+    if (t < n_tiles && t < 3LL * cus) return 0;
     if (t > n_tiles) t = n_tiles;
     return t > 0 ? (int)t : 1;
 }

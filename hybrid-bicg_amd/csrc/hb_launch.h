@@ -31,7 +31,9 @@ int hb_k_iteration(const hb_dev *d, hipStream_t s);
  * stages inside a neighbouring one) */
 int hb_k_stage_runs(const hb_dev *d, int stage);
 /* Tile workgroups of the shift kernel that runs the scalar stage (d->shift_tw)
- * for n_tiles flush tiles and M shifts on `device`; < 0 on a HIP error. */
+ * for n_tiles flush tiles and M shifts on `device`; 0 when M leaves too few
+ * resident tiles (the stage then keeps a launch of its own); < 0 on a HIP
+ * error. */
 int hb_k_shift_stage_tiles(int n_tiles, int M, int device);
 
 /* The kernels of an iteration, one launch at a time (per-kernel timing). */

@@ -561,3 +561,28 @@ def test_fused_launches_bitwise_gathered_q():
     assert a.k == c.k and a.switches == c.switches and a.report["stop_count"] == len(sigma)
     assert np.array_equal(a.x, c.x) and np.array_equal(a.r, c.r)
     assert not (a.report["status"] & 16)
+
+
+def test_many_shifts_take_the_five_launch_form_bitwise():
+    """1024 shifts on 32^3: the shift kernel with the scalar stage would leave
+    too few resident tiles next to its 1024 coefficient workgroups, so the
+    default (fused = 1) runs the stage in a launch of its own; results
+    bitwise equal to fused = 3 and within the reference drivers'
+    conventions (b = 1, sigma_i = (i+1) 0.01 / M, seed M/2 - 1)."""
+    import torch
+    from hbicg import Plan
+    A = P.conv_diff_3d(32, 32, 32, 6.0, 0.2)
+    M = 1024
+    sigma = (np.arange(M) + 1) * (0.01 / M)
+    b = torch.ones(A.n, dtype=torch.float64, device="cuda")
+    print("case many shifts fused 1 vs 3, 32^3 M=1024", flush=True)
+    out = []
+    for fused in (1, 3):
+        pl = Plan(A, M, make_params(tol=1e-10, graph=1, fused=fused))
+        k, x, _ = pl.solve(b, sigma, M // 2 - 1)
+        rep, stop, sw = pl.report()
+        out.append((k, x.cpu().numpy(), stop, sw, rep["stop_count"]))
+        pl.close()
+    (k1, x1, s1, w1, c1), (k3, x3, s3, w3, c3) = out
+    assert k1 == k3 and w1 == w3 and c1 == c3 == M
+    assert np.array_equal(s1, s3) and np.array_equal(x1, x3)
