@@ -88,7 +88,9 @@ typedef struct hbicg_params {
                             formed, the window decision after), each handing its
                             values to the launch's other workgroups, and the combined
                             window coefficients run in the shift kernel's next
-                            workgroups; 3: the same with the scalar stage in a launch
+                            workgroups (with too many shifts for its tiles to stay
+                            resident beside them, about 256 and more, the stage runs
+                            as in 3); 3: the same with the scalar stage in a launch
                             of its own (five launches); 2: also alpha~ on its own
                             (six); 0: eight launches.  Same results bit for bit; the
                             batched solver always uses eight, exact order seven (no
